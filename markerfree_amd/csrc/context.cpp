@@ -1,0 +1,118 @@
+#include "context.h"
+
+#include "fft.h"
+
+namespace mf {
+
+void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s at %s:%d", (int)e,
+           hipGetErrorString(e), what, file, line);
+  mf_status code = (e == hipErrorOutOfMemory) ? MF_ENOMEM : MF_EDEVICE;
+  throw Error(code, buf);
+}
+
+hipEvent_t* KernelTimer::begin(const char* name, hipStream_t s) {
+  if (!enabled_) return nullptr;
+  Rec* r = new Rec;
+  r->name = name;
+  MF_HIP(hipEventCreate(&r->ev[0]));
+  MF_HIP(hipEventCreate(&r->ev[1]));
+  MF_HIP(hipEventRecord(r->ev[0], s));
+  std::lock_guard<std::mutex> g(mu_);
+  pending_.push_back(r);
+  return r->ev;
+}
+
+void KernelTimer::end(hipEvent_t* ev, hipStream_t s) {
+  if (!ev) return;
+  (void)hipEventRecord(ev[1], s);
+}
+
+void KernelTimer::drain() {
+  for (Rec* r : pending_) {
+    (void)hipEventSynchronize(r->ev[1]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r->ev[0], r->ev[1]);
+    auto& d = done_[r->name];
+    d.first += ms;
+    d.second += 1;
+    (void)hipEventDestroy(r->ev[0]);
+    (void)hipEventDestroy(r->ev[1]);
+    delete r;
+  }
+  pending_.clear();
+}
+
+void KernelTimer::reset() {
+  std::lock_guard<std::mutex> g(mu_);
+  drain();
+  done_.clear();
+}
+
+void KernelTimer::query(const std::string& name, double* total_ms,
+                        long long* n) {
+  std::lock_guard<std::mutex> g(mu_);
+  drain();
+  auto it = done_.find(name);
+  *total_ms = it == done_.end() ? 0.0 : it->second.first;
+  *n = it == done_.end() ? 0 : it->second.second;
+}
+
+void* dev_alloc(mf_ctx* ctx, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->budget && ctx->in_use + bytes > ctx->budget)
+      throw Error(MF_ENOMEM, "HBM budget of the context exceeded");
+    ctx->in_use += bytes;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->in_use -= bytes;
+    throw_hip(e, "hipMalloc", __FILE__, __LINE__);
+  }
+  return p;
+}
+
+void dev_free(mf_ctx* ctx, void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  if (bytes == 0) bytes = 256;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->in_use -= bytes;
+}
+
+DevBuf::~DevBuf() { release(); }
+
+void* DevBuf::ensure(mf_ctx* ctx, size_t bytes) {
+  if (p_ && n_ >= bytes) return p_;
+  release();
+  ctx_ = ctx;
+  p_ = dev_alloc(ctx, bytes);
+  n_ = bytes;
+  return p_;
+}
+
+void DevBuf::release() {
+  if (p_) {
+    if (ctx_) (void)hipStreamSynchronize(ctx_->stream);
+    dev_free(ctx_, p_, n_);
+  }
+  p_ = nullptr;
+  n_ = 0;
+}
+
+}  // namespace mf
+
+mf_ctx::~mf_ctx() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  slab_ws.release();
+  fft.reset();
+  for (auto& kv : user_allocs) (void)hipFree(kv.first);
+  user_allocs.clear();
+  timer.reset();
+  if (stream) (void)hipStreamDestroy(stream);
+}

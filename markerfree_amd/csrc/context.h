@@ -1,0 +1,85 @@
+// Library context: device, stream, HBM accounting, kernel timing, workspaces.
+#pragma once
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+struct mf_ctx;
+
+namespace mf {
+
+class FftCache;
+
+// Sums HIP-event-timed durations of named kernels (bench/roofline support).
+class KernelTimer {
+ public:
+  void enable(bool on) { enabled_ = on; }
+  bool enabled() const { return enabled_; }
+  // Returns an event pair to record around one launch, or null when off.
+  hipEvent_t* begin(const char* name, hipStream_t s);
+  void end(hipEvent_t* ev, hipStream_t s);
+  void reset();
+  void query(const std::string& name, double* total_ms, long long* n);
+
+ private:
+  struct Rec {
+    std::string name;
+    hipEvent_t ev[2];
+  };
+  bool enabled_ = false;
+  std::vector<Rec*> pending_;
+  std::map<std::string, std::pair<double, long long>> done_;
+  std::mutex mu_;
+  void drain();
+};
+
+// Scope helper: times one launch when timing is enabled.
+struct TimedLaunch {
+  KernelTimer& t;
+  hipStream_t s;
+  hipEvent_t* ev;
+  TimedLaunch(KernelTimer& t_, const char* name, hipStream_t s_)
+      : t(t_), s(s_), ev(t_.begin(name, s_)) {}
+  ~TimedLaunch() { t.end(ev, s); }
+};
+
+// A grow-only device buffer owned by the context or a series.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void* ensure(mf_ctx* ctx, size_t bytes);
+  void release();
+  void* get() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  mf_ctx* ctx_ = nullptr;
+  void* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+void* dev_alloc(mf_ctx* ctx, size_t bytes);
+void dev_free(mf_ctx* ctx, void* p, size_t bytes);
+
+}  // namespace mf
+
+struct mf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;      // bytes the context may hold (0: unlimited)
+  size_t in_use = 0;      // bytes currently allocated through the context
+  std::string last_error;
+  mf::KernelTimer timer;
+  std::map<void*, size_t> user_allocs;  // mf_dev_malloc bookkeeping
+  mf::DevBuf slab_ws;                   // materialised-slab workspace
+  std::unique_ptr<mf::FftCache> fft;
+  std::mutex mu;
+  ~mf_ctx();
+};

@@ -1,0 +1,245 @@
+// Device-side FFT building blocks (LDS Stockham stages, Bluestein, real
+// packing).  Included by the .hip files whose kernels run transforms inside
+// their own workgroups (correlation, R-weighting, common-line spectra).
+#pragma once
+#include "fft.h"
+
+namespace mf {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) {
+  return make_float2(a.x * s, a.y * s);
+}
+// Multiply by -i (forward) or +i (inverse).
+__device__ __forceinline__ float2 cmul_mi(float2 a, bool inv) {
+  return inv ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// One radix-R butterfly of a Stockham stage: reads R inputs src[q*ss],
+// applies twiddles tw[q*tws], DFT_R, writes dst[p*ds].
+__device__ __forceinline__ void fft_butterfly(int R, const float2* src, int ss,
+                                              float2* dst, int ds, int tws,
+                                              const float2* __restrict__ tw,
+                                              int N, bool inv) {
+  const float kS = 0.70710678118654752f;
+  float2 x[8];
+  if (R <= 8 && R != 6 && R != 7) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      if (q < R) {
+        float2 v = src[q * ss];
+        if (q > 0 && tws) {
+          float2 w = tw[q * tws];
+          v = cmul(v, inv ? cconj(w) : w);
+        }
+        x[q] = v;
+      }
+    }
+  }
+  switch (R) {
+    case 2: {
+      dst[0] = cadd(x[0], x[1]);
+      dst[ds] = csub(x[0], x[1]);
+      return;
+    }
+    case 3: {
+      const float c = -0.5f, s = 0.86602540378443865f;
+      float2 t = cadd(x[1], x[2]);
+      float2 u = csub(x[1], x[2]);
+      float2 a = make_float2(x[0].x + c * t.x, x[0].y + c * t.y);
+      float2 b = cmul_mi(cscale(u, s), inv);
+      dst[0] = cadd(x[0], t);
+      dst[ds] = cadd(a, b);
+      dst[2 * ds] = csub(a, b);
+      return;
+    }
+    case 4: {
+      float2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+      float2 c = cadd(x[1], x[3]), d = cmul_mi(csub(x[1], x[3]), inv);
+      dst[0] = cadd(a, c);
+      dst[ds] = cadd(b, d);
+      dst[2 * ds] = csub(a, c);
+      dst[3 * ds] = csub(b, d);
+      return;
+    }
+    case 5: {
+      const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+      const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+      float2 a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+      float2 a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+      float2 r1 = make_float2(x[0].x + c1 * a1.x + c2 * a2.x,
+                              x[0].y + c1 * a1.y + c2 * a2.y);
+      float2 r2 = make_float2(x[0].x + c2 * a1.x + c1 * a2.x,
+                              x[0].y + c2 * a1.y + c1 * a2.y);
+      float2 i1 = cmul_mi(make_float2(s1 * b1.x + s2 * b2.x,
+                                      s1 * b1.y + s2 * b2.y), inv);
+      float2 i2 = cmul_mi(make_float2(s2 * b1.x - s1 * b2.x,
+                                      s2 * b1.y - s1 * b2.y), inv);
+      dst[0] = make_float2(x[0].x + a1.x + a2.x, x[0].y + a1.y + a2.y);
+      dst[ds] = cadd(r1, i1);
+      dst[4 * ds] = csub(r1, i1);
+      dst[2 * ds] = cadd(r2, i2);
+      dst[3 * ds] = csub(r2, i2);
+      return;
+    }
+    case 8: {
+      // Two radix-4 transforms on even/odd inputs, then radix-2 combine.
+      float2 a = cadd(x[0], x[4]), b = csub(x[0], x[4]);
+      float2 c = cadd(x[2], x[6]), d = cmul_mi(csub(x[2], x[6]), inv);
+      float2 e0 = cadd(a, c), e1 = cadd(b, d), e2 = csub(a, c), e3 = csub(b, d);
+      a = cadd(x[1], x[5]);
+      b = csub(x[1], x[5]);
+      c = cadd(x[3], x[7]);
+      d = cmul_mi(csub(x[3], x[7]), inv);
+      float2 o0 = cadd(a, c), o1 = cadd(b, d), o2 = csub(a, c), o3 = csub(b, d);
+      // w8^1 = (s, -s) forward, w8^2 = -i, w8^3 = (-s, -s) forward.
+      float2 w1 = inv ? make_float2(kS, kS) : make_float2(kS, -kS);
+      float2 w3 = inv ? make_float2(-kS, kS) : make_float2(-kS, -kS);
+      o1 = cmul(o1, w1);
+      o2 = cmul_mi(o2, inv);
+      o3 = cmul(o3, w3);
+      dst[0] = cadd(e0, o0);
+      dst[4 * ds] = csub(e0, o0);
+      dst[ds] = cadd(e1, o1);
+      dst[5 * ds] = csub(e1, o1);
+      dst[2 * ds] = cadd(e2, o2);
+      dst[6 * ds] = csub(e2, o2);
+      dst[3 * ds] = cadd(e3, o3);
+      dst[7 * ds] = csub(e3, o3);
+      return;
+    }
+    default: {
+      // Generic odd-prime radix: O(R^2) from LDS; DFT_R twiddles come from
+      // the length-N table, w_R^{pq} = tw[(pq mod R) * N/R].
+      const int step = N / R;
+      for (int pp = 0; pp < R; pp++) {
+        float2 acc = make_float2(0.f, 0.f);
+        for (int q = 0; q < R; q++) {
+          float2 v = src[q * ss];
+          if (q > 0 && tws) {
+            float2 w = tw[q * tws];
+            v = cmul(v, inv ? cconj(w) : w);
+          }
+          float2 wr = tw[((pp * q) % R) * step];
+          acc = cadd(acc, cmul(v, inv ? cconj(wr) : wr));
+        }
+        dst[pp * ds] = acc;
+      }
+      return;
+    }
+  }
+}
+
+// Direct Stockham transform of nl lines of length N (line stride ls) held in
+// x; y is scratch of the same shape.  Returns the buffer holding the result.
+// Caller must __syncthreads() before (data loaded) -- stages sync themselves.
+__device__ __forceinline__ float2* fft_stockham(float2* x, float2* y, int N,
+                                                int ls, int nrad,
+                                                const int* rad,
+                                                const float2* __restrict__ tw,
+                                                int nl, bool inv) {
+  int Ns = 1;
+  for (int s = 0; s < nrad; s++) {
+    const int R = rad[s];
+    const int NR = N / R;
+    const int tstride = N / (Ns * R);
+    const int total = nl * NR;
+    for (int b = threadIdx.x; b < total; b += blockDim.x) {
+      const int l = b / NR, j = b - l * NR;
+      const int k = j % Ns;
+      const float2* src = x + l * ls + j;
+      float2* dst = y + l * ls + (j / Ns) * Ns * R + k;
+      fft_butterfly(R, src, NR, dst, Ns, k * tstride, tw, N, inv);
+    }
+    __syncthreads();
+    float2* t = x;
+    x = y;
+    y = t;
+    Ns *= R;
+  }
+  return x;
+}
+
+// Line stride in LDS for a plan.
+__host__ __device__ __forceinline__ int fft_lds_len(const FftDevPlan& P) {
+  return P.blue ? P.m : P.n;
+}
+
+// Complex transform of nl lines (length P.n, stride fft_lds_len(P)) held in
+// buf; tmp is scratch of the same size.  Input in the first n entries of each
+// line; returns the buffer holding the n outputs.  Syncs internally; the
+// caller must sync after loading.
+__device__ __forceinline__ float2* fft_lines(float2* buf, float2* tmp,
+                                             const FftDevPlan& P, int nl,
+                                             bool inv) {
+  if (!P.blue)
+    return fft_stockham(buf, tmp, P.n, P.n, P.nrad, P.rad, P.tw, nl, inv);
+  const int n = P.n, m = P.m;
+  for (int i = threadIdx.x; i < nl * m; i += blockDim.x) {
+    const int l = i / m, k = i - l * m;
+    float2 v = make_float2(0.f, 0.f);
+    if (k < n) {
+      float2 c = P.chirp[k];
+      v = cmul(buf[l * m + k], inv ? cconj(c) : c);
+    }
+    buf[l * m + k] = v;
+  }
+  __syncthreads();
+  float2* r = fft_stockham(buf, tmp, m, m, P.nrad, P.rad, P.tw, nl, false);
+  float2* o = (r == buf) ? tmp : buf;
+  const float2* f = inv ? P.bi : P.bf;
+  for (int i = threadIdx.x; i < nl * m; i += blockDim.x) {
+    const int k = i % m;
+    r[i] = cmul(r[i], f[k]);
+  }
+  __syncthreads();
+  r = fft_stockham(r, o, m, m, P.nrad, P.rad, P.tw, nl, true);
+  for (int i = threadIdx.x; i < nl * m; i += blockDim.x) {
+    const int l = i / m, k = i - l * m;
+    if (k < n) {
+      float2 c = P.chirp[k];
+      r[l * m + k] = cmul(r[l * m + k], inv ? cconj(c) : c);
+    }
+  }
+  __syncthreads();
+  return r;
+}
+
+// Real-to-complex post-processing of one line: Z (N complex, in LDS) holds
+// the length-N transform of z_n = x_2n + i x_2n+1; writes X_k, k = 0..N, of
+// the length-2N real input, times 'scale', through out(k, X).
+template <typename Out>
+__device__ __forceinline__ void r2c_unpack(const float2* Z, int N,
+                                           const float2* __restrict__ half,
+                                           float scale, int k, Out out) {
+  const float2 zk = Z[k == N ? 0 : k];
+  const float2 zn = cconj(Z[(N - k) % N]);
+  const float2 e = cscale(cadd(zk, zn), 0.5f);
+  const float2 o = cscale(csub(zk, zn), 0.5f);
+  // X_k = E + (-i) w_k O,  w_k = exp(-i pi k / N)
+  const float2 t = cmul_mi(cmul(half[k], o), false);
+  out(k, cscale(cadd(e, t), scale));
+}
+
+// Complex-to-real pre-processing: given X_k (k = 0..N) returns Z'_k (k < N)
+// whose unnormalised inverse length-N transform holds x_2m in .x and x_2m+1
+// in .y (cufftExecC2R convention: no 1/n).
+__device__ __forceinline__ float2 c2r_pack(float2 xk, float2 xnk, float2 hk) {
+  const float2 c = cconj(xnk);
+  const float2 a = cadd(xk, c);
+  const float2 b = csub(xk, c);
+  // + i * conj(w_k) * b
+  const float2 t = cmul(cconj(hk), b);
+  return make_float2(a.x - t.y, a.y + t.x);
+}
+
+}  // namespace mf

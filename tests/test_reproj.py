@@ -1,0 +1,258 @@
+"""Reprojection for projection matching (CalcReprojFBP, CalcReproj.cu).
+
+CPU tests pin the oracle with properties the geometry fixes; GPU tests check
+the HIP kernels against the FP64 oracle: identical invalid masks and
+<= 1e-5 relative L2 on the valid reprojected pixels (the tolerance stated for
+reprojected images).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+INVALID = -1e30
+
+
+def tilt_series(nz=21, step=3.0, start=None):
+    start = -(nz // 2) * step if start is None else start
+    return (start + step * np.arange(nz)).astype(np.float32)
+
+
+# ---------------------------------------------------------------- oracle pins
+
+def test_find_proj_range_hand_cases(orc):
+    t = tilt_series(21, 3.0)              # -30 .. 30, zero at index 10
+    skip = np.ones(21, np.uint8)
+    skip[10] = 0
+    # First target after the zero tilt: only the zero tilt is available.
+    assert orc.find_proj_range(t, skip, 11, 10) == (10, 10)
+    skip[11] = 0
+    assert orc.find_proj_range(t, skip, 9, 10) == (10, 11)
+    # Everything processed except the target: +-20.5 deg window at 3 deg steps
+    # keeps 6 neighbours on each side; NPROJ=10 caps the count, trimming the
+    # side away from zero tilt for negative targets and towards it otherwise.
+    skip[:] = 0
+    skip[4] = 1
+    assert orc.find_proj_range(t, skip, 4, 10) == (0, 9)
+    skip[:] = 0
+    skip[16] = 1
+    assert orc.find_proj_range(t, skip, 16, 10) == (11, 20)
+    # Stretch limit cos(a_i)/cos(a_t) <= 1.2 at a 60 degree target.
+    t2 = tilt_series(41, 3.0)             # -60 .. 60
+    skip = np.zeros(41, np.uint8)
+    skip[40] = 1
+    lo, hi = orc.find_proj_range(t2, skip, 40, 10)
+    assert (lo, hi) == (38, 39)           # a_i >= 53.13 deg: 54 and 57
+    assert all(np.cos(np.deg2rad(t2[i])) / 0.5 <= 1.2 for i in range(lo, hi + 1))
+    # Nothing qualifies: empty range.
+    skip[:] = 1
+    assert orc.find_proj_range(t2, skip, 20, 10) == (-1, -1)
+
+
+def test_find_proj_range_matches_product(mf, orc):
+    rng = np.random.default_rng(5)
+    t = tilt_series(61, 2.0)
+    for _ in range(200):
+        skip = (rng.random(61) < 0.4).astype(np.uint8)
+        target = int(rng.integers(0, 61))
+        nproj = int(rng.integers(1, 14))
+        assert mf.find_proj_range(t, skip, target, nproj) == \
+            orc.find_proj_range(t, skip, target, nproj)
+
+
+def test_zero_tilt_identity(orc):
+    """One reference at 0 deg, target at 0 deg: RP(BP(p)) == p exactly,
+    except the last detector column: its voxels have rho = bx - 0.5, above
+    the reference's bound rho <= bx - 1 (CalcReproj.cu:124), so the column is
+    never backprojected and reprojects as invalid."""
+    bx, by, tb = 32, 4, 12
+    rng = np.random.default_rng(0)
+    stack = rng.random((3, by, bx), dtype=np.float32) + 0.5
+    tilts = np.array([0.0, 0.0, 40.0], np.float32)
+    skip = np.array([0, 1, 1], np.uint8)
+    rp = orc.reproject(stack, tb, tilts, skip, target=1, nproj=10)
+    np.testing.assert_array_equal(rp[:, :-1], stack[0][:, :-1])
+    assert np.all(rp[:, -1] == np.float32(INVALID))
+
+
+def test_linearity(orc):
+    """BP (weighted sum / sum of weights) and RP (mean) are linear in the
+    projection values: f(a*p + q) = a*f(p) + f(q) with unchanged masks."""
+    rng = np.random.default_rng(1)
+    bx, by, tb = 48, 5, 16
+    tilts = tilt_series(11, 4.0)
+    p = rng.random((11, by, bx), dtype=np.float32)
+    q = rng.random((11, by, bx), dtype=np.float32)
+    skip = np.zeros(11, np.uint8)
+    skip[8] = 1
+    rp = orc.reproject(p, tb, tilts, skip, 8, 6)
+    rq = orc.reproject(q, tb, tilts, skip, 8, 6)
+    r = orc.reproject(np.float32(3.0) * p + q, tb, tilts, skip, 8, 6)
+    valid = rp > -1e10
+    assert valid.sum() > 0.5 * rp.size
+    np.testing.assert_array_equal(valid, r > -1e10)
+    np.testing.assert_allclose(r[valid], 3.0 * rp[valid] + rq[valid],
+                               rtol=1e-5, atol=1e-6)
+
+
+def test_weights_closed_form(orc):
+    """Constant rows p_i = v_i: every fully covered voxel is
+    sum(v_i cos a_i w_i) / sum(w_i) with w_i = cos((a_t - a_i) * 0.01745)
+    and the 0.01745 degree factor of the backprojector (quirk Q12), so the
+    central reprojected pixels equal that value."""
+    bx, by, tb = 96, 2, 16
+    tilts = tilt_series(11, 4.0)           # -20 .. 20
+    vals = np.linspace(1.0, 2.0, 11).astype(np.float32)
+    stack = np.broadcast_to(vals[:, None, None], (11, by, bx)).copy()
+    skip = np.zeros(11, np.uint8)
+    target = 9
+    skip[target] = 1
+    lo, hi = orc.find_proj_range(tilts, skip, target, 10)
+    a = tilts[lo:hi + 1].astype(np.float64)
+    w = np.cos((tilts[target] - a) * 0.01745)
+    c = np.cos(a * 0.01745)
+    expect = np.sum(vals[lo:hi + 1] * c * w) / np.sum(w)
+    rp = orc.reproject(stack, tb, tilts, skip, target, 10)
+    centre = rp[:, bx // 2 - 8: bx // 2 + 8]
+    np.testing.assert_allclose(centre, expect, rtol=1e-6)
+
+
+def test_point_source_geometry(orc):
+    """A point at (x0, z0) projects to rho_i = x0 cos a_i + z0 sin a_i + bx/2;
+    backprojecting the neighbours and reprojecting at the target puts the
+    peak at x0 cos a_t + z0 sin a_t + bx/2 (catches sign/axis mistakes)."""
+    # A thin slab keeps the backprojected streaks from overlapping the target
+    # ray far from the point (otherwise the response is a plateau); a flipped
+    # sin term would move the peak by 2 z0 sin(30) = 6 px.
+    bx, by, tb = 128, 1, 16
+    tilts = tilt_series(13, 5.0)           # -30 .. 30
+    x0, z0 = 17.0, 6.0
+    stack = np.zeros((13, by, bx), np.float32)
+    for i, a in enumerate(tilts):
+        r = x0 * np.cos(np.deg2rad(a)) + z0 * np.sin(np.deg2rad(a)) + bx / 2
+        stack[i, 0, int(r)] = 1.0
+    skip = np.zeros(13, np.uint8)
+    target = 12                            # +30 deg
+    skip[target] = 1
+    rp = orc.reproject(stack, tb, tilts, skip, target, 10)
+    at = np.deg2rad(tilts[target])
+    expect = x0 * np.cos(at) + z0 * np.sin(at) + bx / 2
+    # Detector pixel u covers [u, u+1); spikes were placed at int(rho), so
+    # the response centre sits near u + 1 on average.
+    assert abs(int(np.argmax(rp[0])) + 1.0 - expect) <= 1.5
+
+
+def test_invalid_band_matches_sample_count(orc):
+    """Rays with fewer than 0.8 L in-slab samples are invalid (-1e30)."""
+    bx, by, tb = 64, 2, 40
+    tilts = tilt_series(21, 3.0)
+    stack = np.ones((21, by, bx), np.float32)
+    skip = np.zeros(21, np.uint8)
+    target = 20                            # +30 deg
+    skip[target] = 1
+    rp = orc.reproject(stack, tb, tilts, skip, target, 10)
+    valid = rp > -1e10
+    assert 0 < valid.sum() < rp.size
+    # Validity is a property of the geometry only: it must agree with the
+    # closed-form count of in-slab samples of each ray (every in-slab voxel
+    # is covered at this size), and be symmetric for the mirrored target.
+    c = np.cos(np.deg2rad(np.float64(tilts[target])))
+    L = int(np.float32(tb) / np.float32(c) + np.float32(0.5))
+    assert valid[0].sum() == valid[1].sum()
+    n_valid_rays = valid[0].sum()
+    tilts_m = -tilts[::-1].copy()
+    skip_m = skip[::-1].copy()
+    rp_m = orc.reproject(stack, tb, tilts_m, skip_m, 20 - target, 10)
+    assert (rp_m[0] > -1e10).sum() == n_valid_rays
+    assert L >= tb
+
+
+# --------------------------------------------------------------- GPU parity
+
+def _series(bx, by, nz, step, seed):
+    rng = np.random.default_rng(seed)
+    tilts = tilt_series(nz, step)
+    stack = (rng.standard_normal((nz, by, bx)) * 0.3 + 1.0).astype(np.float32)
+    return stack, tilts
+
+
+def _check_target(ctx, orc, stack, tilts, skip, target, nproj, tb, mode):
+    nz, by, bx = stack.shape
+    ref = orc.reproject(stack, tb, tilts, skip, target, nproj, acc64=True)
+    d_stack = ctx.array(stack)
+    d_out = ctx.empty((by, bx))
+    ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, nproj, d_out,
+                  mode=mode)
+    got = d_out.download()
+    d_stack.free()
+    d_out.free()
+    vr, vg = ref > -1e10, got > -1e10
+    np.testing.assert_array_equal(vr, vg)
+    assert rel_l2(got, ref, vr) <= 1e-5
+    return got
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("bx,by,nz,step,tb", [
+    (128, 16, 21, 3.0, 40),
+    (256, 8, 41, 3.0, 100),   # small-series binned geometry, reduced rows
+    (200, 6, 61, 2.0, 64),    # non-multiple-of-256 slab width
+])
+def test_gpu_reproject_parity(ctx, orc, mode, bx, by, nz, step, tb):
+    stack, tilts = _series(bx, by, nz, step, seed=bx + nz)
+    z0 = int(np.argmin(np.abs(tilts)))
+    rng = np.random.default_rng(7)
+    for target in [z0 + 1, z0 - 3, nz - 1, 0]:
+        skip = (rng.random(nz) < 0.3).astype(np.uint8)
+        skip[target] = 1
+        for nproj in (1, 10):
+            _check_target(ctx, orc, stack, tilts, skip, target, nproj, tb, mode)
+
+
+@pytest.mark.gpu
+def test_gpu_reproject_invalid_samples(ctx, orc):
+    """Invalid (-1e30) corrected pixels are skipped exactly as the reference
+    does (mDIntProj, CalcReproj.cu:17-24)."""
+    stack, tilts = _series(96, 4, 21, 3.0, seed=3)
+    stack[:, :, 10:14] = INVALID
+    stack[4, 1, :] = INVALID
+    skip = np.zeros(21, np.uint8)
+    skip[15] = 1
+    for mode in (0, 1):
+        _check_target(ctx, orc, stack, tilts, skip, 15, 10, 30, mode)
+
+
+@pytest.mark.gpu
+def test_gpu_reproject_empty_range(ctx, orc):
+    stack, tilts = _series(64, 3, 11, 3.0, seed=4)
+    skip = np.ones(11, np.uint8)
+    d_stack = ctx.array(stack)
+    d_out = ctx.empty((3, 64))
+    for mode in (0, 1):
+        ctx.reproject(d_stack, 64, 3, 11, 20, tilts, skip, 5, 10, d_out,
+                      mode=mode)
+        assert np.all(d_out.download() == np.float32(INVALID))
+    d_stack.free()
+    d_out.free()
+
+
+@pytest.mark.gpu
+def test_gpu_slab_matches_oracle(ctx, orc):
+    """The materialised slab itself (mGBackProj output) against the oracle."""
+    stack, tilts = _series(64, 4, 21, 3.0, seed=9)
+    skip = np.zeros(21, np.uint8)
+    skip[18] = 1
+    tb = 24
+    ref, slab = orc.reproject(stack, tb, tilts, skip, 18, 10, want_slab=True)
+    d_stack = ctx.array(stack)
+    d_out = ctx.empty((4, 64))
+    d_slab = ctx.empty((4, tb, 128))
+    ctx.reproject(d_stack, 64, 4, 21, tb, tilts, skip, 18, 10, d_out, mode=0,
+                  d_slab=d_slab)
+    got = d_slab.download()
+    vr = slab > -1e10
+    np.testing.assert_array_equal(vr, got > -1e10)
+    assert rel_l2(got, slab, vr) <= 1e-6
+    for d in (d_stack, d_out, d_slab):
+        d.free()
