@@ -1,0 +1,196 @@
+"""Benchmark: tilt series aligned per hour on one MI355X.
+
+Workload: the single-series configuration -- a 1024x1024x61 tilt series
+(-60..60 deg in 2 deg steps, 3000-blob phantom, axis U(80,95), shifts
+U(-40,40), SNR 0.05, stored as int16 counts), projection-matching thickness
+300, NPROJ 10, aligned end to end by the plain per-series GPU form (the
+reference's Process::DoIt sequence) with the output stack downloaded.  One
+step = one series: preprocessing, 3 coarse sweeps + axis searches, offset
+scan, projection matching (3 ProjAlignOnce rounds + 4 axis searches) and the
+final corrected stack in host memory.  Raw stacks are resident in HBM before
+the timed region starts (each step gets its own series).
+
+Prints one JSON line with the series/hour, the backprojection kernel's
+roofline (algorithmic bytes per launch over its HIP-event-timed duration)
+and a CPU baseline (the oracle on a bounded sample, extrapolated).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pa_sweep_bytes(mf, tilts, bx, by, tb, nproj):
+    """Algorithmic bytes of the backprojection over one projection-matching
+    sweep: per target 4*(2bx*tb*by) slab bytes written + 4*n*bx*by sinogram
+    bytes read, with n from FindProjRange under the sweep's skip pattern."""
+    nz = len(tilts)
+    z0 = int(np.argmin(np.abs(tilts)))
+    skip = np.ones(nz, np.uint8)
+    skip[z0] = 0
+    total, voxels, targets = 0, 0, 0
+    order = []
+    for i in range(1, nz):
+        for t in (z0 + i, z0 - i):
+            if 0 <= t < nz:
+                order.append(t)
+    for t in order:
+        lo, hi = mf.find_proj_range(tilts, skip, t, nproj)
+        n = hi - lo + 1 if lo >= 0 else 0
+        total += 4 * (2 * bx * tb * by) + 4 * n * bx * by
+        voxels += 2 * bx * tb * by
+        targets += 1
+        skip[t] = 0
+    return total, voxels, targets
+
+
+def cpu_baseline(series, pa_targets_per_series, budget_s=20.0):
+    """The oracle (OpenMP C++ restatement) timed on a bounded sample: two
+    projection-matching targets (reprojection + correlation) of the same
+    1024^2 x 61 series at T=300, extrapolated to one series by the number of
+    target evaluations the GPU run performed."""
+    import oracle
+    stack = series.stack.astype(np.float32)
+    nz, by, bx = stack.shape
+    tilts = series.tilts
+    z0 = int(np.argmin(np.abs(tilts)))
+    times = []
+    t_start = time.time()
+    for k in (5, 28):                      # a 10-neighbour and a 3-neighbour target
+        t = z0 + k if z0 + k < nz else nz - 1
+        skip = np.ones(nz, np.uint8)
+        skip[max(0, z0 - k + 1): z0 + k] = 0
+        skip[t] = 1
+        t0 = time.time()
+        rp = oracle.reproject(stack, 300, tilts, skip, t, 10, acc64=False)
+        oracle.pa_measure(rp, stack[t])
+        times.append(time.time() - t0)
+        if time.time() - t_start > budget_s:
+            break
+    per_target = float(np.mean(times))
+    per_series = per_target * pa_targets_per_series
+    return {"value": round(3600.0 / per_series, 3), "unit": "series/hour",
+            "cores": oracle.num_threads(), "kind": "port",
+            "sample": (f"{len(times)} projection-matching targets (reprojection"
+                       " + CC) of the 1024x1024x61 T=300 series on the host, "
+                       f"{per_target:.2f} s/target, extrapolated by the "
+                       f"{pa_targets_per_series} target evaluations of one "
+                       "GPU series (other stages excluded)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["materialized", "fused"],
+                    default="materialized")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the alignment path runs one process "
+              "on one MI355X (multi-GPU sharding is described in DESIGN.md, "
+              "not measured)", file=sys.stderr)
+        sys.exit(2)
+
+    import markerfree_amd as mf
+    from markerfree_amd import synth
+
+    cfg = synth.PRESETS["single"]
+    series = synth.generate(cfg)
+    mode = 0 if args.mode == "materialized" else 1
+    align_z, nproj = 300, 10
+    ctx = mf.Context(0)
+    n = args.warmup + args.steps
+    pool = [mf.Series(ctx, series.stack, series.tilts) for _ in range(n)]
+    opts = mf.default_options(align_z=align_z, nproj=nproj, reproj_mode=mode)
+
+    def step(s):
+        s.run(opts)
+        return s.correct(1)
+
+    for s in pool[:args.warmup]:
+        step(s)
+    ctx.synchronize()
+    ctx.timing(True)
+    ctx.timing_reset()
+    lat = []
+    t0 = time.perf_counter()
+    for s in pool[args.warmup:]:
+        ts = time.perf_counter()
+        step(s)
+        lat.append(time.perf_counter() - ts)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kname = "bp_slab" if mode == 0 else "bprp_fused"
+    k_ms, k_n = ctx.kernel_time(kname)
+    ctx.timing(False)
+
+    s_last = pool[-1]
+    tilts_final = s_last.tilts()
+    bx = by = 1024
+    sweep_bytes, sweep_vox, targets = pa_sweep_bytes(mf, tilts_final, bx, by,
+                                                     align_z, nproj)
+    sweeps = k_n / max(args.steps, 1) / max(targets, 1)
+    avg_ms = k_ms / max(k_n, 1)
+    bytes_per_launch = sweep_bytes / targets
+    vox_per_launch = sweep_vox / targets
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    for s in pool:
+        s.close()
+    ctx.close()
+
+    value = 3600.0 * args.steps / elapsed
+    out = {
+        "metric": "tilt series aligned/hour + p50/p99 latency; backproj "
+                  "GVoxel/s vs HBM roofline",
+        "value": round(value, 2),
+        "unit": "series/hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 2),
+        "latency_ms": {"p50": round(1000 * float(np.percentile(lat, 50)), 2),
+                       "p99": round(1000 * float(np.percentile(lat, 99)), 2)},
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 3000-blob phantom, int16 counts)",
+        "config": {"workload": "single series 1024x1024x61, plain per-series "
+                               "GPU form", "align_z": align_z, "nproj": nproj,
+                   "reproj_mode": args.mode, "global_batch": 1},
+        "roofline": {
+            "kernel": kname,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "avg_launch_ms": round(avg_ms, 4),
+            "gvoxel_per_s": round(vox_per_launch / (avg_ms * 1e-3) / 1e9, 1),
+            "algorithmic_bytes_per_launch": int(bytes_per_launch),
+            "launches": int(k_n),
+            "pa_sweeps_per_series": round(sweeps, 2),
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(series, int(k_n / max(args.steps, 1)))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -104,6 +104,105 @@ mf_status mf_fft_r2c_1d(mf_ctx* ctx, float* d_buf, int n, int nlines,
 mf_status mf_fft_c2r_1d(mf_ctx* ctx, float* d_buf, int n, int nlines,
                         int pitch);
 
+/* ----------------------------------------------------- stage-level kernels */
+
+/* CorrTomoStack::mCorrectProj (method/CorrStack.cu:503-530) on one raw
+ * projection d_raw [ny][nx]: rotate by axis_deg (0 when shift_only) and
+ * shift (nearest pixel), random fill or invalid outside, bin x bin sum
+ * binning, optional R-weighting; d_out receives the unpadded [by][bx] image
+ * with bx = nx/bin/2*2, by = ny/bin/2*2. */
+mf_status mf_correct_proj(mf_ctx* ctx, const float* d_raw, int nx, int ny,
+                          float shift_x, float shift_y, float axis_deg,
+                          int shift_only, int random_fill, int bin,
+                          int rweight, float* d_out);
+
+/* CalcCC::DoIt + getshift (method/CalcCC.cu:237-271, 437-441) for one pair
+ * of bx x by images: shift2 (host) receives the peak offset in binned
+ * pixels. */
+mf_status mf_stretch_cc(mf_ctx* ctx, const float* d_ref, const float* d_img,
+                        int bx, int by, float ref_tilt, float tilt,
+                        float axis_deg, float bfactor, float* shift2);
+
+/* ProjAlign::MeaAlignProj (method/ProjAlign.cpp:321-375): correlation of a
+ * reprojection with a corrected projection; shift2 (host) = ishift. */
+mf_status mf_pa_measure(mf_ctx* ctx, const float* d_reproj,
+                        const float* d_proj, int bx, int by, float* shift2);
+
+/* ------------------------------------------------------------- a series */
+
+/* Flags of the reference command line (main/opts.h:59-84, -g and -p). */
+typedef struct {
+  float offset;      /* -g field 1: tilt-angle offset, 0 = search it */
+  float pitch_angle; /* -g field 2: tilt axis, 0 = search it */
+  float zshift;      /* -g field 3 */
+  int thickness;     /* -g field 4 (parsed, unused by the reference) */
+  int align_z;       /* -g field 5: projection-matching thickness */
+  int out_bin;       /* -g field 6 */
+  int nproj;         /* -p: neighbouring projections per target */
+  int verbose;       /* print the reference's stage trace to stdout */
+  int reproj_mode;   /* 0 materialised slab, 1 fused (default) */
+} mf_options;
+
+/* Defaults of InitOpts (main/opts.h:110-124): AlignZ 600, OutBin 1, NPROJ 10;
+ * reproj_mode 1. */
+void mf_options_default(mf_options* o);
+
+typedef struct mf_series mf_series;
+
+/* Copies a raw stack [nz][ny][nx] (file order, as read from the MRC) and its
+ * tilt angles (file order, the .rawtlt) into HBM.  Replaces
+ * Process::ReadStack/ReadAngles (main/Process.cpp:129-206) minus file I/O. */
+mf_status mf_series_create(mf_ctx* ctx, int nx, int ny, int nz,
+                           const float* stack_zyx, const float* tilts,
+                           mf_series** out);
+/* Destroy every series before the context it was created on. */
+void mf_series_destroy(mf_series* s);
+const char* mf_series_last_error(const mf_series* s);
+mf_status mf_series_set_verbose(mf_series* s, int verbose);
+
+/* PreProcess::SetPositive + MassNormalization (method/Preprocess.cu:103-177);
+ * reverses descending files into ascending order; zeroes the parameters. */
+mf_status mf_preprocess(mf_series* s);
+/* Preprocessed stack [nz][ny][nx] (ascending tilt order) to the host. */
+mf_status mf_get_stack(mf_series* s, float* out);
+/* Current tilts (ascending; including the offset once applied). */
+mf_status mf_get_tilts(mf_series* s, float* tilts);
+mf_status mf_set_tilts(mf_series* s, const float* tilts);
+/* AlignParam (method/Util.h:14-21) in ascending tilt order. */
+mf_status mf_get_params(mf_series* s, double* shift_x, double* shift_y,
+                        float* rotate, float* angle_offset);
+mf_status mf_set_params(mf_series* s, const double* shift_x,
+                        const double* shift_y, const float* rotate,
+                        float angle_offset);
+
+/* Transform::Setup / DoIt (method/Transform.cpp:55-99): one coarse
+ * stretched-CC sweep; maxerr = largest measured pair shift (px). */
+mf_status mf_coarse_setup(mf_series* s);
+mf_status mf_coarse_iter(mf_series* s, float* maxerr);
+/* CalcTIltAxis::DoIt (method/Rotate.cu:292-337): common-line search over
+ * 'num' candidates spanning 'range' degrees; sets every rotate[i]. */
+mf_status mf_tiltaxis(mf_series* s, float range, int num, float* axis);
+/* FindOffset::DoIt (method/FindOffset.cu:113-124): best tilt-angle offset
+ * of the 31-step scan, added to angle_offset. */
+mf_status mf_find_offset(mf_series* s, float* best);
+/* Adds angle_offset to the tilts (main/Process.cpp:266-269). */
+mf_status mf_apply_offset(mf_series* s);
+/* ProjAlign::Setup (method/ProjAlign.cpp:126-203). */
+mf_status mf_projalign_setup(mf_series* s, float zshift, int align_z,
+                             int nproj, int reproj_mode);
+/* ProjAlign::DoIt (method/ProjAlign.cpp:205-257): one projection-matching
+ * sweep over all targets; maxerr = largest measured shift (px). */
+mf_status mf_projalign_iter(mf_series* s, float* maxerr);
+/* Process::ProjAlignOnce (main/Process.cpp:292-319). */
+mf_status mf_projalign_once(mf_series* s);
+/* Process::CorrectStack (main/Process.cpp:329-351): the aligned stack
+ * [nz][by][bx] at OutBin 'out_bin' to the host (out may be NULL to query
+ * bx, by). */
+mf_status mf_correct(mf_series* s, int out_bin, float* out, int* bx, int* by);
+/* Process::DoIt (main/Process.cpp:34-92) minus file I/O: preprocess, coarse
+ * alignment, axis and offset searches, projection matching. */
+mf_status mf_run(mf_series* s, const mf_options* o);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,11 +140,14 @@ class Context:
     """mf_ctx: one device, one stream, an HBM budget and kernel timers."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 0):
+        import weakref
         h = _vp()
         st = _lib.mf_ctx_create(device, hbm_budget, C.byref(h))
         if st:
             raise MFError(st, "mf_ctx_create failed (no usable HIP device?)")
         self.h = h.value
+        # Series must be destroyed before their context (C-ABI rule).
+        self._series = weakref.WeakSet()
 
     def _check(self, st):
         if st:
@@ -152,6 +155,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for s in list(self._series):
+                s.close()
             _lib.mf_ctx_destroy(self.h)
             self.h = None
 
@@ -204,3 +209,195 @@ class Context:
 
     def fft_c2r_1d(self, d: DeviceArray, n, nlines, pitch):
         self._check(_lib.mf_fft_c2r_1d(self.h, d.ptr, n, nlines, pitch))
+
+
+# ------------------------------------------------------------ stage kernels
+_dp = C.POINTER(C.c_double)
+_sig("mf_correct_proj", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_float,
+     C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _vp)
+_sig("mf_stretch_cc", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.c_float,
+     C.c_float, C.c_float, C.c_float, _fp)
+_sig("mf_pa_measure", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _fp)
+
+
+class Options(C.Structure):
+    """mf_options: the reference's -g / -p flags (main/opts.h:59-84)."""
+    _fields_ = [("offset", C.c_float), ("pitch_angle", C.c_float),
+                ("zshift", C.c_float), ("thickness", C.c_int),
+                ("align_z", C.c_int), ("out_bin", C.c_int),
+                ("nproj", C.c_int), ("verbose", C.c_int),
+                ("reproj_mode", C.c_int)]
+
+
+_sig("mf_options_default", None, C.POINTER(Options))
+_sig("mf_series_create", C.c_int, _vp, C.c_int, C.c_int, C.c_int, _fp, _fp,
+     C.POINTER(_vp))
+_sig("mf_series_destroy", None, _vp)
+_sig("mf_series_last_error", C.c_char_p, _vp)
+_sig("mf_series_set_verbose", C.c_int, _vp, C.c_int)
+_sig("mf_preprocess", C.c_int, _vp)
+_sig("mf_get_stack", C.c_int, _vp, _fp)
+_sig("mf_get_tilts", C.c_int, _vp, _fp)
+_sig("mf_set_tilts", C.c_int, _vp, _fp)
+_sig("mf_get_params", C.c_int, _vp, _dp, _dp, _fp, _fp)
+_sig("mf_set_params", C.c_int, _vp, _dp, _dp, _fp, C.c_float)
+_sig("mf_coarse_setup", C.c_int, _vp)
+_sig("mf_coarse_iter", C.c_int, _vp, _fp)
+_sig("mf_tiltaxis", C.c_int, _vp, C.c_float, C.c_int, _fp)
+_sig("mf_find_offset", C.c_int, _vp, _fp)
+_sig("mf_apply_offset", C.c_int, _vp)
+_sig("mf_projalign_setup", C.c_int, _vp, C.c_float, C.c_int, C.c_int, C.c_int)
+_sig("mf_projalign_iter", C.c_int, _vp, _fp)
+_sig("mf_projalign_once", C.c_int, _vp)
+_sig("mf_correct", C.c_int, _vp, C.c_int, _fp, _ip, _ip)
+_sig("mf_run", C.c_int, _vp, C.POINTER(Options))
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    _lib.mf_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _ctx_correct_proj(self, d_raw, nx, ny, shift_x, shift_y, axis_deg,
+                      shift_only, random_fill, binning, rweight, d_out):
+    self._check(_lib.mf_correct_proj(self.h, d_raw.ptr, nx, ny, shift_x,
+                                     shift_y, axis_deg, int(shift_only),
+                                     int(random_fill), binning, int(rweight),
+                                     d_out.ptr))
+
+
+def _ctx_stretch_cc(self, d_ref, d_img, bx, by, ref_tilt, tilt, axis_deg,
+                    bfactor=400.0):
+    out = np.zeros(2, np.float32)
+    self._check(_lib.mf_stretch_cc(self.h, d_ref.ptr, d_img.ptr, bx, by,
+                                   ref_tilt, tilt, axis_deg, bfactor,
+                                   f32p(out)))
+    return float(out[0]), float(out[1])
+
+
+def _ctx_pa_measure(self, d_reproj, d_proj, bx, by):
+    out = np.zeros(2, np.float32)
+    self._check(_lib.mf_pa_measure(self.h, d_reproj.ptr, d_proj.ptr, bx, by,
+                                   f32p(out)))
+    return float(out[0]), float(out[1])
+
+
+Context.correct_proj = _ctx_correct_proj
+Context.stretch_cc = _ctx_stretch_cc
+Context.pa_measure = _ctx_pa_measure
+
+
+class Series:
+    """mf_series: one tilt series resident in HBM (Process counterpart)."""
+
+    def __init__(self, ctx: Context, stack, tilts_file_order):
+        stack = np.ascontiguousarray(stack, np.float32)
+        self.nz, self.ny, self.nx = stack.shape
+        t = np.ascontiguousarray(tilts_file_order, np.float32)
+        h = _vp()
+        ctx._check(_lib.mf_series_create(ctx.h, self.nx, self.ny, self.nz,
+                                         f32p(stack), f32p(t), C.byref(h)))
+        self.ctx = ctx
+        self.h = h.value
+        ctx._series.add(self)
+
+    def _check(self, st):
+        if st:
+            raise MFError(st, _lib.mf_series_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            _lib.mf_series_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def verbose(self, on=True):
+        self._check(_lib.mf_series_set_verbose(self.h, int(on)))
+
+    def preprocess(self):
+        self._check(_lib.mf_preprocess(self.h))
+
+    def stack(self):
+        out = np.empty((self.nz, self.ny, self.nx), np.float32)
+        self._check(_lib.mf_get_stack(self.h, f32p(out)))
+        return out
+
+    def tilts(self):
+        out = np.empty(self.nz, np.float32)
+        self._check(_lib.mf_get_tilts(self.h, f32p(out)))
+        return out
+
+    def set_tilts(self, t):
+        t = np.ascontiguousarray(t, np.float32)
+        self._check(_lib.mf_set_tilts(self.h, f32p(t)))
+
+    def params(self):
+        sx = np.empty(self.nz, np.float64)
+        sy = np.empty(self.nz, np.float64)
+        rot = np.empty(self.nz, np.float32)
+        off = C.c_float()
+        self._check(_lib.mf_get_params(self.h, sx.ctypes.data_as(_dp),
+                                       sy.ctypes.data_as(_dp), f32p(rot),
+                                       C.byref(off)))
+        return sx, sy, rot, float(off.value)
+
+    def set_params(self, sx, sy, rot, offset=0.0):
+        sx = np.ascontiguousarray(sx, np.float64)
+        sy = np.ascontiguousarray(sy, np.float64)
+        rot = np.ascontiguousarray(rot, np.float32)
+        self._check(_lib.mf_set_params(self.h, sx.ctypes.data_as(_dp),
+                                       sy.ctypes.data_as(_dp), f32p(rot),
+                                       offset))
+
+    def coarse_setup(self):
+        self._check(_lib.mf_coarse_setup(self.h))
+
+    def coarse_iter(self):
+        e = C.c_float()
+        self._check(_lib.mf_coarse_iter(self.h, C.byref(e)))
+        return float(e.value)
+
+    def tilt_axis(self, arange, num=100):
+        a = C.c_float()
+        self._check(_lib.mf_tiltaxis(self.h, arange, num, C.byref(a)))
+        return float(a.value)
+
+    def find_offset(self):
+        b = C.c_float()
+        self._check(_lib.mf_find_offset(self.h, C.byref(b)))
+        return float(b.value)
+
+    def apply_offset(self):
+        self._check(_lib.mf_apply_offset(self.h))
+
+    def projalign_setup(self, zshift, align_z, nproj, reproj_mode=1):
+        self._check(_lib.mf_projalign_setup(self.h, zshift, align_z, nproj,
+                                            reproj_mode))
+
+    def projalign_iter(self):
+        e = C.c_float()
+        self._check(_lib.mf_projalign_iter(self.h, C.byref(e)))
+        return float(e.value)
+
+    def projalign_once(self):
+        self._check(_lib.mf_projalign_once(self.h))
+
+    def correct(self, out_bin=1):
+        bx, by = C.c_int(), C.c_int()
+        self._check(_lib.mf_correct(self.h, out_bin, None, C.byref(bx),
+                                    C.byref(by)))
+        out = np.empty((self.nz, by.value, bx.value), np.float32)
+        self._check(_lib.mf_correct(self.h, out_bin, f32p(out), None, None))
+        return out
+
+    def run(self, options: Options = None, **kw):
+        o = options if options is not None else default_options(**kw)
+        self._check(_lib.mf_run(self.h, C.byref(o)))

@@ -1,10 +1,14 @@
 // extern "C" entry points: argument checks, exception-to-status translation.
+#include <cmath>
 #include <cstring>
 #include <new>
 
 #include "bprp.h"
+#include "cc.h"
 #include "context.h"
 #include "fft.h"
+#include "series.h"
+#include "stages.h"
 
 using namespace mf;
 
@@ -236,6 +240,279 @@ mf_status mf_fft_c2r_1d(mf_ctx* ctx, float* d_buf, int n, int nlines,
                "bad 1-D FFT arguments");
     DeviceScope d(ctx);
     fft_c2r_rows(ctx, ctx->stream, d_buf, n, nlines, pitch, 1.0f);
+  });
+}
+
+mf_status mf_correct_proj(mf_ctx* ctx, const float* d_raw, int nx, int ny,
+                          float shift_x, float shift_y, float axis_deg,
+                          int shift_only, int random_fill, int bin,
+                          int rweight, float* d_out) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(d_raw && d_out && nx > 1 && ny > 1 && bin >= 1, "bad arguments");
+    DeviceScope d(ctx);
+    const CorrGeom g = CorrGeom::make(nx, ny, bin);
+    CorrProj p;
+    const float th = shift_only ? 0.0f : (float)(axis_deg * kD2R);
+    p.shift_x = shift_x;
+    p.shift_y = shift_y;
+    p.cos_a = hcos(th);
+    p.sin_a = hsin(th);
+    DevBuf pb, padb;
+    CorrProj* dp = (CorrProj*)pb.ensure(ctx, sizeof(CorrProj));
+    MF_HIP(hipMemcpyAsync(dp, &p, sizeof(p), hipMemcpyHostToDevice, ctx->stream));
+    if (rweight) {
+      const int pitch = pad_x(g.bx);
+      float* pad = (float*)padb.ensure(ctx, (size_t)pitch * g.by * sizeof(float));
+      correct_bin(ctx, ctx->stream, d_raw, g, dp, 0, 1, random_fill != 0, pad,
+                  pitch, (size_t)pitch * g.by);
+      rweight_rows(ctx, ctx->stream, pad, g.bx, g.by, d_out);
+    } else {
+      correct_bin(ctx, ctx->stream, d_raw, g, dp, 0, 1, random_fill != 0,
+                  d_out, g.bx, (size_t)g.bx * g.by);
+    }
+    MF_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+mf_status mf_stretch_cc(mf_ctx* ctx, const float* d_ref, const float* d_img,
+                        int bx, int by, float ref_tilt, float tilt,
+                        float axis_deg, float bfactor, float* shift2) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(d_ref && d_img && shift2 && bx >= 4 && by >= 4 && bx % 2 == 0,
+               "bad arguments");
+    DeviceScope d(ctx);
+    const size_t plane = (size_t)bx * by;
+    DevBuf stk, work, mom, tab, sh;
+    float* s2 = (float*)stk.ensure(ctx, 2 * plane * sizeof(float));
+    MF_HIP(hipMemcpyAsync(s2, d_ref, plane * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    MF_HIP(hipMemcpyAsync(s2 + plane, d_img, plane * 4, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    PairDesc p;
+    p.ref = 0;
+    p.img = 1;
+    double ds = std::cos(kD2R * ref_tilt) / std::cos(kD2R * tilt);
+    stretch_matrix((float)ds, axis_deg, p.m);
+    float* w = (float*)work.ensure(ctx, stretch_cc_work_floats(bx, by, 1) * 4);
+    double* m = (double*)mom.ensure(ctx, 8 * sizeof(double));
+    float* dsh = (float*)sh.ensure(ctx, 2 * sizeof(float));
+    stretch_cc_pairs(ctx, ctx->stream, s2, bx, by, {p}, bfactor, w, m, tab, dsh);
+    MF_HIP(hipMemcpy(shift2, dsh, 2 * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+mf_status mf_pa_measure(mf_ctx* ctx, const float* d_reproj,
+                        const float* d_proj, int bx, int by, float* shift2) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(d_reproj && d_proj && shift2 && bx >= 8 && by >= 8 &&
+                   bx % 2 == 0,
+               "bad arguments");
+    DeviceScope d(ctx);
+    DevBuf work, mom, sh;
+    float* w = (float*)work.ensure(ctx, 2 * (size_t)by * pad_x(bx) * 4);
+    double* m = (double*)mom.ensure(ctx, 8 * sizeof(double));
+    float* dsh = (float*)sh.ensure(ctx, 2 * sizeof(float));
+    pa_measure(ctx, ctx->stream, d_reproj, d_proj, bx, by, w, m, dsh);
+    MF_HIP(hipMemcpyAsync(shift2, dsh, 2 * sizeof(float),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    MF_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+void mf_options_default(mf_options* o) {
+  if (!o) return;
+  o->offset = 0.0f;
+  o->pitch_angle = 0.0f;
+  o->zshift = 0.0f;
+  o->thickness = 0;
+  o->align_z = 600;
+  o->out_bin = 1;
+  o->nproj = 10;
+  o->verbose = 0;
+  o->reproj_mode = 1;
+}
+
+mf_status mf_series_create(mf_ctx* ctx, int nx, int ny, int nz,
+                           const float* stack, const float* tilts,
+                           mf_series** out) {
+  if (!ctx || !out) return MF_EINVAL;
+  *out = nullptr;
+  mf_series* s = nullptr;
+  mf_status st = guard(ctx, [&] {
+    MF_REQUIRE(stack && tilts && nx >= 16 && ny >= 16 && nz >= 2,
+               "series needs at least 2 images of 16x16");
+    DeviceScope d(ctx);
+    s = new mf_series;
+    s->ctx = ctx;
+    s->nx = nx;
+    s->ny = ny;
+    s->nz = nz;
+    s->file_tilts.assign(tilts, tilts + nz);
+    MF_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    const size_t bytes = (size_t)nx * ny * nz * sizeof(float);
+    float* raw = (float*)s->raw.ensure(ctx, bytes);
+    MF_HIP(hipMemcpyAsync(raw, stack, bytes, hipMemcpyHostToDevice, s->st));
+    MF_HIP(hipStreamSynchronize(s->st));
+    s->sx.assign(nz, 0.0);
+    s->sy.assign(nz, 0.0);
+    s->rot.assign(nz, 0.0f);
+  });
+  if (st != MF_OK) {
+    delete s;
+    return st;
+  }
+  *out = s;
+  return MF_OK;
+}
+
+void mf_series_destroy(mf_series* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->ctx->device);
+  delete s;
+}
+
+const char* mf_series_last_error(const mf_series* s) {
+  return s ? s->last_error.c_str() : "null series";
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename F>
+mf_status sguard(mf_series* s, F&& f) {
+  if (!s) return MF_EINVAL;
+  mf_status st = guard(s->ctx, [&] {
+    DeviceScope d(s->ctx);
+    f();
+  });
+  if (st != MF_OK) s->last_error = s->ctx->last_error;
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+mf_status mf_series_set_verbose(mf_series* s, int verbose) {
+  return sguard(s, [&] { s->verbose = verbose; });
+}
+
+mf_status mf_preprocess(mf_series* s) {
+  return sguard(s, [&] {
+    if (!s->preprocessed) series_preprocess(s);
+  });
+}
+
+mf_status mf_get_stack(mf_series* s, float* out) {
+  return sguard(s, [&] {
+    MF_REQUIRE(out, "null output");
+    MF_REQUIRE(s->preprocessed, "preprocess the series first");
+    MF_HIP(hipMemcpyAsync(out, s->stack.get(),
+                          (size_t)s->nx * s->ny * s->nz * sizeof(float),
+                          hipMemcpyDeviceToHost, s->st));
+    MF_HIP(hipStreamSynchronize(s->st));
+  });
+}
+
+mf_status mf_get_tilts(mf_series* s, float* tilts) {
+  return sguard(s, [&] {
+    MF_REQUIRE(tilts, "null output");
+    const auto& t = s->preprocessed ? s->tilts : s->file_tilts;
+    std::copy(t.begin(), t.end(), tilts);
+  });
+}
+
+mf_status mf_set_tilts(mf_series* s, const float* tilts) {
+  return sguard(s, [&] {
+    MF_REQUIRE(tilts, "null input");
+    s->tilts.assign(tilts, tilts + s->nz);
+  });
+}
+
+mf_status mf_get_params(mf_series* s, double* shift_x, double* shift_y,
+                        float* rotate, float* angle_offset) {
+  return sguard(s, [&] {
+    if (shift_x) std::copy(s->sx.begin(), s->sx.end(), shift_x);
+    if (shift_y) std::copy(s->sy.begin(), s->sy.end(), shift_y);
+    if (rotate) std::copy(s->rot.begin(), s->rot.end(), rotate);
+    if (angle_offset) *angle_offset = s->offset;
+  });
+}
+
+mf_status mf_set_params(mf_series* s, const double* shift_x,
+                        const double* shift_y, const float* rotate,
+                        float angle_offset) {
+  return sguard(s, [&] {
+    if (shift_x) s->sx.assign(shift_x, shift_x + s->nz);
+    if (shift_y) s->sy.assign(shift_y, shift_y + s->nz);
+    if (rotate) s->rot.assign(rotate, rotate + s->nz);
+    s->offset = angle_offset;
+  });
+}
+
+mf_status mf_coarse_setup(mf_series* s) {
+  return sguard(s, [&] { series_coarse_setup(s); });
+}
+
+mf_status mf_coarse_iter(mf_series* s, float* maxerr) {
+  return sguard(s, [&] {
+    float e = series_coarse_iter(s);
+    if (maxerr) *maxerr = e;
+  });
+}
+
+mf_status mf_tiltaxis(mf_series* s, float range, int num, float* axis) {
+  return sguard(s, [&] {
+    float a = series_tilt_axis(s, range, num);
+    if (axis) *axis = a;
+  });
+}
+
+mf_status mf_find_offset(mf_series* s, float* best) {
+  return sguard(s, [&] {
+    float b = series_find_offset(s);
+    if (best) *best = b;
+  });
+}
+
+mf_status mf_apply_offset(mf_series* s) {
+  return sguard(s, [&] { series_apply_offset(s); });
+}
+
+mf_status mf_projalign_setup(mf_series* s, float zshift, int align_z,
+                             int nproj, int reproj_mode) {
+  return sguard(s, [&] {
+    MF_REQUIRE(reproj_mode == 0 || reproj_mode == 1, "reproj_mode is 0 or 1");
+    s->reproj_mode = reproj_mode;
+    series_projalign_setup(s, zshift, align_z, nproj);
+  });
+}
+
+mf_status mf_projalign_iter(mf_series* s, float* maxerr) {
+  return sguard(s, [&] {
+    float e = series_projalign_iter(s);
+    if (maxerr) *maxerr = e;
+  });
+}
+
+mf_status mf_projalign_once(mf_series* s) {
+  return sguard(s, [&] { series_projalign_once(s); });
+}
+
+mf_status mf_correct(mf_series* s, int out_bin, float* out, int* bx, int* by) {
+  return sguard(s, [&] { series_correct(s, out_bin, out, bx, by); });
+}
+
+mf_status mf_run(mf_series* s, const mf_options* o) {
+  return sguard(s, [&] {
+    MF_REQUIRE(o, "null options");
+    MF_REQUIRE(o->nproj > 0, "nProj must be > 0");
+    MF_REQUIRE(o->reproj_mode == 0 || o->reproj_mode == 1,
+               "reproj_mode is 0 or 1");
+    series_run(s, o);
   });
 }
 

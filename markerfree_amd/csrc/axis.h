@@ -1,0 +1,46 @@
+// Tilt-axis (CalcTIltAxis, method/Rotate.cu) and tilt-angle offset
+// (FindOffset, method/FindOffset.cu) searches.
+#pragma once
+#include <vector>
+
+#include "context.h"
+
+namespace mf {
+
+// Candidate grid and line length of CLsetup (Rotate.cu:339-374).
+struct AxisSetup {
+  int num = 0, L = 0;
+  std::vector<float> rot;  // candidate angles (degrees)
+  std::vector<float> cs;   // cos, sin of rot * 3.1415926f / 180.0f
+  static AxisSetup make(int nx, int ny, float axis0, float range, int num);
+};
+
+struct AxisResult {
+  float axis = 0.0f;          // best candidate, before the wrap rules
+  std::vector<float> scores;  // per candidate
+  int L = 0;
+};
+
+struct AxisWork {
+  DevBuf lines, reg, small;
+};
+
+struct OffsetWork {
+  DevBuf buf;
+};
+
+// Radon lines of every image for every candidate, their spectra and the
+// leave-one-out line correlation score (Rotate.cu:292-561).  Synchronous.
+AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
+                       int nx, int ny, int nz, const float* tilts,
+                       const double* sx, const double* sy, float axis0,
+                       float range, int num, AxisWork& ws);
+
+// 31-step offset scan over a bin-4 corrected stack [nz][h][w]
+// (FindOffset.cu:157-266).  Returns the best offset; ccs (optional) gets the
+// mean correlation per step.  Synchronous.
+float offset_search(mf_ctx* ctx, hipStream_t st, const float* d_corr, int w,
+                    int h, int nz, const float* tilts, OffsetWork& ws,
+                    std::vector<float>* ccs);
+
+}  // namespace mf

@@ -1,0 +1,43 @@
+// Cross-correlation stages: CalcCC (method/CalcCC.cu) for the stretched
+// coarse alignment and ProjAlign::MeaAlignProj (method/ProjAlign.cpp:243-375)
+// for projection matching.
+#pragma once
+#include <vector>
+
+#include "context.h"
+
+namespace mf {
+
+// One stretched pair: reference image and image indices into an image array,
+// and the inverse stretch matrix of CalcCC::Stretch (CalcCC.cu:318-335).
+struct PairDesc {
+  int ref, img;
+  float m[3];
+};
+
+// Host: the stretch matrix exactly as CalcCC::Stretch forms it.
+void stretch_matrix(float stretch, float axis_deg, float m[3]);
+
+// Masks of GRoundEdge (CalcCC.cu:173-195) as host-built tables: 'coarse'
+// (centre w/2,h/2, size w,h) and 'pa' (centre w/2,h/2, size pad_x(w),h,
+// ProjAlign.cpp:295-312).
+const float* cc_mask(mf_ctx* ctx, int w, int h, bool pa);
+
+// Batched CalcCC::DoIt for the pairs of w x h images from 'stack'
+// (contiguous [n][h][w]).  Workspace: stretch_cc_work_floats() floats and
+// 4*npair doubles of moments.  d_shift receives float2 shifts (binned
+// pixels, before the bin factor); returns after the stream has drained.
+void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
+                      int w, int h, const std::vector<PairDesc>& pairs,
+                      float bfactor, float* d_work, double* d_mom,
+                      DevBuf& tab, float* d_shift);
+
+// MeaAlignProj: d_reproj and d_proj are w x h; writes the float2 ishift.
+// Workspace: 2*h*pad_x(w) floats + 4 doubles.
+void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
+                const float* d_proj, int w, int h, float* d_work,
+                double* d_mom, float* d_ishift);
+
+size_t stretch_cc_work_floats(int w, int h, int npair);
+
+}  // namespace mf

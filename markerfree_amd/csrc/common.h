@@ -36,7 +36,7 @@ constexpr double kD2R = 3.14159265358979323846 / 180.0;
 constexpr float kBPD2R = 0.01745f;
 constexpr float kInvalid = -1e30f;
 
-inline int pad_x(int nx) { return (nx / 2 + 1) * 2; }
+__host__ __device__ inline int pad_x(int nx) { return (nx / 2 + 1) * 2; }
 inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 // Float cos/sin of a float argument evaluated in double on the host; every

@@ -85,6 +85,20 @@ void dev_free(mf_ctx* ctx, void* p, size_t bytes) {
   ctx->in_use -= bytes;
 }
 
+const void* ctx_table(mf_ctx* ctx, const std::string& key,
+                      const std::function<std::vector<float>()>& build) {
+  std::lock_guard<std::mutex> g(ctx->table_mu);
+  auto it = ctx->tables.find(key);
+  if (it != ctx->tables.end()) return it->second->get();
+  const std::vector<float> host = build();
+  const size_t bytes = host.size() * sizeof(float);
+  std::unique_ptr<DevBuf> b(new DevBuf);
+  void* d = b->ensure(ctx, bytes);
+  MF_HIP(hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice));
+  ctx->tables[key] = std::move(b);
+  return d;
+}
+
 DevBuf::~DevBuf() { release(); }
 
 void* DevBuf::ensure(mf_ctx* ctx, size_t bytes) {
@@ -110,6 +124,7 @@ void DevBuf::release() {
 mf_ctx::~mf_ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   slab_ws.release();
+  tables.clear();
   fft.reset();
   for (auto& kv : user_allocs) (void)hipFree(kv.first);
   user_allocs.clear();

@@ -1,5 +1,6 @@
 // Library context: device, stream, HBM accounting, kernel timing, workspaces.
 #pragma once
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -68,6 +69,12 @@ class DevBuf {
 void* dev_alloc(mf_ctx* ctx, size_t bytes);
 void dev_free(mf_ctx* ctx, void* p, size_t bytes);
 
+// Device copy of a constant host table, uploaded once per key and owned by
+// the context (filters, tapers, trigonometric tables).  'build' runs only the
+// first time a key is requested.
+const void* ctx_table(mf_ctx* ctx, const std::string& key,
+                      const std::function<std::vector<float>()>& build);
+
 }  // namespace mf
 
 struct mf_ctx {
@@ -80,6 +87,7 @@ struct mf_ctx {
   std::map<void*, size_t> user_allocs;  // mf_dev_malloc bookkeeping
   mf::DevBuf slab_ws;                   // materialised-slab workspace
   std::unique_ptr<mf::FftCache> fft;
-  std::mutex mu;
+  std::map<std::string, std::unique_ptr<mf::DevBuf>> tables;
+  std::mutex mu, table_mu;
   ~mf_ctx();
 };

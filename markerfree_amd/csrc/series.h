@@ -1,0 +1,64 @@
+// One tilt series resident in HBM and the pipeline that aligns it
+// (the counterpart of Process, main/Process.cpp, and of the stage classes it
+// drives).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "axis.h"
+#include "context.h"
+#include "stages.h"
+
+struct mf_series {
+  mf_ctx* ctx = nullptr;
+  hipStream_t st = nullptr;  // each series owns a stream (batched form)
+  int nx = 0, ny = 0, nz = 0;
+  int verbose = 0;
+  int reproj_mode = 1;       // 0 materialised slab, 1 fused
+  std::vector<float> file_tilts;  // as read (file order)
+  std::vector<float> tilts;       // ascending; + offset once applied
+  // AlignParam (Util.h:14-21), shifts in FP64.
+  std::vector<double> sx, sy;
+  std::vector<float> rot;
+  float offset = 0.0f;
+  bool preprocessed = false;
+  mf::DevBuf raw, stack, tmp;
+  // Transform (coarse alignment) state.
+  bool tr_ready = false;
+  int tr_bin = 1;
+  std::vector<float> tr_angles;
+  std::vector<double> psx, psy;
+  mf::DevBuf tr_corr, tr_work, tr_mom, tr_shift, tr_tab, cproj;
+  // Tilt axis / offset workspaces.
+  mf::AxisWork axw;
+  mf::OffsetWork offw;
+  mf::DevBuf off_corr;
+  // ProjAlign state.
+  bool pa_ready = false;
+  int pa_bin = 1, pa_tb = 0, pa_nproj = 10;
+  float pa_z0 = 0.0f;
+  std::vector<float> pa_angles;
+  mf::CorrGeom pa_geom{};
+  mf::DevBuf pa_corr, pa_pad, pa_reproj, pa_work, pa_mom, pa_dev, pa_slab;
+  std::string last_error;
+  ~mf_series();
+};
+
+namespace mf {
+
+void series_preprocess(mf_series* s);
+void series_coarse_setup(mf_series* s);
+float series_coarse_iter(mf_series* s);
+float series_tilt_axis(mf_series* s, float range, int num);
+float series_find_offset(mf_series* s);
+void series_apply_offset(mf_series* s);
+void series_projalign_setup(mf_series* s, float zshift, int align_z,
+                            int nproj);
+float series_projalign_iter(mf_series* s);
+void series_projalign_once(mf_series* s);
+// Final correction (Process::CorrectStack); out [nz][by][bx] host, or
+// nullptr to query the size only.
+void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by);
+void series_run(mf_series* s, const mf_options* o);
+
+}  // namespace mf

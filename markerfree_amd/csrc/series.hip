@@ -1,0 +1,531 @@
+// Per-series pipeline: the Process::DoIt stage sequence (main/Process.cpp)
+// over HBM-resident data and the device stages.
+//
+// Differences in mechanism from the reference, none in result:
+//  - the preprocessed stack stays in HBM (the reference writes it to buf.mrc
+//    and re-reads one slice per correction, Process.cpp:161-170,
+//    CorrStack.cu:515);
+//  - during a projection-matching sweep the alignment parameters live on the
+//    device (FP64) and each target is measured, applied and re-corrected
+//    without a host round trip; the host reads them back once per sweep;
+//  - per-tilt shifts are accumulated in FP64 (the reference uses FP32).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "bprp.h"
+#include "cc.h"
+#include "series.h"
+
+mf_series::~mf_series() {
+  if (st) (void)hipStreamSynchronize(st);
+  for (mf::DevBuf* b : {&raw, &stack, &tmp, &tr_corr, &tr_work, &tr_mom,
+                        &tr_shift, &tr_tab, &cproj, &off_corr, &pa_corr,
+                        &pa_pad, &pa_reproj, &pa_work, &pa_mom, &pa_dev,
+                        &pa_slab, &axw.lines, &axw.reg, &axw.small, &offw.buf})
+    b->release();
+  if (st) (void)hipStreamDestroy(st);
+}
+
+namespace mf {
+
+namespace {
+
+int zero_index(const std::vector<float>& t) {
+  // GetFrameIdxFromTilt(0) (Util.cpp:8-20): first minimum of |0 - a_i|.
+  int idx = 0;
+  float m = std::fabs(0.0f - t[0]);
+  for (size_t i = 1; i < t.size(); i++) {
+    float d = std::fabs(0.0f - t[i]);
+    if (d >= m) continue;
+    m = d;
+    idx = (int)i;
+  }
+  return idx;
+}
+
+int ref_index(const std::vector<float>& t, int z) {
+  // FindRefIndex (Util.cpp:22-35).
+  const int nz = (int)t.size();
+  int i0 = z - 1, i2 = z + 1;
+  if (i0 < 0) i0 = z;
+  if (i2 >= nz) i2 = z;
+  double a0 = std::fabs((double)t[i0]), a = std::fabs((double)t[z]),
+         a2 = std::fabs((double)t[i2]);
+  if (a0 < a) return i0;
+  if (a2 < a) return i2;
+  return z;
+}
+
+// Correction parameters for every projection; shift_only uses angle 0.
+std::vector<CorrProj> corr_params(const mf_series* s, bool shift_only) {
+  std::vector<CorrProj> p(s->nz);
+  for (int z = 0; z < s->nz; z++) {
+    float th = shift_only ? 0.0f : (float)(s->rot[z] * kD2R);
+    p[z].shift_x = (float)s->sx[z];
+    p[z].shift_y = (float)s->sy[z];
+    p[z].cos_a = hcos(th);
+    p[z].sin_a = hsin(th);
+  }
+  return p;
+}
+
+CorrProj* upload_corr(mf_series* s, const std::vector<CorrProj>& p) {
+  CorrProj* d = (CorrProj*)s->cproj.ensure(s->ctx, p.size() * sizeof(CorrProj));
+  MF_HIP(hipMemcpyAsync(d, p.data(), p.size() * sizeof(CorrProj),
+                        hipMemcpyHostToDevice, s->st));
+  return d;
+}
+
+// Device-side parameter block of a projection-matching sweep.
+struct PaDev {
+  float2* ishift;  // [1]
+  float2* meas;    // [nz] measured shift (raw pixels) per target
+  float* err;      // [nz]
+  float2* ind;     // [nz] Z-induced shift (CalcZInducedShift)
+  float2* rcs;     // [nz] cos, sin of the axis for RotShift
+  double* sx;      // [nz]
+  double* sy;      // [nz]
+};
+
+PaDev pa_layout(mf_series* s) {
+  const int nz = s->nz;
+  const size_t bytes = 8 + (size_t)nz * (8 + 4 + 8 + 8 + 8 + 8) + 64;
+  char* p = (char*)s->pa_dev.ensure(s->ctx, bytes);
+  PaDev d;
+  d.sx = (double*)p;
+  d.sy = d.sx + nz;
+  d.ishift = (float2*)(d.sy + nz);
+  d.meas = d.ishift + 1;
+  d.ind = d.meas + nz;
+  d.rcs = d.ind + nz;
+  d.err = (float*)(d.rcs + nz);
+  return d;
+}
+
+// ClacAlignProj's parameter update (ProjAlign.cpp:276-296) for target t:
+// shift = RotShift(ishift * bin, axis) + Z-induced; param += shift.  The
+// target's correction parameters are refreshed for its re-correction.
+__global__ void k_pa_update(PaDev d, int t, float binx, float biny,
+                            CorrProj* cp) {
+  const float2 ish = d.ishift[0];
+  const float a0 = ish.x * binx, a1 = ish.y * biny;
+  d.meas[t] = make_float2(a0, a1);
+  d.err[t] = (float)sqrt((double)(a0 * a0 + a1 * a1));
+  const float c = d.rcs[t].x, sn = d.rcs[t].y;
+  float r0 = a0 * c - a1 * sn;
+  float r1 = a0 * sn + a1 * c;
+  r0 += d.ind[t].x;
+  r1 += d.ind[t].y;
+  d.sx[t] += (double)r0;
+  d.sy[t] += (double)r1;
+  cp[t].shift_x = (float)d.sx[t];
+  cp[t].shift_y = (float)d.sy[t];
+}
+
+void rot_shift(const float* in, float angle, float* out) {
+  // ProjAlign::RotShift (ProjAlign.cpp:448-457).
+  float c = (float)std::cos(angle * kD2R), s = (float)std::sin(angle * kD2R);
+  float x = in[0] * c - in[1] * s;
+  float y = in[0] * s + in[1] * c;
+  out[0] = x;
+  out[1] = y;
+}
+
+}  // namespace
+
+void series_preprocess(mf_series* s) {
+  // PreProcess::SetPositive + MassNormalization (Preprocess.cu:103-177).
+  const int nz = s->nz;
+  const size_t plane = (size_t)s->nx * s->ny;
+  const bool rev = nz > 1 && s->file_tilts[0] > s->file_tilts[1];
+  std::vector<float> mm(2 * nz);
+  slice_minmax(s->ctx, s->st, (const float*)s->raw.get(), plane, nz, s->tmp,
+               mm.data());
+  float stack_min = 1e30f;
+  if (s->verbose) printf("Set positivity ...\n  num   residue     min       max\n");
+  std::vector<int> perm(nz);
+  for (int a = 0; a < nz; a++) {
+    const int i = rev ? nz - 1 - a : a;
+    perm[a] = i;
+    if (stack_min > mm[2 * a]) stack_min = mm[2 * a];
+    if (s->verbose)
+      printf("%4d      %4d    %8.2f  %8.2f\n", i, nz - i, mm[2 * a], mm[2 * a + 1]);
+  }
+  if (s->verbose) printf("StackMin:%f\n", stack_min);
+  float* stk = (float*)s->stack.ensure(s->ctx, plane * nz * sizeof(float));
+  reorder_subtract(s->ctx, s->st, (const float*)s->raw.get(), stk, plane, nz,
+                   perm.data(), stack_min < 0 ? stack_min : 0.0f);
+  if (s->verbose) printf("Positivity set.\n\nLinear mass normalization...\n");
+  std::vector<float> means(nz);
+  box_means(s->ctx, s->st, stk, s->nx, s->ny, nz, s->nx * 1 / 6, s->ny * 1 / 6,
+            s->nx * 4 / 6, s->ny * 4 / 6, s->tmp, means.data());
+  // The zero-tilt index is taken from the file-order angles (quirk Q3).
+  const int z0 = zero_index(s->file_tilts);
+  float ref = means[z0];
+  if (ref > 1000.0f) ref = 1000.0f;
+  std::vector<float> scale(nz);
+  for (int i = 0; i < nz; i++) scale[i] = ref / (means[i] + 0.00001f);
+  scale_slices(s->ctx, s->st, stk, plane, nz, s->tmp, scale.data());
+  if (s->verbose) printf("Linear mass normalization: done.\n\n");
+  s->raw.release();
+  s->tilts = s->file_tilts;
+  if (rev) std::reverse(s->tilts.begin(), s->tilts.end());
+  s->sx.assign(nz, 0.0);
+  s->sy.assign(nz, 0.0);
+  s->rot.assign(nz, 0.0f);
+  s->offset = 0.0f;
+  s->preprocessed = true;
+}
+
+void series_coarse_setup(mf_series* s) {
+  // Transform::Setup (Transform.cpp:55-89).
+  MF_REQUIRE(s->preprocessed, "preprocess the series first");
+  int bx = (int)(s->nx / 1024.0f + 0.1f), by = (int)(s->ny / 1024.0f + 0.1f);
+  s->tr_bin = std::max(std::max(bx, by), 1);
+  s->psx.assign(s->nz, 0.0);
+  s->psy.assign(s->nz, 0.0);
+  s->tr_angles = s->tilts;
+  s->tr_ready = true;
+  if (s->verbose) printf("Pre-align tilt series\n");
+}
+
+float series_coarse_iter(mf_series* s) {
+  // Transform::DoIt / Measure / UpdateShift (Transform.cpp:91-162).
+  if (!s->tr_ready) throw Error(MF_ESTATE, "coarse alignment not set up");
+  const int nz = s->nz;
+  const CorrGeom g = CorrGeom::make(s->nx, s->ny, s->tr_bin);
+  const size_t plane = (size_t)g.bx * g.by;
+  float* corr = (float*)s->tr_corr.ensure(s->ctx, plane * nz * sizeof(float));
+  CorrProj* cp = upload_corr(s, corr_params(s, true));
+  correct_bin(s->ctx, s->st, (const float*)s->stack.get(), g, cp, 0, nz, true,
+              corr, g.bx, plane);
+  std::vector<PairDesc> pairs;
+  std::vector<int> pz;
+  for (int z = 0; z < nz; z++) {
+    const int r = ref_index(s->tr_angles, z);
+    if (r == z) continue;
+    PairDesc p;
+    p.ref = r;
+    p.img = z;
+    double ds = std::cos(kD2R * s->tr_angles[r]) / std::cos(kD2R * s->tr_angles[z]);
+    stretch_matrix((float)ds, s->rot[z], p.m);
+    pairs.push_back(p);
+    pz.push_back(z);
+  }
+  const int np = (int)pairs.size();
+  float* work = (float*)s->tr_work.ensure(
+      s->ctx, stretch_cc_work_floats(g.bx, g.by, std::max(np, 1)) * sizeof(float));
+  double* mom = (double*)s->tr_mom.ensure(s->ctx, 4 * (np + 1) * sizeof(double));
+  float* dsh = (float*)s->tr_shift.ensure(s->ctx, 2 * (np + 1) * sizeof(float));
+  stretch_cc_pairs(s->ctx, s->st, corr, g.bx, g.by, pairs, 400.0f, work, mom,
+                   s->tr_tab, dsh);
+  std::vector<float> sh(2 * np);
+  MF_HIP(hipMemcpyAsync(sh.data(), dsh, 2 * np * sizeof(float),
+                        hipMemcpyDeviceToHost, s->st));
+  MF_HIP(hipStreamSynchronize(s->st));
+  const float binx = s->nx * 1.0f / g.bx, biny = s->ny * 1.0f / g.by;
+  float maxerr = 0.0f;
+  for (int p = 0; p < np; p++) {
+    const int z = pz[p];
+    const float px = sh[2 * p] * binx, py = sh[2 * p + 1] * biny;
+    s->psx[z] = px;
+    s->psy[z] = py;
+    if (s->verbose) printf("Image %4d shift: %8.2f, %8.2f px\n", z + 1, px, py);
+    float e = (float)std::sqrt(px * px + py * py);
+    if (e > maxerr) maxerr = e;
+  }
+  const int z0 = zero_index(s->tr_angles);
+  for (int i = 0; i < nz; i++) {
+    s->psx[i] -= s->psx[z0];
+    s->psy[i] -= s->psy[z0];
+  }
+  // Accumulate outward from the zero tilt, un-stretching the running sum
+  // into each image's frame (UnStretch, Transform.cpp:164-187), in FP64.
+  auto unstretch = [&](int ref, int f, double* sum) {
+    double ds = std::cos(s->tr_angles[ref] * kD2R) / std::cos(s->tr_angles[f] * kD2R);
+    double d2t = 2 * kD2R * s->rot[f];
+    double P = 0.5 * (ds + 1), M = 0.5 * (ds - 1);
+    float a0 = (float)(P + M * std::cos(d2t)), a1 = (float)(1 * M * std::sin(d2t)),
+          a2 = (float)(P - M * std::cos(d2t));
+    float det = a0 * a2 - a1 * a1;
+    double m0 = a2 / det, m1 = -a1 / det, m2 = a0 / det;
+    double b = sum[0] * m0 + sum[1] * m1;
+    sum[1] = sum[0] * m1 + sum[1] * m2;
+    sum[0] = b;
+  };
+  double sum[2] = {0.0, 0.0};
+  for (int i = z0 - 1; i >= 0; i--) {
+    sum[0] += s->psx[i];
+    sum[1] += s->psy[i];
+    unstretch(i + 1, i, sum);
+    s->sx[i] += sum[0];
+    s->sy[i] += sum[1];
+  }
+  sum[0] = sum[1] = 0.0;
+  for (int i = z0 + 1; i < nz; i++) {
+    sum[0] += s->psx[i];
+    sum[1] += s->psy[i];
+    unstretch(i - 1, i, sum);
+    s->sx[i] += sum[0];
+    s->sy[i] += sum[1];
+  }
+  if (s->verbose) printf("maxerr: %8.2f\n\n", maxerr);
+  return maxerr;
+}
+
+float series_tilt_axis(mf_series* s, float range, int num) {
+  // CalcTIltAxis::DoIt (Rotate.cu:292-337).
+  MF_REQUIRE(s->preprocessed, "preprocess the series first");
+  MF_REQUIRE(num >= 2, "need at least two axis candidates");
+  const int z0 = zero_index(s->tilts);
+  AxisResult r = axis_search(s->ctx, s->st, (const float*)s->stack.get(), s->nx,
+                             s->ny, s->nz, s->tilts.data(), s->sx.data(),
+                             s->sy.data(), s->rot[z0], range, num, s->axw);
+  float axis = r.axis;
+  if (axis < -45 && s->rot[0] == 0) axis += 180.0f;
+  if (axis > 135) axis -= 180.0f;
+  for (int i = 0; i < s->nz; i++) s->rot[i] = axis;
+  if (s->verbose)
+    printf("Initial estimate of tilt axes:\nNew tilt axis: %.2f\n\n", axis);
+  return axis;
+}
+
+float series_find_offset(mf_series* s) {
+  // FindOffset::DoIt / FOsetup (FindOffset.cu:113-155): bin 4 (quirk Q8),
+  // rotation, no random fill, no R-weighting.
+  MF_REQUIRE(s->preprocessed, "preprocess the series first");
+  const CorrGeom g = CorrGeom::make(s->nx, s->ny, 4);
+  const size_t plane = (size_t)g.bx * g.by;
+  float* corr = (float*)s->off_corr.ensure(s->ctx, plane * s->nz * sizeof(float));
+  CorrProj* cp = upload_corr(s, corr_params(s, false));
+  correct_bin(s->ctx, s->st, (const float*)s->stack.get(), g, cp, 0, s->nz,
+              false, corr, g.bx, plane);
+  std::vector<float> ccs;
+  const float best = offset_search(s->ctx, s->st, corr, g.bx, g.by, s->nz,
+                                   s->tilts.data(), s->offw, &ccs);
+  if (s->verbose) {
+    printf("Determine tilt angle offset.\n");
+    float mx = 0.0f;
+    for (int i = 0; i < 31; i++) {
+      printf("...... %8.2f  %.4e\n", (float)(i - 15), ccs[i]);
+      if (ccs[i] > mx) mx = ccs[i];
+    }
+    printf("Tilt offset: %8.2f,  CC: %.4f\n\n", best, mx);
+  }
+  s->offset += best;
+  if (s->verbose)
+    printf("Tilt axis angle offset is: %f\n", s->offset);
+  return best;
+}
+
+void series_apply_offset(mf_series* s) {
+  // Process::mCoarseAlign (Process.cpp:266-269).
+  for (int z = 0; z < s->nz; z++) s->tilts[z] += s->offset;
+}
+
+void series_projalign_setup(mf_series* s, float zshift, int align_z,
+                            int nproj) {
+  // ProjAlign::Setup / Setsize (ProjAlign.cpp:126-203).
+  MF_REQUIRE(s->preprocessed, "preprocess the series first");
+  MF_REQUIRE(align_z > 0, "AlignZ must be > 0");
+  MF_REQUIRE(nproj > 0 && nproj <= kMaxRefProj, "NPROJ out of range");
+  s->pa_z0 = zshift;
+  int bx = (int)(s->nx / 2048.0f + 0.5f), by = (int)(s->ny / 2048.0f + 0.5f);
+  s->pa_bin = std::max(std::max(bx, by), 1);
+  s->pa_geom = CorrGeom::make(s->nx, s->ny, s->pa_bin);
+  s->pa_tb = align_z / s->pa_bin / 2 * 2;
+  MF_REQUIRE(s->pa_tb >= 2, "projection-matching thickness too small");
+  s->pa_nproj = nproj;
+  s->pa_angles = s->tilts;
+  const CorrGeom& g = s->pa_geom;
+  const size_t plane = (size_t)g.bx * g.by, padb = pad_x(g.bx);
+  s->pa_corr.ensure(s->ctx, plane * s->nz * sizeof(float));
+  s->pa_pad.ensure(s->ctx, (size_t)g.by * padb * s->nz * sizeof(float));
+  s->pa_reproj.ensure(s->ctx, plane * sizeof(float));
+  s->pa_work.ensure(s->ctx, 2 * (size_t)g.by * padb * sizeof(float));
+  s->pa_mom.ensure(s->ctx, 8 * sizeof(double));
+  s->pa_ready = true;
+  if (s->verbose)
+    printf("bin:%d   cmpsize:%d,%d\nbinsize:%d, %d\n", s->pa_bin, g.bx / 2 + 1,
+           g.by, g.bx, g.by);
+}
+
+float series_projalign_iter(mf_series* s) {
+  // ProjAlign::DoIt / DoItBin (ProjAlign.cpp:205-257).
+  if (!s->pa_ready) throw Error(MF_ESTATE, "projection matching not set up");
+  mf_ctx* ctx = s->ctx;
+  const int nz = s->nz;
+  const CorrGeom& g = s->pa_geom;
+  const size_t plane = (size_t)g.bx * g.by;
+  const int padb = pad_x(g.bx);
+  const size_t pimg = (size_t)g.by * padb;
+  float* corr = (float*)s->pa_corr.get();
+  float* pad = (float*)s->pa_pad.get();
+  // Correct + bin + R-weight the whole stack with the current parameters.
+  CorrProj* cp = upload_corr(s, corr_params(s, false));
+  correct_bin(ctx, s->st, (const float*)s->stack.get(), g, cp, 0, nz, true,
+              pad, padb, pimg);
+  rweight_rows(ctx, s->st, pad, g.bx, nz * g.by, corr);
+  // FitRotCenterZ (ProjAlign.cpp:459-486) with dA starting at 0 (quirk Q13),
+  // then RemoveOffsetZ(-1) (:488-501).
+  const std::vector<float>& ang = s->pa_angles;
+  const int z0 = zero_index(ang);
+  float s0[2] = {(float)s->sx[z0], (float)s->sy[z0]};
+  rot_shift(s0, -s->rot[z0], s0);
+  const double cent0 = s0[0] / std::cos(ang[z0] * kD2R);
+  double dA = 0.0, dB = 0.0;
+  for (int i = 0; i < nz; i++) {
+    float r[2] = {(float)s->sx[i], (float)s->sy[i]};
+    rot_shift(r, -s->rot[i], r);
+    double sn = std::sin(ang[i] * kD2R), cs = std::cos(ang[i] * kD2R);
+    dA += (cent0 * cs - r[0]) * sn;
+    dB += sn * sn;
+  }
+  const float old = s->pa_z0;
+  s->pa_z0 = (float)(dA / (dB + 1e-30));
+  if (s->verbose)
+    printf("Rot center Z: %8.2f  %8.2f  %8.2f\n\n", old, s->pa_z0, s->pa_z0 - old);
+  std::vector<float> ind(2 * nz, 0.0f), rcs(2 * nz);
+  for (int i = 0; i < nz; i++) {
+    // CalcZInducedShift (ProjAlign.cpp:553-561).
+    if (s->pa_z0 != 0) {
+      float t = (float)(ang[i] * kD2R);
+      float v[2] = {-s->pa_z0 * hsin(t), 0.0f};
+      rot_shift(v, s->rot[i], v);
+      ind[2 * i] = v[0];
+      ind[2 * i + 1] = v[1];
+    }
+    rcs[2 * i] = (float)std::cos(s->rot[i] * kD2R);
+    rcs[2 * i + 1] = (float)std::sin(s->rot[i] * kD2R);
+  }
+  if (s->pa_z0 != 0)
+    for (int i = 0; i < nz; i++) {
+      if (i == z0) continue;
+      s->sx[i] += (double)(ind[2 * i] * -1.0f);
+      s->sy[i] += (double)(ind[2 * i + 1] * -1.0f);
+    }
+  PaDev d = pa_layout(s);
+  MF_HIP(hipMemcpyAsync(d.sx, s->sx.data(), nz * 8, hipMemcpyHostToDevice, s->st));
+  MF_HIP(hipMemcpyAsync(d.sy, s->sy.data(), nz * 8, hipMemcpyHostToDevice, s->st));
+  MF_HIP(hipMemcpyAsync(d.ind, ind.data(), nz * 8, hipMemcpyHostToDevice, s->st));
+  MF_HIP(hipMemcpyAsync(d.rcs, rcs.data(), nz * 8, hipMemcpyHostToDevice, s->st));
+  std::vector<unsigned char> skip(nz, 1);
+  skip[z0] = 0;
+  const float binx = s->nx / (float)g.bx, biny = s->ny / (float)g.by;
+  float* reproj = (float*)s->pa_reproj.get();
+  float* work = (float*)s->pa_work.get();
+  double* mom = (double*)s->pa_mom.get();
+  float* slab = nullptr;
+  if (s->reproj_mode == 0)
+    slab = (float*)s->pa_slab.ensure(ctx, plane * 2 * s->pa_tb * sizeof(float));
+  std::vector<int> order;
+  auto align = [&](int t) {
+    // ClacAlignProj (ProjAlign.cpp:259-319).
+    reproject_target(ctx, s->st, corr, g.bx, g.by, nz, s->pa_tb, ang.data(),
+                     skip.data(), t, s->pa_nproj, s->reproj_mode, reproj, slab);
+    pa_measure(ctx, s->st, reproj, corr + t * plane, g.bx, g.by, work, mom,
+               (float*)d.ishift);
+    hipLaunchKernelGGL(k_pa_update, dim3(1), dim3(1), 0, s->st, d, t, binx,
+                       biny, cp);
+    MF_HIP(hipGetLastError());
+    correct_bin(ctx, s->st, (const float*)s->stack.get(), g, cp, t, 1, true,
+                pad + t * pimg, padb, pimg);
+    rweight_rows(ctx, s->st, pad + t * pimg, g.bx, g.by, corr + t * plane);
+    skip[t] = 0;
+    order.push_back(t);
+  };
+  for (int i = 1; i < nz; i++) {
+    if (z0 + i < nz) align(z0 + i);
+    if (z0 - i >= 0) align(z0 - i);
+  }
+  std::vector<float> err(nz), meas(2 * nz);
+  MF_HIP(hipMemcpyAsync(s->sx.data(), d.sx, nz * 8, hipMemcpyDeviceToHost, s->st));
+  MF_HIP(hipMemcpyAsync(s->sy.data(), d.sy, nz * 8, hipMemcpyDeviceToHost, s->st));
+  MF_HIP(hipMemcpyAsync(err.data(), d.err, nz * 4, hipMemcpyDeviceToHost, s->st));
+  MF_HIP(hipMemcpyAsync(meas.data(), d.meas, nz * 8, hipMemcpyDeviceToHost, s->st));
+  MF_HIP(hipStreamSynchronize(s->st));
+  if (s->verbose)
+    printf("# Projection matching measurements\n# tilt angle  x shift  y shift\n");
+  float maxerr = -1e20f;
+  for (int t : order) {
+    if (s->verbose)
+      printf("  %6.2f  %8.2f  %8.2f\n", ang[t], meas[2 * t], meas[2 * t + 1]);
+    if (err[t] > maxerr) maxerr = err[t];
+  }
+  return maxerr;
+}
+
+void series_projalign_once(mf_series* s) {
+  // Process::ProjAlignOnce (Process.cpp:292-319); the saved parameters are
+  // never restored (quirk Q15), so none are saved here.
+  float last = series_projalign_iter(s);
+  if (s->verbose) printf("fLastErr为: %8.2f\n", last);
+  for (int i = 1; i < 10; i++) {
+    float err = series_projalign_iter(s);
+    if (s->verbose) printf("fErr为: %8.2f\n", err);
+    if (err <= last) {
+      if (err < 2.0f) break;
+      last = err;
+    }
+  }
+}
+
+void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by) {
+  // Process::CorrectStack (Process.cpp:329-351): rotation + shift, random
+  // fill, no R-weighting, OutBin sum binning.
+  MF_REQUIRE(s->preprocessed, "preprocess the series first");
+  MF_REQUIRE(out_bin >= 1, "OutBin must be >= 1");
+  const CorrGeom g = CorrGeom::make(s->nx, s->ny, out_bin);
+  if (bx) *bx = g.bx;
+  if (by) *by = g.by;
+  if (!out) return;
+  const size_t plane = (size_t)g.bx * g.by;
+  float* d = (float*)s->tmp.ensure(s->ctx, plane * s->nz * sizeof(float));
+  CorrProj* cp = upload_corr(s, corr_params(s, false));
+  correct_bin(s->ctx, s->st, (const float*)s->stack.get(), g, cp, 0, s->nz,
+              true, d, g.bx, plane);
+  MF_HIP(hipMemcpyAsync(out, d, plane * s->nz * sizeof(float),
+                        hipMemcpyDeviceToHost, s->st));
+  MF_HIP(hipStreamSynchronize(s->st));
+}
+
+void series_run(mf_series* s, const mf_options* o) {
+  // Process::DoIt (Process.cpp:34-92) minus file I/O.
+  s->verbose = o->verbose;
+  s->reproj_mode = o->reproj_mode;
+  if (!s->preprocessed) series_preprocess(s);
+  if (o->align_z == 0) return;
+  // SetParam (Process.cpp:208-221).
+  std::fill(s->sx.begin(), s->sx.end(), 0.0);
+  std::fill(s->sy.begin(), s->sy.end(), 0.0);
+  std::fill(s->rot.begin(), s->rot.end(), 0.0f);
+  s->offset = 0.0f;
+  // mCoarseAlign (Process.cpp:238-270).
+  if (o->pitch_angle == 0) {
+    series_coarse_setup(s);
+    for (int i = 1; i <= 3; i++) {
+      series_coarse_iter(s);
+      series_tilt_axis(s, 180.0f / i, 100);
+    }
+  } else {
+    std::fill(s->rot.begin(), s->rot.end(), o->pitch_angle);
+  }
+  if (o->offset == 0)
+    series_find_offset(s);
+  else
+    s->offset = o->offset;
+  series_apply_offset(s);
+  // mProjAlign (Process.cpp:272-290); ResetShift first (quirk Q16).
+  std::fill(s->sx.begin(), s->sx.end(), 0.0);
+  std::fill(s->sy.begin(), s->sy.end(), 0.0);
+  series_projalign_setup(s, o->zshift, o->align_z, o->nproj);
+  series_projalign_once(s);
+  for (int i = 1; i <= 4; i++) {
+    series_tilt_axis(s, 20.0f / i, 100);
+    if (i == 1) series_projalign_once(s);
+  }
+  series_projalign_once(s);
+}
+
+}  // namespace mf

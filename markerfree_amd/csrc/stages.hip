@@ -1,0 +1,357 @@
+// Preprocessing, stack correction / binning and R-weighting kernels.
+//
+// Reference: PreProcess (method/Preprocess.cu), CorrTomoStack, BinImage,
+// GCorrPatchShift and RWeight (method/CorrStack.cu).  The reference corrects
+// one projection per call after a disk read and H2D copy of it; here the
+// preprocessed stack stays in HBM and one launch corrects any number of
+// projections.
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "context.h"
+#include "devutil.h"
+#include "fft_dev.h"
+#include "stages.h"
+
+namespace mf {
+
+CorrGeom CorrGeom::make(int nx, int ny, int bin) {
+  CorrGeom g;
+  g.nx = nx;
+  g.ny = ny;
+  g.cx = (pad_x(nx) / 2 - 1) * 2;
+  g.bx = nx / bin / 2 * 2;
+  g.by = ny / bin / 2 * 2;
+  MF_REQUIRE(g.bx >= 2 && g.by >= 2, "binned image too small");
+  g.binx = g.cx / g.bx;
+  g.biny = ny / g.by;
+  return g;
+}
+
+namespace {
+
+constexpr int kRedThreads = 1024;
+
+__global__ __launch_bounds__(kRedThreads) void k_minmax(const float* d,
+                                                        size_t plane,
+                                                        float* out) {
+  __shared__ float sh[16];
+  const float* p = d + (size_t)blockIdx.x * plane;
+  float mn = 1e20f, mx = -1e20f;
+  for (size_t i = threadIdx.x; i < plane; i += blockDim.x) {
+    float v = p[i];
+    if (v < -1e10f) continue;
+    mn = fminf(mn, v);
+    mx = fmaxf(mx, v);
+  }
+  mn = block_reduce(mn, sh, [](float a, float b) { return fminf(a, b); }, 1e20f);
+  mx = block_reduce(mx, sh, [](float a, float b) { return fmaxf(a, b); }, -1e20f);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = mn;
+    out[2 * blockIdx.x + 1] = mx;
+  }
+}
+
+__global__ void k_subtract(float* d, size_t n, float sub) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    float v = d[i];
+    if (v < -1e10f) continue;
+    d[i] = v - sub;
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_box_mean(
+    const float* d, int nx, int ny, int x0, int y0, int w, int h,
+    double* out) {
+  __shared__ double sh[16];
+  __shared__ int shc[16];
+  const float* p = d + (size_t)blockIdx.x * nx * ny;
+  double s = 0.0;
+  int c = 0;
+  const int n = w * h;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int y = i / w, x = i - y * w;
+    float v = p[(size_t)(y + y0) * nx + x + x0];
+    if (v <= -1e20f) continue;
+    s += v;
+    c++;
+  }
+  s = block_reduce(s, sh, [](double a, double b) { return a + b; }, 0.0);
+  c = block_reduce(c, shc, [](int a, int b) { return a + b; }, 0);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = s;
+    out[2 * blockIdx.x + 1] = (double)c;
+  }
+}
+
+__global__ void k_scale(float* d, size_t plane, const float* scale) {
+  float* p = d + (size_t)blockIdx.y * plane;
+  const float f = scale[blockIdx.y];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < plane;
+       i += (size_t)gridDim.x * blockDim.x) {
+    float v = p[i];
+    if (v <= -1e20f) continue;
+    p[i] = v * f;
+  }
+}
+
+// mGRandom (CorrStack.cu:116-140): window 51, next = (19 next + 57) mod 2601.
+__device__ float corr_random(int x, int y, int nx, int ny, const float* raw) {
+  if (x < 0) x = -x;
+  if (y < 0) y = -y;
+  if (x >= nx) x = nx - 1 - (x % nx);
+  if (y >= ny) y = ny - 1 - (y % ny);
+  unsigned next = (unsigned)(y * nx + x);
+  for (int i = 0; i < 20; i++) {
+    next = (next * 19u + 57u) % 2601u;
+    int ix = (int)next - 25 + x;
+    if (ix < 0 || ix >= nx) continue;
+    int iy = (int)(next / 51u) - 25 + y;
+    if (iy < 0 || iy >= ny) continue;
+    return raw[(size_t)iy * nx + ix];
+  }
+  return raw[(size_t)y * nx + x];
+}
+
+struct CorrArgs {
+  const float* raw;  // [nz][ny][nx]
+  const CorrProj* proj;
+  float* out;
+  int nx, ny, cx, bx, by, binx, biny, z0, fill, out_pitch;
+  size_t out_stride;
+};
+
+// One binned output pixel per thread: each of its bin x bin corrected pixels
+// is the nearest raw pixel at R(axis) p + shift + centre (mGCorrect), random
+// filled or invalid outside; the bin sum is invalid if any is (mGBinImage).
+__global__ __launch_bounds__(256) void k_correct_bin(const CorrArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.bx * a.by) return;
+  const int y = idx / a.bx, x = idx - y * a.bx;
+  const int z = a.z0 + blockIdx.y;
+  const CorrProj pr = a.proj[z];
+  const float* raw = a.raw + (size_t)z * a.nx * a.ny;
+  const float hx = (float)a.cx * 0.5f, hy = (float)a.ny * 0.5f;
+  const float ox = pr.shift_x + (float)a.nx * 0.5f;
+  const float oy = pr.shift_y + (float)a.ny * 0.5f;
+  float sum = 0.f;
+  bool bad = false;
+  for (int iy = 0; iy < a.biny && !bad; iy++) {
+    const float py = (float)(y * a.biny + iy) + 0.5f - hy;
+    for (int ix = 0; ix < a.binx; ix++) {
+      const float px = (float)(x * a.binx + ix) + 0.5f - hx;
+      float qx = px * pr.cos_a - py * pr.sin_a;
+      float qy = px * pr.sin_a + py * pr.cos_a;
+      qx += ox;
+      qy += oy;
+      const int rx = (int)(qx + 0.5f), ry = (int)(qy + 0.5f);
+      float v;
+      if (rx >= 0 && rx < a.nx && ry >= 0 && ry < a.ny)
+        v = raw[(size_t)ry * a.nx + rx];
+      else if (a.fill)
+        v = corr_random(rx, ry, a.nx, a.ny, raw);
+      else
+        v = kInvalid;
+      if (v < -1e10f) {
+        bad = true;
+        break;
+      }
+      sum += v;
+    }
+  }
+  a.out[(size_t)blockIdx.y * a.out_stride + (size_t)y * a.out_pitch + x] =
+      bad ? kInvalid : sum;
+}
+
+constexpr int kRowThreads = 256;
+
+// R-weighting of nl padded rows per workgroup: R2C (length bx), scale by
+// 1/N and the ramp filter, C2R, unpadded write (RWeight::DoIt + Unpad).
+__global__ __launch_bounds__(kRowThreads) void k_rweight(
+    const float* in, int pitch, float* out, int nrows, FftDevPlan P, int nl,
+    const float* filt) {
+  extern __shared__ float2 sm2[];
+  const int N = P.n, ls = fft_lds_len(P), xs = N + 1;
+  const float fn = 2.0f * ((float)(N + 1) - 1.0f);
+  float2* a = sm2;
+  float2* b = a + nl * ls;
+  float2* X = b + nl * ls;
+  const int r0 = blockIdx.x * nl;
+  const int nv = min(nl, nrows - r0);
+  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+    int l = i / N, k = i - l * N;
+    float2 v = make_float2(0.f, 0.f);
+    if (l < nv) v = reinterpret_cast<const float2*>(in + (size_t)(r0 + l) * pitch)[k];
+    a[l * ls + k] = v;
+  }
+  __syncthreads();
+  float2* Z = fft_lines(a, b, P, nl, false);
+  for (int i = threadIdx.x; i < nl * xs; i += blockDim.x) {
+    int l = i / xs, k = i - l * xs;
+    r2c_unpack(Z + l * ls, N, P.half, 1.0f, k, [&](int kk, float2 v) {
+      // GRWeight (CorrStack.cu:564-579): x /= N, then x *= ramp.
+      const float f = filt[kk];
+      X[l * xs + kk] = make_float2((v.x / fn) * f, (v.y / fn) * f);
+    });
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+    int l = i / N, k = i - l * N;
+    a[l * ls + k] = c2r_pack(X[l * xs + k], X[l * xs + N - k], P.half[k]);
+  }
+  __syncthreads();
+  float2* z = fft_lines(a, b, P, nl, true);
+  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+    int l = i / N, k = i - l * N;
+    if (l >= nv) continue;
+    reinterpret_cast<float2*>(out + (size_t)(r0 + l) * 2 * N)[k] = z[l * ls + k];
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_moments(const float* d, int w,
+                                                         int h, int pitch,
+                                                         size_t stride,
+                                                         double* out) {
+  __shared__ double sh[16];
+  const float* p = d + (size_t)blockIdx.x * stride;
+  double s1 = 0.0, s2 = 0.0;
+  const int n = w * h;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int y = i / w, x = i - y * w;
+    float v = p[(size_t)y * pitch + x];
+    if (v < -1e10f) continue;
+    s1 += v;
+    s2 += (double)(v * v);
+  }
+  s1 = block_reduce(s1, sh, [](double a, double b) { return a + b; }, 0.0);
+  s2 = block_reduce(s2, sh, [](double a, double b) { return a + b; }, 0.0);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = s1;
+    out[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// Ramp-filter table of GRWeight for row length n (host-evaluated cosine).
+const float* rweight_filter(mf_ctx* ctx, int n) {
+  return (const float*)ctx_table(ctx, "rweight/" + std::to_string(n), [&] {
+    const int nc = n / 2 + 1;
+    const float fn = 2.0f * ((float)nc - 1.0f);
+    std::vector<float> h(nc);
+    for (int k = 0; k < nc; k++) {
+      float r = (float)k / fn;
+      h[k] = 2.0f * r * (0.55f + 0.45f * hcos(6.2831852f * r));
+    }
+    return h;
+  });
+}
+
+}  // namespace
+
+void slice_minmax(mf_ctx* ctx, hipStream_t st, const float* d, size_t plane,
+                  int nz, DevBuf& tmp, float* h_out) {
+  float* buf = (float*)tmp.ensure(ctx, sizeof(float) * 2 * nz);
+  hipLaunchKernelGGL(k_minmax, dim3(nz), dim3(kRedThreads), 0, st, d, plane,
+                     buf);
+  MF_HIP(hipGetLastError());
+  MF_HIP(hipMemcpyAsync(h_out, buf, sizeof(float) * 2 * nz,
+                        hipMemcpyDeviceToHost, st));
+  MF_HIP(hipStreamSynchronize(st));
+}
+
+void reorder_subtract(mf_ctx* ctx, hipStream_t st, const float* d_raw,
+                      float* d_stack, size_t plane, int nz, const int* h_perm,
+                      float sub) {
+  (void)ctx;
+  for (int a = 0; a < nz; a++)
+    MF_HIP(hipMemcpyAsync(d_stack + (size_t)h_perm[a] * plane,
+                          d_raw + (size_t)a * plane, plane * sizeof(float),
+                          hipMemcpyDeviceToDevice, st));
+  if (sub != 0.0f) {
+    hipLaunchKernelGGL(k_subtract, dim3(4096), dim3(256), 0, st, d_stack,
+                       plane * nz, sub);
+    MF_HIP(hipGetLastError());
+  }
+}
+
+void box_means(mf_ctx* ctx, hipStream_t st, const float* d, int nx, int ny,
+               int nz, int x0, int y0, int w, int h, DevBuf& tmp,
+               float* h_out) {
+  double* buf = (double*)tmp.ensure(ctx, sizeof(double) * 2 * nz);
+  hipLaunchKernelGGL(k_box_mean, dim3(nz), dim3(kRedThreads), 0, st, d, nx, ny,
+                     x0, y0, w, h, buf);
+  MF_HIP(hipGetLastError());
+  std::vector<double> hb(2 * nz);
+  MF_HIP(hipMemcpyAsync(hb.data(), buf, sizeof(double) * 2 * nz,
+                        hipMemcpyDeviceToHost, st));
+  MF_HIP(hipStreamSynchronize(st));
+  for (int z = 0; z < nz; z++)
+    h_out[z] = hb[2 * z + 1] == 0 ? 0.0f : (float)(hb[2 * z] / hb[2 * z + 1]);
+}
+
+void scale_slices(mf_ctx* ctx, hipStream_t st, float* d, size_t plane, int nz,
+                  DevBuf& tmp, const float* h_scale) {
+  float* s = (float*)tmp.ensure(ctx, sizeof(float) * nz);
+  MF_HIP(hipMemcpyAsync(s, h_scale, sizeof(float) * nz, hipMemcpyHostToDevice,
+                        st));
+  hipLaunchKernelGGL(k_scale, dim3(256, nz), dim3(256), 0, st, d, plane, s);
+  MF_HIP(hipGetLastError());
+  MF_HIP(hipStreamSynchronize(st));
+}
+
+void correct_bin(mf_ctx* ctx, hipStream_t st, const float* d_raw,
+                 const CorrGeom& g, const CorrProj* d_proj, int z0, int count,
+                 bool random_fill, float* d_out, int out_pitch,
+                 size_t out_stride) {
+  if (count <= 0) return;
+  CorrArgs a;
+  a.raw = d_raw;
+  a.proj = d_proj;
+  a.out = d_out;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.cx = g.cx;
+  a.bx = g.bx;
+  a.by = g.by;
+  a.binx = g.binx;
+  a.biny = g.biny;
+  a.z0 = z0;
+  a.fill = random_fill ? 1 : 0;
+  a.out_pitch = out_pitch;
+  a.out_stride = out_stride;
+  TimedLaunch tl(ctx->timer, "correct", st);
+  hipLaunchKernelGGL(k_correct_bin, dim3(cdiv((long long)g.bx * g.by, 256), count),
+                     dim3(256), 0, st, a);
+  MF_HIP(hipGetLastError());
+}
+
+void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
+                  int nrows, float* d_out) {
+  MF_REQUIRE(bx % 2 == 0 && bx >= 2, "R-weighting needs an even row length");
+  const FftDevPlan& P = ctx->fft->plan(bx / 2);
+  const int ls = fft_lds_len(P);
+  const size_t per = (size_t)(2 * ls + P.n + 1) * sizeof(float2);
+  const int nl = std::max(1, std::min(16, (int)((64 * 1024) / per)));
+  const size_t lds = nl * per;
+  MF_REQUIRE(lds <= 160 * 1024, "R-weighting row exceeds LDS");
+  if (lds > 64 * 1024)
+    MF_HIP(hipFuncSetAttribute((const void*)k_rweight,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+  const float* filt = rweight_filter(ctx, bx);
+  TimedLaunch tl(ctx->timer, "rweight", st);
+  hipLaunchKernelGGL(k_rweight, dim3(cdiv(nrows, nl)), dim3(kRowThreads), lds,
+                     st, d_pad, pad_x(bx), d_out, nrows, P, nl, filt);
+  MF_HIP(hipGetLastError());
+}
+
+void image_moments(mf_ctx* ctx, hipStream_t st, const float* d, int w, int h,
+                   int pitch, size_t stride, int nimg, double* d_out) {
+  (void)ctx;
+  hipLaunchKernelGGL(k_moments, dim3(nimg), dim3(kRedThreads), 0, st, d, w, h,
+                     pitch, stride, d_out);
+  MF_HIP(hipGetLastError());
+}
+
+}  // namespace mf

@@ -109,3 +109,182 @@ def fft_c2r_2d(spec, nx):
     buf = np.ascontiguousarray(spec, np.complex64).view(np.float32).copy()
     _lib.orc_fft_c2r_2d(_f(buf), nx, ny)
     return buf[:, :nx].copy()
+
+
+# ------------------------------------------------------------ stage functions
+_sig("orc_correct_proj", None, _fp, C.c_int, C.c_int, C.c_float, C.c_float,
+     C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _ip)
+_sig("orc_binned_size", None, C.c_int, C.c_int, C.c_int, _ip)
+_sig("orc_stretch_cc", None, _fp, _fp, C.c_int, C.c_int, C.c_float, C.c_float,
+     C.c_float, C.c_float, _fp)
+_sig("orc_pa_measure", None, _fp, _fp, C.c_int, C.c_int, _fp)
+_sig("orc_tilt_axis", C.c_float, _fp, C.c_int, C.c_int, C.c_int, _fp, _fp, _fp,
+     _fp, C.c_float, C.c_int, _fp, _ip)
+_sig("orc_find_offset", C.c_float, _fp, C.c_int, C.c_int, C.c_int, _fp, _fp,
+     _fp, _fp, _fp)
+
+
+def binned_size(nx, ny, binning):
+    r = (C.c_int * 2)()
+    _lib.orc_binned_size(nx, ny, binning, r)
+    return int(r[0]), int(r[1])
+
+
+def correct_proj(raw, shift_x, shift_y, axis_deg, shift_only, random_fill,
+                 binning, rweight):
+    """CorrTomoStack::mCorrectProj for one raw image [ny][nx] -> [by][bx]."""
+    raw = np.ascontiguousarray(raw, np.float32)
+    ny, nx = raw.shape
+    bx, by = binned_size(nx, ny, binning)
+    out = np.empty((by, bx), np.float32)
+    _lib.orc_correct_proj(_f(raw), nx, ny, shift_x, shift_y, axis_deg,
+                          int(shift_only), int(random_fill), binning,
+                          int(rweight), _f(out), None)
+    return out
+
+
+def stretch_cc(ref, img, ref_tilt, tilt, axis_deg, bfactor=400.0):
+    ref = np.ascontiguousarray(ref, np.float32)
+    img = np.ascontiguousarray(img, np.float32)
+    by, bx = ref.shape
+    out = np.zeros(2, np.float32)
+    _lib.orc_stretch_cc(_f(ref), _f(img), bx, by, ref_tilt, tilt, axis_deg,
+                        bfactor, _f(out))
+    return float(out[0]), float(out[1])
+
+
+def pa_measure(reproj, proj):
+    reproj = np.ascontiguousarray(reproj, np.float32)
+    proj = np.ascontiguousarray(proj, np.float32)
+    by, bx = proj.shape
+    out = np.zeros(2, np.float32)
+    _lib.orc_pa_measure(_f(reproj), _f(proj), bx, by, _f(out))
+    return float(out[0]), float(out[1])
+
+
+def tilt_axis(stack, tilts, sx, sy, rot, arange, num=100):
+    stack = np.ascontiguousarray(stack, np.float32)
+    nz, ny, nx = stack.shape
+    scores = np.zeros(num, np.float32)
+    L = C.c_int()
+    a = [np.ascontiguousarray(v, np.float32) for v in (tilts, sx, sy, rot)]
+    axis = _lib.orc_tilt_axis(_f(stack), nx, ny, nz, *map(_f, a), arange, num,
+                              _f(scores), C.byref(L))
+    return float(axis), scores, int(L.value)
+
+
+def find_offset(stack, tilts, sx, sy, rot):
+    stack = np.ascontiguousarray(stack, np.float32)
+    nz, ny, nx = stack.shape
+    ccs = np.zeros(31, np.float32)
+    a = [np.ascontiguousarray(v, np.float32) for v in (tilts, sx, sy, rot)]
+    best = _lib.orc_find_offset(_f(stack), nx, ny, nz, *map(_f, a), _f(ccs))
+    return float(best), ccs
+
+
+# ------------------------------------------------------------------ series
+class Options(C.Structure):
+    _fields_ = [("offset", C.c_float), ("pitch_angle", C.c_float),
+                ("zshift", C.c_float), ("thickness", C.c_int),
+                ("align_z", C.c_int), ("out_bin", C.c_int), ("nproj", C.c_int),
+                ("verbose", C.c_int)]
+
+
+_sig("orc_series_create", _vp, C.c_int, C.c_int, C.c_int, _fp, _fp)
+_sig("orc_series_destroy", None, _vp)
+_sig("orc_preprocess", None, _vp)
+_sig("orc_get_stack", None, _vp, _fp)
+_sig("orc_get_tilts", None, _vp, _fp)
+_sig("orc_set_tilts", None, _vp, _fp)
+_sig("orc_get_params", None, _vp, _fp, _fp, _fp, _fp)
+_sig("orc_set_params", None, _vp, _fp, _fp, _fp, C.c_float)
+_sig("orc_coarse_setup", None, _vp)
+_sig("orc_coarse_iter", C.c_float, _vp)
+_sig("orc_series_tilt_axis", C.c_float, _vp, C.c_float, C.c_int)
+_sig("orc_series_find_offset", C.c_float, _vp)
+_sig("orc_apply_offset", None, _vp)
+_sig("orc_projalign_setup", None, _vp, C.c_float, C.c_int, C.c_int)
+_sig("orc_projalign_iter", C.c_float, _vp)
+_sig("orc_projalign_once", None, _vp)
+_sig("orc_correct_final", None, _vp, C.c_int, _fp, _ip)
+_sig("orc_run", None, _vp, C.POINTER(Options))
+_sig("orc_get_corrected", None, _vp, _fp, _ip)
+
+
+class Series:
+    """orc_series: the reference pipeline state for one tilt series."""
+
+    def __init__(self, stack, tilts_file_order):
+        self.stack_raw = np.ascontiguousarray(stack, np.float32)
+        self.nz, self.ny, self.nx = self.stack_raw.shape
+        t = np.ascontiguousarray(tilts_file_order, np.float32)
+        self.h = _lib.orc_series_create(self.nx, self.ny, self.nz,
+                                        _f(self.stack_raw), _f(t))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _lib.orc_series_destroy(self.h)
+            self.h = None
+
+    def preprocess(self):
+        _lib.orc_preprocess(self.h)
+
+    def stack(self):
+        out = np.empty((self.nz, self.ny, self.nx), np.float32)
+        _lib.orc_get_stack(self.h, _f(out))
+        return out
+
+    def tilts(self):
+        out = np.empty(self.nz, np.float32)
+        _lib.orc_get_tilts(self.h, _f(out))
+        return out
+
+    def set_tilts(self, t):
+        t = np.ascontiguousarray(t, np.float32)
+        _lib.orc_set_tilts(self.h, _f(t))
+
+    def params(self):
+        sx, sy, rot = (np.empty(self.nz, np.float32) for _ in range(3))
+        off = C.c_float()
+        _lib.orc_get_params(self.h, _f(sx), _f(sy), _f(rot), C.byref(off))
+        return sx, sy, rot, float(off.value)
+
+    def set_params(self, sx, sy, rot, offset=0.0):
+        a = [np.ascontiguousarray(v, np.float32) for v in (sx, sy, rot)]
+        _lib.orc_set_params(self.h, *map(_f, a), offset)
+
+    def coarse_setup(self):
+        _lib.orc_coarse_setup(self.h)
+
+    def coarse_iter(self):
+        return float(_lib.orc_coarse_iter(self.h))
+
+    def tilt_axis(self, arange, num=100):
+        return float(_lib.orc_series_tilt_axis(self.h, arange, num))
+
+    def find_offset(self):
+        return float(_lib.orc_series_find_offset(self.h))
+
+    def apply_offset(self):
+        _lib.orc_apply_offset(self.h)
+
+    def projalign_setup(self, zshift, align_z, nproj):
+        _lib.orc_projalign_setup(self.h, zshift, align_z, nproj)
+
+    def projalign_iter(self):
+        return float(_lib.orc_projalign_iter(self.h))
+
+    def projalign_once(self):
+        _lib.orc_projalign_once(self.h)
+
+    def correct_final(self, out_bin=1):
+        bx, by = binned_size(self.nx, self.ny, out_bin)
+        out = np.empty((self.nz, by, bx), np.float32)
+        _lib.orc_correct_final(self.h, out_bin, _f(out), None)
+        return out
+
+    def run(self, align_z, nproj=10, offset=0.0, pitch_angle=0.0, zshift=0.0,
+            out_bin=1, verbose=False):
+        o = Options(offset, pitch_angle, zshift, 0, align_z, out_bin, nproj,
+                    int(verbose))
+        _lib.orc_run(self.h, C.byref(o))

@@ -103,7 +103,11 @@ void backproject_row(const float* stack, int bx, int by, int tb, int y,
         if (rho < 0.0f || rho > iend) continue;
         float v = stack[(size_t)i * plane + (size_t)y * bx + (int)rho];
         if (v < -1e10f) continue;  // mDIntProj -> -1e30 -> skipped
-        sum += (Acc)v * (Acc)c * (Acc)w;
+        if (sizeof(Acc) == sizeof(float))
+          // fInt += v*cos*w as nvcc contracts it: fma(v*cos, w, fInt).
+          sum = (Acc)std::fmaf(v * c, w, (float)sum);
+        else
+          sum += (Acc)v * (Acc)c * (Acc)w;
         cnt += (Acc)w;
       }
       vol[(size_t)iz * vx + ix] =
@@ -184,12 +188,10 @@ extern "C" void orc_reproject(const float* stack, int bx, int by, int nz,
       }
       if (slab_out)
         std::memcpy(slab_out + (size_t)y * vsz, vol.data(), vsz * 4);
-      if (acc64)
-        reproject_row<double>(vol.data(), bx, tb, c, s, L,
-                              reproj + (size_t)y * bx);
-      else
-        reproject_row<float>(vol.data(), bx, tb, c, s, L,
-                             reproj + (size_t)y * bx);
+      // Ray sums accumulate in FP64 in both modes (the reference's FP32 tree
+      // order is unpinned; a sequential FP64 sum is the exact-ish value).
+      reproject_row<double>(vol.data(), bx, tb, c, s, L,
+                            reproj + (size_t)y * bx);
     }
   }
 }

@@ -211,6 +211,27 @@ def test_gpu_reproject_parity(ctx, orc, mode, bx, by, nz, step, tb):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gpu_reproject_bitexact_fp32(ctx, orc, mode):
+    """Against the oracle's FP32 mode (backprojection accumulated as
+    fma(v*cos, w, acc) in reference order, ray sums in FP64) the HIP
+    reprojection is bit-identical."""
+    stack, tilts = _series(160, 12, 41, 3.0, seed=21)
+    stack[3, 2, 40:44] = INVALID
+    skip = np.zeros(41, np.uint8)
+    for target in (27, 5):
+        skip[target] = 1
+        ref = orc.reproject(stack, 48, tilts, skip, target, 10, acc64=False)
+        d_stack = ctx.array(stack)
+        d_out = ctx.empty(ref.shape)
+        ctx.reproject(d_stack, 160, 12, 41, 48, tilts, skip, target, 10, d_out,
+                      mode=mode)
+        np.testing.assert_array_equal(d_out.download(), ref)
+        d_stack.free()
+        d_out.free()
+
+
+@pytest.mark.gpu
 def test_gpu_reproject_invalid_samples(ctx, orc):
     """Invalid (-1e30) corrected pixels are skipped exactly as the reference
     does (mDIntProj, CalcReproj.cu:17-24)."""

@@ -1,0 +1,172 @@
+"""Stack correction, stretched CC, projection-matching measurement.
+
+CPU tests pin the oracle with closed forms (pixel-centre convention of the
+nearest-neighbour correction, sum binning, the R-weighting filter as an FFT
+product, shift recovery of the correlation); GPU tests compare the HIP
+kernels with the oracle on the same inputs: correction bit-exact, correlated
+shifts within 0.05 px.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+INVALID = np.float32(-1e30)
+
+
+def blobs(h, w, n=60, seed=0):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    img = np.zeros((h, w))
+    for _ in range(n):
+        cy, cx = rng.uniform(0, h), rng.uniform(0, w)
+        s = rng.uniform(1.5, 5)
+        img += rng.uniform(0.5, 1) * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2)
+                                            / (2 * s * s))
+    return (img + 0.1).astype(np.float32)
+
+
+# ------------------------------------------------------------ oracle pins
+
+def test_correct_integer_shift_convention(orc):
+    """Shift-only correction maps out[y, i] = raw[y + sy + 1, i + sx + 1]:
+    pixel centres at i + 0.5 and rounding by (int)(v + 0.5) (quirk Q11)."""
+    raw = np.arange(40 * 32, dtype=np.float32).reshape(40, 32)
+    out = orc.correct_proj(raw, 3.0, -2.0, 0.0, True, False, 1, False)
+    assert out.shape == (40, 32)
+    np.testing.assert_array_equal(out[5:30, 2:25], raw[4:29, 6:29])
+    assert np.all(out[:, -3:] == INVALID)       # x + 4 >= 32 falls outside
+
+
+def test_correct_rotation_half_turn(orc):
+    raw = np.arange(24 * 16, dtype=np.float32).reshape(24, 16)
+    # A quarter-pixel shift keeps q + 0.5 away from integers (a half turn
+    # maps pixel centres exactly onto rounding boundaries otherwise).
+    out = orc.correct_proj(raw, 0.25, 0.25, 180.0, False, False, 1, False)
+    # q = -p + 0.25 + c  ->  index (int)(W + 0.25 - i) = W - i for i >= 1.
+    np.testing.assert_array_equal(out[1:, 1:], raw[::-1, ::-1][:-1, :-1])
+
+
+def test_correct_random_fill_is_nearby_raw_pixel(orc):
+    raw = np.arange(64 * 64, dtype=np.float32).reshape(64, 64)
+    out = orc.correct_proj(raw, 30.0, 0.0, 0.0, True, True, 1, False)
+    # Columns whose source x = i + 31 >= 64 are filled from the mirrored
+    # position (x -> 63 - x % 64) within the 51 x 51 window.
+    for i in range(40, 64):
+        for y in (5, 30, 60):
+            v = int(out[y, i])
+            sy_, sx_ = divmod(v, 64)
+            x = i + 31
+            xr = 63 - (x % 64)
+            yr = y + 1 if y + 1 < 64 else 63 - ((y + 1) % 64)
+            assert abs(sx_ - xr) <= 25 and abs(sy_ - yr) <= 25
+
+
+def test_binning_sums(orc):
+    raw = np.arange(20 * 16, dtype=np.float32).reshape(20, 16)
+    out = orc.correct_proj(raw, -1.0, -1.0, 0.0, True, False, 2, False)
+    # shift -1 cancels the +1 of the pixel-centre convention: identity.
+    expect = raw.reshape(10, 2, 8, 2).sum(axis=(1, 3))
+    np.testing.assert_array_equal(out, expect)
+    # Any invalid pixel in a bin invalidates the bin sum.
+    out2 = orc.correct_proj(raw, 0.0, -1.0, 0.0, True, False, 2, False)
+    assert np.all(out2[:, -1] == INVALID)
+
+
+def test_rweight_is_ramp_filter(orc):
+    """R2C, x (1/N) ramp 2f(0.55+0.45cos 2pi f), C2R (unnormalised) equals
+    irfft(rfft(x) * ramp) -- the filter of GRWeight (CorrStack.cu:564-579)."""
+    rng = np.random.default_rng(3)
+    raw = rng.standard_normal((12, 40)).astype(np.float32)
+    out = orc.correct_proj(raw, -1.0, -1.0, 0.0, True, False, 1, True)
+    n = 40
+    f = np.arange(n // 2 + 1) / n
+    ramp = 2 * f * (0.55 + 0.45 * np.cos(2 * np.pi * f))
+    expect = np.fft.irfft(np.fft.rfft(raw.astype(np.float64), axis=1) * ramp,
+                          n, axis=1)
+    assert rel_l2(out, expect) < 1e-5
+
+
+@pytest.mark.parametrize("dx,dy", [(5, -3), (-7, 4), (0, 0)])
+def test_stretch_cc_recovers_shift(orc, dx, dy):
+    ref = blobs(96, 128, seed=1)
+    img = np.roll(np.roll(ref, dy, axis=0), dx, axis=1)
+    sx, sy = orc.stretch_cc(ref, img, 0.0, 0.0, 0.0, 400.0)
+    assert abs(sx - dx) < 0.3 and abs(sy - dy) < 0.3
+
+
+def test_pa_measure_recovers_shift(orc):
+    ref = blobs(128, 128, seed=2)
+    img = np.roll(np.roll(ref, -4, axis=0), 6, axis=1)
+    sx, sy = orc.pa_measure(ref, img)
+    assert abs(sx - 6) < 0.3 and abs(sy + 4) < 0.3
+
+
+def test_stretch_undoes_cosine_foreshortening(orc):
+    """An image compressed along x by cos(t)/cos(t_ref) about the axis 0
+    correlates best with its reference after stretching: zero shift."""
+    ref = blobs(128, 128, seed=4)
+    w = 128
+    xs = (np.arange(w) + 0.5 - w / 2) / 0.8 + w / 2 - 0.5
+    img = np.stack([np.interp(xs, np.arange(w), r) for r in ref]).astype(np.float32)
+    ang = np.degrees(np.arccos(0.8))
+    sx, sy = orc.stretch_cc(ref, img, 0.0, float(ang), 0.0, 400.0)
+    assert abs(sx) < 0.5 and abs(sy) < 0.5
+
+
+# --------------------------------------------------------------- GPU parity
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args", [
+    (3.5, -2.25, 0.0, True, True, 1, False),
+    (-10.0, 7.0, 85.3, False, True, 1, False),
+    (12.7, -3.3, 93.0, False, False, 4, False),
+    (0.4, 0.6, 84.0, False, True, 1, True),
+    (-5.5, 2.0, 86.0, False, True, 3, True),
+])
+def test_gpu_correct_matches_oracle(ctx, orc, args):
+    raw = blobs(150, 182, seed=5)
+    ref = orc.correct_proj(raw, *args)
+    d_raw = ctx.array(raw)
+    d_out = ctx.empty(ref.shape)
+    ctx.correct_proj(d_raw, 182, 150, *args, d_out)
+    got = d_out.download()
+    if args[-1]:   # R-weighted: FP32 FFT against the FP64 oracle FFT
+        assert rel_l2(got, ref) < 1e-5
+    else:          # identical FP32 arithmetic: bit-exact
+        np.testing.assert_array_equal(got, ref)
+    d_raw.free()
+    d_out.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tilts,axis", [((0.0, 0.0), 0.0), ((3.0, 6.0), 85.0),
+                                        ((-30.0, -33.0), 93.0),
+                                        ((50.0, 57.0), 80.0)])
+def test_gpu_stretch_cc_matches_oracle(ctx, orc, tilts, axis):
+    ref = blobs(128, 160, seed=6)
+    img = np.roll(ref, (3, -5), axis=(0, 1)) + \
+        0.05 * np.random.default_rng(7).standard_normal(ref.shape).astype(np.float32)
+    img = img.astype(np.float32)
+    o = orc.stretch_cc(ref, img, tilts[0], tilts[1], axis, 400.0)
+    d_ref, d_img = ctx.array(ref), ctx.array(img)
+    g = ctx.stretch_cc(d_ref, d_img, 160, 128, tilts[0], tilts[1], axis, 400.0)
+    assert abs(g[0] - o[0]) <= 0.05 and abs(g[1] - o[1]) <= 0.05, (g, o)
+    d_ref.free()
+    d_img.free()
+
+
+@pytest.mark.gpu
+def test_gpu_pa_measure_matches_oracle(ctx, orc):
+    rng = np.random.default_rng(8)
+    ref = blobs(120, 144, seed=9)
+    img = (np.roll(ref, (2, 5), axis=(0, 1)) +
+           0.1 * rng.standard_normal(ref.shape)).astype(np.float32)
+    ref2 = ref.copy()
+    ref2[:, :6] = INVALID          # invalid reprojection columns
+    o = orc.pa_measure(ref2, img)
+    d_r, d_i = ctx.array(ref2), ctx.array(img)
+    g = ctx.pa_measure(d_r, d_i, 144, 120)
+    assert abs(g[0] - o[0]) <= 0.05 and abs(g[1] - o[1]) <= 0.05, (g, o)
+    d_r.free()
+    d_i.free()
