@@ -16,9 +16,9 @@
 //   BP: rho = fma(x, cos_i, fma(z, sin_i, bx/2)); index (int)rho
 //   RP: X = fma(-t', sin, tx), Z = fma(t', cos, tz); index (int)Z, (int)X
 //
-// Kernels are instantiated for 1..kMaxFixedN reference projections so the
-// per-projection cos/sin/weight tables sit in scalar registers; larger counts
-// use the runtime-n instantiation (N = 0).
+// The fused kernel is instantiated for 1..kMaxFixedN reference projections so
+// the per-projection cos/sin/weight tables sit in scalar registers; larger
+// counts use the runtime-n instantiation (N = 0).
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -31,7 +31,8 @@ namespace mf {
 namespace {
 
 constexpr int kBpThreads = 512;
-constexpr int kBpBlock = 256;  // voxels along x per wave task (4 per lane)
+constexpr int kBpBlock = 256;  // voxels along x per wave block (4 per lane)
+constexpr int kFuThreads = 512;  // rays per fused work unit
 constexpr int kMaxFixedN = 16;
 
 struct BpParams {
@@ -39,6 +40,7 @@ struct BpParams {
   float* slab;         // [by][tb][2bx]
   int bx, by, tb, n, i0;
   float cnt_all;       // sum of the n weights in reference order
+  float rcp_all;       // 1 / cnt_all, correctly rounded
   float c[kMaxRefProj], s[kMaxRefProj], w[kMaxRefProj];
 };
 
@@ -51,34 +53,39 @@ struct RpParams {
 
 // Loads row y of the n reference projections into LDS as value*cos_i (the
 // first product of the reference's v*cos*w), with a zero guard entry at
-// index bx and a validity byte per entry.  Returns nonzero if any sample is
-// invalid (< -1e10, the reference's mDIntProj test).
-__device__ int load_sino_rows(const BpParams& p, int n, int y, float* sv,
-                              unsigned char* sm, float* tc, float* ts) {
+// index bx.  A sample the reference skips (< -1e10, its mDIntProj test) is
+// stored as -inf, which no valid finite sample times cos can equal.  Returns
+// nonzero (in every thread) if any sample of the rows is skipped.
+__device__ int load_sino_rows(const BpParams& p, int n, int y, float* sv) {
   const int bx = p.bx, rowp = bx + 1;
   const size_t plane = (size_t)bx * p.by;
+  const float* src = p.stack + (size_t)p.i0 * plane + (size_t)y * bx;
   int bad = 0;
-  for (int k = threadIdx.x; k < n * rowp; k += blockDim.x) {
-    int i = k / rowp, x = k - i * rowp;
-    float vc = 0.f;
-    unsigned char ok = 0;
-    if (x < bx) {
-      float v = p.stack[(size_t)(p.i0 + i) * plane + (size_t)y * bx + x];
-      if (v < -1e10f) {
-        bad = 1;
-      } else {
-        vc = v * p.c[i];
-        ok = 1;
+  for (int i = 0; i < n; i++) {
+    const float c = p.c[i];
+    for (int x = threadIdx.x; x < rowp; x += blockDim.x) {
+      float vc = 0.f;
+      if (x < bx) {
+        const float v = src[(size_t)i * plane + x];
+        if (v < -1e10f) {
+          bad = 1;
+          vc = -INFINITY;
+        } else {
+          vc = v * c;
+        }
       }
+      sv[i * rowp + x] = vc;
     }
-    sv[k] = vc;
-    sm[k] = ok;
-  }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    tc[i] = p.c[i];
-    ts[i] = p.s[i];
   }
   return __syncthreads_or(bad);
+}
+
+// Quotient a/d from d's correctly rounded reciprocal: q = a*r, then one
+// residual step.  With r = RN(1/d) this returns RN(a/d) for operands away
+// from overflow and the subnormal range (Markstein's division theorem).
+__device__ __forceinline__ float div_by_rcp(float a, float d, float r) {
+  const float q = a * r;
+  return fmaf(fmaf(-q, d, a), r, q);
 }
 
 // V voxels per lane with per-projection range and validity tests (the
@@ -86,7 +93,6 @@ __device__ int load_sino_rows(const BpParams& p, int n, int y, float* sv,
 template <int N, int V>
 __device__ __forceinline__ void bp_general(const BpParams& p, const float* fx,
                                            float fz, const float* sv,
-                                           const unsigned char* sm,
                                            int any_bad, float* out) {
   const int n = N ? N : p.n;
   const int bx = p.bx, rowp = bx + 1;
@@ -99,15 +105,17 @@ __device__ __forceinline__ void bp_general(const BpParams& p, const float* fx,
     const float c = p.c[i], w = p.w[i];
     const float zsc = fmaf(fz, p.s[i], cx);
     const float* row = sv + i * rowp;
-    const unsigned char* mrow = sm + i * rowp;
 #pragma unroll
     for (int v = 0; v < V; v++) {
       const float r = fmaf(fx[v], c, zsc);
       const bool inr = (r >= 0.f) && (r <= iend);
-      const int k = inr ? (int)r : bx;
+      float val = row[inr ? (int)r : bx];
       bool ok = inr;
-      if (any_bad) ok = ok && mrow[k];
-      acc[v] = fmaf(row[k], w, acc[v]);
+      if (any_bad) {
+        ok = ok && (val != -INFINITY);
+        val = ok ? val : 0.f;
+      }
+      acc[v] = fmaf(val, w, acc[v]);
       cnt[v] += ok ? w : 0.f;
     }
   }
@@ -116,75 +124,145 @@ __device__ __forceinline__ void bp_general(const BpParams& p, const float* fx,
     out[v] = (cnt[v] < 0.001f) ? kInvalid : acc[v] / cnt[v];
 }
 
-template <int N>
-__global__ __launch_bounds__(kBpThreads) void k_bp_slab(const BpParams p) {
+// Per-projection constants of one target in lane registers: lane i < n
+// holds cos_i, sin_i and w_i (lanes >= n repeat projection 0; their bits are
+// masked out of every ballot).
+struct LaneProj {
+  float c, s, w;
+  unsigned long long mask;  // bits 0..n-1
+  __device__ LaneProj(const BpParams& p, int n, int lane) {
+    const int i = lane < n ? lane : 0;
+    c = p.c[i];
+    s = p.s[i];
+    w = p.w[i];
+    mask = n >= 64 ? ~0ull : ((1ull << n) - 1);
+  }
+};
+
+__device__ __forceinline__ float lane_f(float v, int i) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+}
+
+// Persistent backprojection.  Workgroup g owns the contiguous range
+// [g*per, (g+1)*per) of slab rows (y, iz) in y-major order and loads the
+// sinogram rows of each y it meets into LDS once; its waves take whole slab
+// rows (iz) and walk them in blocks of kBpBlock voxels along x.  For each
+// block and projection, rho's monotonicity in x classifies the projection as
+// in range over the whole block, out of range over the whole block, or
+// mixed, from the block's two end voxels:
+//   - no projection in range: every voxel has weight 0 -> invalid;
+//   - all in range, no skipped samples: no per-voxel tests, and the constant
+//     weight sum divides through its correctly rounded reciprocal;
+//   - otherwise: out-of-range projections are skipped (the reference's
+//     'continue'), in-range ones need no test, mixed ones test each voxel.
+// Every voxel accumulates the same terms in the same order as the reference
+// loop (CalcReproj.cu:37-63), so the slab is bit-identical to it.
+__global__ __launch_bounds__(kBpThreads) void k_bp_slab(const BpParams p,
+                                                        int per) {
   extern __shared__ float smem[];
-  const int n = N ? N : p.n;
-  const int y = blockIdx.x;
+  constexpr int V = kBpBlock / 64;
+  const int n = p.n;
   const int bx = p.bx, rowp = bx + 1;
   float* sv = smem;
-  float* tc = sv + n * rowp;
-  float* ts = tc + kMaxRefProj;
-  unsigned char* sm = (unsigned char*)(ts + kMaxRefProj);
-  const int any_bad = load_sino_rows(p, n, y, sv, sm, tc, ts);
-
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
   const int vx = 2 * bx;
-  const int nblk = (vx + kBpBlock - 1) / kBpBlock;
   const float cx = (float)bx * 0.5f, iend = (float)(bx - 1);
   const float hvx = (float)vx * 0.5f, htb = (float)p.tb * 0.5f;
-  float* out_row = p.slab + (size_t)y * p.tb * vx;
+  const LaneProj lp(p, n, lane);
+  const bool all_fast_ok = p.cnt_all >= 0.001f;
+  const int total = p.by * p.tb;
+  const int qa = blockIdx.x * per, qb = min(total, qa + per);
 
-  for (int task = wave; task < p.tb * nblk; task += nwaves) {
-    const int iz = task / nblk, b = task - iz * nblk;
-    const int ix0 = b * kBpBlock;
-    const int ixl = min(ix0 + kBpBlock - 1, vx - 1);
-    const float fz = (float)iz + 0.5f - htb;
-    // Classify the block: every projection in range over it (fast), none in
-    // range anywhere in it (empty), or mixed.  rho is monotone in x.
-    bool inall = true, outall = true;
-    if (lane < n) {
-      const float zsc = fmaf(fz, ts[lane], cx);
-      const float rf = fmaf((float)ix0 + 0.5f - hvx, tc[lane], zsc);
-      const float rl = fmaf((float)ixl + 0.5f - hvx, tc[lane], zsc);
-      inall = (rf >= 0.f) && (rl <= iend);
-      outall = (rl < 0.f) || (rf > iend);
-    }
-    const bool fast = !any_bad && (__ballot(inall) == ~0ull);
-    const bool empty = (__ballot(outall) == ~0ull);
-    float* orow = out_row + (size_t)iz * vx;
-    float fx[4], res[4];
-    int ixs[4];
+  for (int q0 = qa; q0 < qb;) {
+    const int y = q0 / p.tb;
+    const int za = q0 - y * p.tb;
+    const int zb = min(p.tb, za + (qb - q0));
+    q0 += zb - za;
+    __syncthreads();  // the previous row's waves are done with the LDS rows
+    const int any_bad = load_sino_rows(p, n, y, sv);
+    for (int iz = za + wave; iz < zb; iz += nwaves) {
+      const float fz = (float)iz + 0.5f - htb;
+      const float lz = fmaf(fz, lp.s, cx);  // lane i: zsc of projection i
+      float* orow = p.slab + ((size_t)y * p.tb + iz) * vx + lane;
+      for (int ix0 = 0; ix0 < vx; ix0 += kBpBlock) {
+        const int ixl = min(ix0 + kBpBlock - 1, vx - 1);
+        const float rf = fmaf((float)ix0 + 0.5f - hvx, lp.c, lz);
+        const float rl = fmaf((float)ixl + 0.5f - hvx, lp.c, lz);
+        const unsigned long long m_in =
+            __ballot(rf >= 0.f && rl <= iend) & lp.mask;
+        const unsigned long long m_out =
+            __ballot(rl < 0.f || rf > iend) & lp.mask;
+        float fx[V], res[V];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      ixs[v] = ix0 + lane + 64 * v;
-      fx[v] = (float)min(ixs[v], ixl) + 0.5f - hvx;
-    }
-    if (empty) {
+        for (int v = 0; v < V; v++)
+          fx[v] = (float)min(ix0 + lane + 64 * v, ixl) + 0.5f - hvx;
+        if (m_out == lp.mask) {
 #pragma unroll
-      for (int v = 0; v < 4; v++) res[v] = kInvalid;
-    } else if (fast) {
-      float acc[4] = {0.f, 0.f, 0.f, 0.f};
+          for (int v = 0; v < V; v++) res[v] = kInvalid;
+        } else if (m_in == lp.mask && !any_bad && all_fast_ok) {
+          float acc[V];
 #pragma unroll
-      for (int i = 0; i < n; i++) {
-        const float c = p.c[i], w = p.w[i];
-        const float zsc = fmaf(fz, p.s[i], cx);
-        const float* row = sv + i * rowp;
+          for (int v = 0; v < V; v++) acc[v] = 0.f;
+#pragma unroll 1
+          for (int i = 0; i < n; i++) {
+            const float c = lane_f(lp.c, i), w = lane_f(lp.w, i);
+            const float zsc = lane_f(lz, i);
+            const float* row = sv + i * rowp;
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const float r = fmaf(fx[v], c, zsc);
-          acc[v] = fmaf(row[(int)r], w, acc[v]);
+            for (int v = 0; v < V; v++)
+              acc[v] = fmaf(row[(int)fmaf(fx[v], c, zsc)], w, acc[v]);
+          }
+#pragma unroll
+          for (int v = 0; v < V; v++)
+            res[v] = div_by_rcp(acc[v], p.cnt_all, p.rcp_all);
+        } else {
+          float acc[V], cnt[V];
+#pragma unroll
+          for (int v = 0; v < V; v++) acc[v] = cnt[v] = 0.f;
+          const unsigned long long m_fast = any_bad ? 0ull : m_in;
+#pragma unroll 1
+          for (int i = 0; i < n; i++) {
+            if ((m_out >> i) & 1ull) continue;
+            const float c = lane_f(lp.c, i), w = lane_f(lp.w, i);
+            const float zsc = lane_f(lz, i);
+            const float* row = sv + i * rowp;
+            if ((m_fast >> i) & 1ull) {
+#pragma unroll
+              for (int v = 0; v < V; v++) {
+                acc[v] = fmaf(row[(int)fmaf(fx[v], c, zsc)], w, acc[v]);
+                cnt[v] += w;
+              }
+            } else {
+#pragma unroll
+              for (int v = 0; v < V; v++) {
+                const float r = fmaf(fx[v], c, zsc);
+                const bool inr = (r >= 0.f) && (r <= iend);
+                float val = row[inr ? (int)r : bx];
+                bool ok = inr;
+                if (any_bad) {
+                  ok = ok && (val != -INFINITY);
+                  val = ok ? val : 0.f;
+                }
+                acc[v] = fmaf(val, w, acc[v]);
+                cnt[v] += ok ? w : 0.f;
+              }
+            }
+          }
+#pragma unroll
+          for (int v = 0; v < V; v++)
+            res[v] = (cnt[v] < 0.001f) ? kInvalid : acc[v] / cnt[v];
+        }
+        if (ix0 + kBpBlock <= vx) {
+#pragma unroll
+          for (int v = 0; v < V; v++) orow[ix0 + 64 * v] = res[v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; v++)
+            if (ix0 + lane + 64 * v < vx) orow[ix0 + 64 * v] = res[v];
         }
       }
-#pragma unroll
-      for (int v = 0; v < 4; v++) res[v] = acc[v] / p.cnt_all;
-    } else {
-      bp_general<N, 4>(p, fx, fz, sv, sm, any_bad, res);
     }
-#pragma unroll
-    for (int v = 0; v < 4; v++)
-      if (ixs[v] < vx) orow[ixs[v]] = res[v];
   }
 }
 
@@ -243,50 +321,170 @@ __global__ __launch_bounds__(256) void k_rp_slab(const RpParams q) {
       ((float)cnt < 0.8f * (float)q.L) ? kInvalid : (float)(sum / (double)cnt);
 }
 
+__device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Sample interval [ta, tb) of one ray on which every sample lies inside the
+// slab and every reference projection is in range for the sample's voxel,
+// with margins that cover the voxel snapping (|rho - g| <= (|c|+|s|)/2) and
+// the FP32 rounding of the sample coordinates (~1e-3 voxel).  Along the ray
+// tau = t + z0 the exact coordinates are X = tx - tau s, Z = tz + tau c, and
+// rho_i's continuous counterpart g_i is linear in tau, so each condition is
+// a tau interval.  Evaluated in FP64; the caller intersects it with the
+// ray's own range.
+__device__ void fast_interval(const BpParams& p, const RpParams& q, int n,
+                              float tx, float tz, float z0, int* ta, int* tb) {
+  const double vx = 2.0 * p.bx, hvx = p.bx, htb = 0.5 * p.tb;
+  const double s = q.s, c = q.c;
+  double lo = -1e30, hi = 1e30;
+  auto clip = [&](double a, double b, double mn, double mx) {
+    // a + b*tau in [mn, mx]
+    if (fabs(b) < 1e-12) {
+      if (a < mn || a > mx) {
+        lo = 1e30;
+        hi = -1e30;
+      }
+      return;
+    }
+    double t1 = (mn - a) / b, t2 = (mx - a) / b;
+    lo = fmax(lo, fmin(t1, t2));
+    hi = fmin(hi, fmax(t1, t2));
+  };
+  clip(tx, -s, 2.0, vx - 3.0);             // X inside the slab
+  clip(tz, c, 2.0, (double)p.tb - 3.0);    // Z inside the slab
+#pragma unroll 1
+  for (int i = 0; i < n; i++) {
+    const double ci = p.c[i], si = p.s[i];
+    const double a = (tx - hvx) * ci + (tz - htb) * si + 0.5 * p.bx;
+    clip(a, -s * ci + c * si, 2.0, (double)p.bx - 3.0);
+  }
+  if (!(lo <= hi)) {
+    *ta = *tb = 0;
+    return;
+  }
+  const double zf = z0;
+  *ta = (int)fmax(-1e9, ceil(lo - zf)) + 1;
+  *tb = (int)fmin(1e9, floor(hi - zf));
+}
+
 // Fused form: each ray sample evaluates the backprojection of the voxel it
 // falls in directly from the LDS-cached sinogram rows, so no slab reaches
-// HBM.  The voxel value uses the same operations as k_bp_slab's general
-// path, hence the output equals the materialised path bit for bit.
+// HBM.  Persistent: workgroup g owns the units [g*per, (g+1)*per) of
+// (image row, chunk of kFuThreads rays), reloading the sinogram rows when
+// the row changes.  Each ray is split at its wave's common fast interval:
+// outside it the sample takes the reference's tests (slab bounds, range,
+// skipped samples, empty voxel); inside it none apply.  Either way the voxel
+// value is computed with the same operations and order as k_bp_slab and the
+// ray sum visits the samples in increasing t, so the output equals the
+// materialised path bit for bit.
 template <int N>
-__global__ __launch_bounds__(256) void k_bprp_fused(const BpParams p,
-                                                    const RpParams q) {
+__global__ __launch_bounds__(kFuThreads) void k_bprp_fused(const BpParams p,
+                                                           const RpParams q,
+                                                           int per) {
   extern __shared__ float smem[];
   const int n = N ? N : p.n;
-  const int y = blockIdx.y;
   const int bx = p.bx, rowp = bx + 1;
   float* sv = smem;
-  float* tc = sv + n * rowp;
-  float* ts = tc + kMaxRefProj;
-  unsigned char* sm = (unsigned char*)(ts + kMaxRefProj);
-  const int any_bad = load_sino_rows(p, n, y, sv, sm, tc, ts);
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= bx) return;
   const int vx = 2 * bx;
+  const float cx = (float)bx * 0.5f;
   const float hvx = (float)vx * 0.5f, htb = (float)p.tb * 0.5f;
-  const float xp = (float)u + 0.5f - (float)bx * 0.5f;
-  const float tx = fmaf(xp, q.c, (float)vx * 0.5f);
-  const float tz = fmaf(xp, q.s, (float)q.tb * 0.5f);
-  const float z0 = (-xp * q.s) / q.c - 0.5f * (float)q.L;
   const float iex = (float)(vx - 1), iez = (float)(q.tb - 1);
-  int t0, t1;
-  rp_sample_range(q, tx, tz, z0, &t0, &t1);
-  double sum = 0.0;
-  int cnt = 0;
-  for (int t = t0; t < t1; t++) {
-    const float fz = (float)t + z0;
-    const float X = fmaf(-fz, q.s, tx);
-    const float Z = fmaf(fz, q.c, tz);
-    const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
-    const float vfx = (float)(inb ? (int)X : 0) + 0.5f - hvx;
-    const float vfz = (float)(inb ? (int)Z : 0) + 0.5f - htb;
-    float v;
-    bp_general<N, 1>(p, &vfx, vfz, sv, sm, any_bad, &v);
-    const bool ok = inb && (v >= -1e10f);
-    sum += ok ? (double)v : 0.0;
-    cnt += ok ? 1 : 0;
+  const float iend = (float)(bx - 1);
+  const LaneProj lp(p, n, threadIdx.x & 63);
+  // (float)k + 0.5f - h == (float)k + (0.5f - h) exactly for the integer
+  // voxel indices and half-sizes here (all terms are small half-integers).
+  const float ox = 0.5f - hvx, oz = 0.5f - htb;
+  const bool fast_ok = p.cnt_all >= 0.001f;
+  const int nch = cdiv(bx, kFuThreads);
+  const int units = p.by * nch;
+  const int ua = blockIdx.x * per, ub = min(units, ua + per);
+  int ycur = -1, any_bad = 0;
+  for (int un = ua; un < ub; un++) {
+    const int y = un / nch;
+    if (y != ycur) {
+      __syncthreads();  // all waves are done with the previous row
+      any_bad = load_sino_rows(p, n, y, sv);
+      ycur = y;
+    }
+    const int u = (un - y * nch) * kFuThreads + threadIdx.x;
+    const bool live = u < bx;
+    const int uu = live ? u : bx - 1;  // idle lanes shadow a real ray
+    const float xp = (float)uu + 0.5f - (float)bx * 0.5f;
+    const float tx = fmaf(xp, q.c, (float)vx * 0.5f);
+    const float tz = fmaf(xp, q.s, (float)q.tb * 0.5f);
+    const float z0 = (-xp * q.s) / q.c - 0.5f * (float)q.L;
+    int t0, t1;
+    rp_sample_range(q, tx, tz, z0, &t0, &t1);
+    int fa, fb;
+    fast_interval(p, q, p.n, tx, tz, z0, &fa, &fb);
+    fa = max(fa, t0);
+    fb = min(fb, t1);
+    // The wave's common fast interval.
+    const int ta = wave_max(fa), tb = wave_min(fb);
+    const bool has_fast = fast_ok && !any_bad && ta < tb;
+    const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
+    double sum = 0.0;
+    int cnt = 0;
+    // General samples: [t0, a) and [b, t1); per-projection constants from
+    // lane registers, not unrolled (a minority of the samples).
+    auto general = [&](int from, int to) {
+      for (int t = from; t < to; t++) {
+        const float fz = (float)t + z0;
+        const float X = fmaf(-fz, q.s, tx);
+        const float Z = fmaf(fz, q.c, tz);
+        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const float vfx = (float)(inb ? (int)X : 0) + ox;
+        const float vfz = (float)(inb ? (int)Z : 0) + oz;
+        float acc = 0.f, wsum = 0.f;
+#pragma unroll 1
+        for (int i = 0; i < n; i++) {
+          const float ci = lane_f(lp.c, i), si = lane_f(lp.s, i);
+          const float wi = lane_f(lp.w, i);
+          const float r = fmaf(vfx, ci, fmaf(vfz, si, cx));
+          const bool inr = (r >= 0.f) && (r <= iend);
+          float val = sv[i * rowp + (inr ? (int)r : bx)];
+          bool okr = inr;
+          if (any_bad) {
+            okr = okr && (val != -INFINITY);
+            val = okr ? val : 0.f;
+          }
+          acc = fmaf(val, wi, acc);
+          wsum += okr ? wi : 0.f;
+        }
+        const float v = (wsum < 0.001f) ? kInvalid : acc / wsum;
+        const bool ok = inb && (v >= -1e10f);
+        sum += ok ? (double)v : 0.0;
+        cnt += ok ? 1 : 0;
+      }
+    };
+    general(t0, a);
+    if (has_fast) {
+      for (int t = a; t < b; t++) {
+        const float fz = (float)t + z0;
+        const float vfx = (float)(int)fmaf(-fz, q.s, tx) + ox;
+        const float vfz = (float)(int)fmaf(fz, q.c, tz) + oz;
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < n; i++) {
+          const float zsc = fmaf(vfz, p.s[i], cx);
+          acc = fmaf(sv[i * rowp + (int)fmaf(vfx, p.c[i], zsc)], p.w[i], acc);
+        }
+        sum += (double)div_by_rcp(acc, p.cnt_all, p.rcp_all);
+      }
+      cnt += b - a;
+    }
+    general(b, t1);
+    if (live)
+      q.out[(size_t)y * bx + u] = ((float)cnt < 0.8f * (float)q.L)
+                                      ? kInvalid
+                                      : (float)(sum / (double)cnt);
   }
-  q.out[(size_t)y * bx + u] =
-      ((float)cnt < 0.8f * (float)q.L) ? kInvalid : (float)(sum / (double)cnt);
 }
 
 __global__ void k_fill(float* p, size_t n, float v) {
@@ -295,33 +493,26 @@ __global__ void k_fill(float* p, size_t n, float v) {
 }
 
 size_t bp_lds_bytes(int n, int bx) {
-  size_t rowp = (size_t)bx + 1;
-  return (size_t)n * rowp * 4 + 2 * kMaxRefProj * 4 + (size_t)n * rowp;
+  return (size_t)n * ((size_t)bx + 1) * 4;
 }
 
-typedef void (*BpKernel)(BpParams);
-typedef void (*FusedKernel)(BpParams, RpParams);
+typedef void (*FusedKernel)(BpParams, RpParams, int);
 
 template <int N>
 struct KernelTable {
-  static void fill(BpKernel* bp, FusedKernel* fu) {
-    bp[N] = k_bp_slab<N>;
+  static void fill(FusedKernel* fu) {
     fu[N] = k_bprp_fused<N>;
-    KernelTable<N - 1>::fill(bp, fu);
+    KernelTable<N - 1>::fill(fu);
   }
 };
 template <>
 struct KernelTable<0> {
-  static void fill(BpKernel* bp, FusedKernel* fu) {
-    bp[0] = k_bp_slab<0>;
-    fu[0] = k_bprp_fused<0>;
-  }
+  static void fill(FusedKernel* fu) { fu[0] = k_bprp_fused<0>; }
 };
 
 struct Kernels {
-  BpKernel bp[kMaxFixedN + 1];
   FusedKernel fu[kMaxFixedN + 1];
-  Kernels() { KernelTable<kMaxFixedN>::fill(bp, fu); }
+  Kernels() { KernelTable<kMaxFixedN>::fill(fu); }
 };
 
 const Kernels& kernels() {
@@ -402,6 +593,7 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
     cnt += bp.w[k];
   }
   bp.cnt_all = cnt;
+  bp.rcp_all = n > 0 ? 1.0f / cnt : 0.f;
 
   RpParams rp{};
   rp.out = d_reproj;
@@ -424,13 +616,20 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
       return;
     }
     FusedKernel k = kernels().fu[kid];
-    dim3 grid(cdiv(bx, 256), by);
     if (lds > 64 * 1024)
       MF_HIP(hipFuncSetAttribute((const void*)k,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
+    // One resident wave of workgroups over the (row, ray chunk) units.
+    int occ = 0;
+    MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k,
+                                                        kFuThreads, lds));
+    const long long units = (long long)by * cdiv(bx, kFuThreads);
+    const long long slots = (long long)std::max(occ, 1) * std::max(ctx->num_cu, 1);
+    const int per = cdiv(units, std::min(units, slots));
     TimedLaunch tl(ctx->timer, "bprp_fused", st);
-    hipLaunchKernelGGL(k, grid, dim3(256), lds, st, bp, rp);
+    hipLaunchKernelGGL(k, dim3(cdiv(units, per)), dim3(kFuThreads), lds, st,
+                       bp, rp, per);
     MF_HIP(hipGetLastError());
     return;
   }
@@ -443,13 +642,22 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
     hipLaunchKernelGGL(k_fill, dim3(4096), dim3(256), 0, st, slab, slab_elems,
                        kInvalid);
   } else {
-    BpKernel k = kernels().bp[kid];
+    const void* k = (const void*)k_bp_slab;
     if (lds > 64 * 1024)
-      MF_HIP(hipFuncSetAttribute((const void*)k,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+      MF_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));
+    // One resident wave of workgroups, each owning an equal share of the
+    // by*tb slab rows.
+    int occ = 0;
+    MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, kBpThreads,
+                                                        lds));
+    const long long rows = (long long)by * tb;
+    const long long slots = (long long)std::max(occ, 1) * std::max(ctx->num_cu, 1);
+    const int per = (int)cdiv(rows, std::min(rows, slots));
+    const int grid = (int)cdiv(rows, per);
     TimedLaunch tl(ctx->timer, "bp_slab", st);
-    hipLaunchKernelGGL(k, dim3(by), dim3(kBpThreads), lds, st, bp);
+    hipLaunchKernelGGL(k_bp_slab, dim3(grid), dim3(kBpThreads), lds, st, bp,
+                       per);
   }
   MF_HIP(hipGetLastError());
   {

@@ -62,7 +62,9 @@ size_t stretch_cc_work_floats(int w, int h, int npair) {
 
 namespace {
 
-constexpr int kRed = 1024;
+// Partial blocks per moment job and per peak map; the combining step uses
+// one 64-lane wave, one partial per lane.
+constexpr int kParts = 64;
 
 // Pixel of an unpadded w-wide image read through the padded pitch the
 // reference uses (pad columns hold zero here; the reference leaves them
@@ -127,28 +129,34 @@ struct MomJob {
   int pitch;
 };
 
-// Jobs come from a device table, or for two-job launches from the kernel
-// arguments (jobs == nullptr).
-__global__ __launch_bounds__(kRed) void k_moments_jobs(const MomJob* jobs,
-                                                       MomJob j0, MomJob j1,
-                                                       int w, int h,
-                                                       double* out) {
+// Partial (sum, sum of squares) of valid pixels over a band of rows: grid
+// (kParts, njobs).  Jobs come from a device table, or for two-job launches
+// from the kernel arguments (jobs == nullptr).  Each job's kParts partials
+// are combined in a fixed order by k_norm_mask, so results are reproducible.
+__global__ __launch_bounds__(256) void k_moments_part(const MomJob* jobs,
+                                                      MomJob j0, MomJob j1,
+                                                      int w, int h,
+                                                      double* part) {
   __shared__ double sh[16];
-  const MomJob j = jobs ? jobs[blockIdx.x] : (blockIdx.x ? j1 : j0);
+  const MomJob j = jobs ? jobs[blockIdx.y] : (blockIdx.y ? j1 : j0);
+  const int rows = cdiv(h, kParts);
+  const int ya = blockIdx.x * rows, yb = min(h, ya + rows);
   double s1 = 0.0, s2 = 0.0;
-  const int n = w * h;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    int y = i / w, x = i - y * w;
-    float v = j.src[(size_t)y * j.pitch + x];
-    if (v < -1e10f) continue;
-    s1 += v;
-    s2 += (double)(v * v);
+  for (int y = ya; y < yb; y++) {
+    const float* row = j.src + (size_t)y * j.pitch;
+    for (int x = threadIdx.x; x < w; x += blockDim.x) {
+      const float v = row[x];
+      if (v < -1e10f) continue;
+      s1 += v;
+      s2 += (double)(v * v);
+    }
   }
   s1 = block_reduce(s1, sh, [](double a, double b) { return a + b; }, 0.0);
   s2 = block_reduce(s2, sh, [](double a, double b) { return a + b; }, 0.0);
   if (threadIdx.x == 0) {
-    out[2 * blockIdx.x] = s1;
-    out[2 * blockIdx.x + 1] = s2;
+    double* o = part + ((size_t)blockIdx.y * kParts + blockIdx.x) * 2;
+    o[0] = s1;
+    o[1] = s2;
   }
 }
 
@@ -164,21 +172,37 @@ struct NormJob {
 __global__ __launch_bounds__(256) void k_norm_mask(const NormJob* jobs,
                                                    NormJob j0, NormJob j1,
                                                    int w, int h,
-                                                   const double* mom,
+                                                   const double* part,
                                                    const float* mask,
                                                    int unit_std) {
+  __shared__ float sh_ms[2];
+  const NormJob j = jobs ? jobs[blockIdx.y] : (blockIdx.y ? j1 : j0);
+  if (threadIdx.x < 64) {
+    // kParts (= 64) partials of this job, summed in a fixed tree order.
+    const double* p = part + ((size_t)j.mom * kParts + threadIdx.x) * 2;
+    double s1 = p[0], s2 = p[1];
+    for (int o = 32; o > 0; o >>= 1) {
+      s1 += __shfl_down(s1, o, 64);
+      s2 += __shfl_down(s2, o, 64);
+    }
+    if (threadIdx.x == 0) {
+      const double npx = (double)w * (double)h;
+      const float mean = (float)(s1 / npx);
+      float sd = 1.0f;
+      if (!unit_std) {
+        const float m2 = (float)(s2 / npx);
+        const float var = m2 - mean * mean;
+        sd = var <= 0.f ? 0.0f : sqrtf(var);
+      }
+      sh_ms[0] = mean;
+      sh_ms[1] = sd;
+    }
+  }
+  __syncthreads();
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= w * h) return;
   const int y = idx / w, x = idx - y * w;
-  const NormJob j = jobs ? jobs[blockIdx.y] : (blockIdx.y ? j1 : j0);
-  const double npx = (double)w * (double)h;
-  const float mean = (float)(mom[2 * j.mom] / npx);
-  float sd = 1.0f;
-  if (!unit_std) {
-    const float m2 = (float)(mom[2 * j.mom + 1] / npx);
-    const float var = m2 - mean * mean;
-    sd = var <= 0.f ? 0.0f : sqrtf(var);
-  }
+  const float mean = sh_ms[0], sd = sh_ms[1];
   const float v = j.src[(size_t)y * j.src_pitch + x];
   const float mk = mask[idx];
   float o = 0.0f;
@@ -227,53 +251,75 @@ __device__ __forceinline__ Best best_of(Best a, Best b) {
   return a;
 }
 
-// Peak search and parabolic sub-pixel fit on one correlation map per block.
-// mode 0: CalcCC::FindPeak (CalcCC.cu:381-435); mode 1: PAFindPeak
-// (ProjAlign.cpp:325-375).  First maximum in row-major order wins.
-__global__ __launch_bounds__(kRed) void k_peak(const float* maps, int w, int h,
-                                               size_t img_stride, int mode,
-                                               float* out) {
-  __shared__ float shv[16];
-  __shared__ int shi[16];
-  const int pitch = pad_x(w);
-  const float* cc = maps + (size_t)blockIdx.x * img_stride;
+// Search window of the peak: mode 0 CalcCC::FindPeak (CalcCC.cu:381-435),
+// mode 1 PAFindPeak (ProjAlign.cpp:325-375).
+struct PeakWin {
   int x0, y0, x1, y1;
   float init;
-  if (mode == 0) {
-    const int sx = w * 8 / 20 * 2, sy = h * 8 / 20 * 2;
-    x0 = (w - sx) / 2;
-    y0 = (h - sy) / 2;
-    x1 = x0 + sx;
-    y1 = y0 + sy;
-    init = -1e30f;
-  } else {
-    x0 = max(w / 20, 3);
-    y0 = max(h / 20, 3);
-    x1 = w - x0;
-    y1 = h - y0;
-    init = -1e20f;
+  __device__ PeakWin(int w, int h, int mode) {
+    if (mode == 0) {
+      const int sx = w * 8 / 20 * 2, sy = h * 8 / 20 * 2;
+      x0 = (w - sx) / 2;
+      y0 = (h - sy) / 2;
+      x1 = x0 + sx;
+      y1 = y0 + sy;
+      init = -1e30f;
+    } else {
+      x0 = max(w / 20, 3);
+      y0 = max(h / 20, 3);
+      x1 = w - x0;
+      y1 = h - y0;
+      init = -1e20f;
+    }
   }
-  const int ww = x1 - x0, n = ww * (y1 - y0);
-  Best b{init, 0x7fffffff};
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int y = y0 + i / ww, x = x0 + i % ww;
-    const float v = cc[(size_t)y * pitch + x];
-    if (v > b.v) b = Best{v, y * w + x};
-  }
-  // Block argmax with smallest-index tie break.
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+};
+
+// Wave argmax with the smallest row-major index winning ties.
+__device__ __forceinline__ Best wave_best(Best b) {
   for (int o = 32; o > 0; o >>= 1) {
     Best c{__shfl_down(b.v, o, 64), __shfl_down(b.i, o, 64)};
     b = best_of(b, c);
   }
-  if (lane == 0) {
-    shv[wv] = b.v;
-    shi[wv] = b.i;
-  }
+  return b;
+}
+
+// Stage 1: argmax over a band of the window's rows, grid (kParts, nmaps).
+__global__ __launch_bounds__(256) void k_peak_part(const float* maps, int w,
+                                                   int h, size_t img_stride,
+                                                   int mode, Best* part) {
+  __shared__ Best sh[4];
+  const int pitch = pad_x(w);
+  const float* cc = maps + (size_t)blockIdx.y * img_stride;
+  const PeakWin win(w, h, mode);
+  const int rows = cdiv(win.y1 - win.y0, kParts);
+  const int ya = win.y0 + blockIdx.x * rows, yb = min(win.y1, ya + rows);
+  Best b{win.init, 0x7fffffff};
+  for (int y = ya; y < yb; y++)
+    for (int x = win.x0 + threadIdx.x; x < win.x1; x += blockDim.x) {
+      const float v = cc[(size_t)y * pitch + x];
+      if (v > b.v) b = Best{v, y * w + x};
+    }
+  b = wave_best(b);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sh[wv] = b;
   __syncthreads();
   if (threadIdx.x != 0) return;
-  for (int k = 1; k < (int)(blockDim.x >> 6); k++) b = best_of(b, Best{shv[k], shi[k]});
-  int px = mode == 0 ? 0 : x0, py = mode == 0 ? 0 : y0;
+  for (int k = 1; k < (int)(blockDim.x >> 6); k++) b = best_of(b, sh[k]);
+  part[(size_t)blockIdx.y * kParts + blockIdx.x] = b;
+}
+
+// Stage 2 (one wave per map): combine the partials, then the parabolic
+// sub-pixel fit of the reference (FP32 terms in mode 0, FP64 in mode 1).
+__global__ __launch_bounds__(64) void k_peak_fit(const float* maps, int w,
+                                                 int h, size_t img_stride,
+                                                 int mode, const Best* part,
+                                                 float* out) {
+  const int pitch = pad_x(w);
+  const float* cc = maps + (size_t)blockIdx.x * img_stride;
+  const PeakWin win(w, h, mode);
+  Best b = wave_best(part[(size_t)blockIdx.x * kParts + threadIdx.x]);
+  if (threadIdx.x != 0) return;
+  int px = mode == 0 ? 0 : win.x0, py = mode == 0 ? 0 : win.y0;
   if (b.i != 0x7fffffff) {
     px = b.i % w;
     py = b.i / w;
@@ -301,6 +347,17 @@ __global__ __launch_bounds__(kRed) void k_peak(const float* maps, int w, int h,
   const float fpx = (float)(px + dx), fpy = (float)(py + dy);
   out[2 * blockIdx.x] = fpx - (float)(w / 2);
   out[2 * blockIdx.x + 1] = fpy - (float)(h / 2);
+}
+
+void find_peaks(mf_ctx* ctx, hipStream_t st, const float* maps, int w, int h,
+                size_t img_stride, int nmaps, int mode, Best* part,
+                float* out) {
+  TimedLaunch tl(ctx->timer, "cc_peak", st);
+  hipLaunchKernelGGL(k_peak_part, dim3(kParts, nmaps), dim3(256), 0, st, maps,
+                     w, h, img_stride, mode, part);
+  hipLaunchKernelGGL(k_peak_fit, dim3(nmaps), dim3(64), 0, st, maps, w, h,
+                     img_stride, mode, (const Best*)part, out);
+  MF_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -337,7 +394,7 @@ void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   MF_HIP(hipGetLastError());
   const MomJob m0{};
   const NormJob n0{};
-  hipLaunchKernelGGL(k_moments_jobs, dim3(2 * npair), dim3(kRed), 0, st,
+  hipLaunchKernelGGL(k_moments_part, dim3(kParts, 2 * npair), dim3(256), 0, st,
                      (const MomJob*)(dt + b0), m0, m0, w, h, d_mom);
   const float* mask = cc_mask(ctx, w, h, false);
   hipLaunchKernelGGL(k_norm_mask, dim3(cdiv((long long)w * h, 256), 2 * npair),
@@ -352,13 +409,15 @@ void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
                      img / 2, bfactor);
   MF_HIP(hipGetLastError());
   fft_c2r_2d(ctx, st, B, w, h, npair, img);
-  {
-    TimedLaunch tl(ctx->timer, "cc_peak", st);
-    hipLaunchKernelGGL(k_peak, dim3(npair), dim3(kRed), 0, st, B, w, h, img, 0,
-                       d_shift);
-  }
-  MF_HIP(hipGetLastError());
+  find_peaks(ctx, st, B, w, h, img, npair, 0,
+             (Best*)(d_mom + (size_t)4 * npair * kParts), d_shift);
   MF_HIP(hipStreamSynchronize(st));  // 'tab' is reusable after return
+}
+
+size_t cc_scratch_doubles(int npair) {
+  // Moment partials: 2 jobs x kParts x (sum, sum of squares) per pair; peak
+  // partials: kParts x (value, index) per pair.
+  return (size_t)5 * npair * kParts;
 }
 
 void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
@@ -370,8 +429,8 @@ void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
   float* B = d_work + img;
   const MomJob m0{d_reproj, w}, m1{d_proj, w};
   const NormJob n0{d_reproj, A, w, 0}, n1{d_proj, B, w, 1};
-  hipLaunchKernelGGL(k_moments_jobs, dim3(2), dim3(kRed), 0, st, nullptr, m0,
-                     m1, w, h, d_mom);
+  hipLaunchKernelGGL(k_moments_part, dim3(kParts, 2), dim3(256), 0, st,
+                     nullptr, m0, m1, w, h, d_mom);
   const float* mask = cc_mask(ctx, w, h, true);
   hipLaunchKernelGGL(k_norm_mask, dim3(cdiv((long long)w * h, 256), 2),
                      dim3(256), 0, st, nullptr, n0, n1, w, h, d_mom, mask, 1);
@@ -382,12 +441,8 @@ void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
                      img / 2, 500.0f);
   MF_HIP(hipGetLastError());
   fft_c2r_2d(ctx, st, B, w, h, 1, img);
-  {
-    TimedLaunch tl(ctx->timer, "cc_peak", st);
-    hipLaunchKernelGGL(k_peak, dim3(1), dim3(kRed), 0, st, B, w, h, img, 1,
-                       d_ishift);
-  }
-  MF_HIP(hipGetLastError());
+  find_peaks(ctx, st, B, w, h, img, 1, 1, (Best*)(d_mom + (size_t)4 * kParts),
+             d_ishift);
 }
 
 }  // namespace mf

@@ -37,7 +37,9 @@ constexpr float kBPD2R = 0.01745f;
 constexpr float kInvalid = -1e30f;
 
 __host__ __device__ inline int pad_x(int nx) { return (nx / 2 + 1) * 2; }
-inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+__host__ __device__ inline int cdiv(long long a, long long b) {
+  return (int)((a + b - 1) / b);
+}
 
 // Float cos/sin of a float argument evaluated in double on the host; every
 // index-deciding trigonometric value reaches the kernels through these.

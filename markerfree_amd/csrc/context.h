@@ -79,6 +79,7 @@ const void* ctx_table(mf_ctx* ctx, const std::string& key,
 
 struct mf_ctx {
   int device = 0;
+  int num_cu = 0;         // compute units of the device (persistent grids)
   hipStream_t stream = nullptr;
   size_t budget = 0;      // bytes the context may hold (0: unlimited)
   size_t in_use = 0;      // bytes currently allocated through the context

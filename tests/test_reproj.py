@@ -232,6 +232,37 @@ def test_gpu_reproject_bitexact_fp32(ctx, orc, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("bx,by,tb", [(512, 6, 150), (1024, 3, 300)])
+def test_gpu_fast_paths_bitexact_wide(ctx, orc, bx, by, tb):
+    """Production widths, where most slab blocks and ray samples take the
+    kernels' test-free fast paths: the slab and both reprojection modes are
+    bit-identical to the oracle's FP32 mode, for targets across the tilt
+    range (full, partial and single-neighbour ranges)."""
+    stack, tilts = _series(bx, by, 61, 2.0, seed=bx)
+    nz = 61
+    z0 = int(np.argmin(np.abs(tilts)))
+    d_stack = ctx.array(stack)
+    d_out = ctx.empty((by, bx))
+    d_slab = ctx.empty((by, tb, 2 * bx))
+    for k in (1, 6, 20, 30):
+        for target in (z0 + k, z0 - k):
+            skip = np.ones(nz, np.uint8)
+            skip[max(0, z0 - k + 1): z0 + k] = 0
+            skip[target] = 1
+            ref, slab = orc.reproject(stack, tb, tilts, skip, target, 10,
+                                      acc64=False, want_slab=True)
+            ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
+                          d_out, mode=0, d_slab=d_slab)
+            np.testing.assert_array_equal(d_slab.download(), slab)
+            np.testing.assert_array_equal(d_out.download(), ref)
+            ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
+                          d_out, mode=1)
+            np.testing.assert_array_equal(d_out.download(), ref)
+    for d in (d_stack, d_out, d_slab):
+        d.free()
+
+
+@pytest.mark.gpu
 def test_gpu_reproject_invalid_samples(ctx, orc):
     """Invalid (-1e30) corrected pixels are skipped exactly as the reference
     does (mDIntProj, CalcReproj.cu:17-24)."""
