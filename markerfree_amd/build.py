@@ -19,6 +19,7 @@ CSRC = PKG / "csrc"
 OBJ = ROOT / "_build" / "obj"
 LIB = PKG / "lib" / "libmarkerfree_hip.so"
 SYNTH = PKG / "lib" / "libmf_synth.so"
+IO = PKG / "lib" / "libmf_io.so"
 CLI = PKG / "bin" / "markerfree"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -34,9 +35,9 @@ DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
 HOST = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-Wall",
         "-Wno-unused-result"]
 
-# Sources of the shared library (the CLI and the synthetic generator are
-# separate artefacts).
-EXCLUDE = {"cli.cpp", "synth.cpp"}
+# Sources of the shared library (the command line under csrc/cli and the
+# synthetic generator are separate artefacts).
+EXCLUDE = {"synth.cpp"}
 
 
 def _sources():
@@ -82,13 +83,15 @@ def build(jobs: int | None = None, verbose: bool = False) -> None:
     # Synthetic tilt-series generator: host C++ with OpenMP.
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
           f"-I{ROOT / 'include'}", str(CSRC / "synth.cpp"), "-o", str(SYNTH)])
-    # Drop-in CLI linked against the library.
-    if (CSRC / "cli.cpp").exists():
-            _run(["g++", *COMMON, *HOST, str(CSRC / "cli.cpp"), "-o", str(CLI),
-              f"-L{LIB.parent}", "-lmarkerfree_hip",
-              "-Wl,-rpath,$ORIGIN/../lib"])
+    # File formats of the command line (host only), then the drop-in CLI
+    # linked against both libraries.
+    _run(["g++", *COMMON, "-Wall", "-shared", str(CSRC / "cli" / "io.cpp"),
+          "-o", str(IO)])
+    _run(["g++", *COMMON, "-Wall", str(CSRC / "cli" / "cli.cpp"), "-o",
+          str(CLI), f"-L{LIB.parent}", "-lmarkerfree_hip", "-lmf_io",
+          "-Wl,-rpath,$ORIGIN/../lib"])
     if verbose:
-        print(f"built {LIB}, {SYNTH}, {CLI}")
+        print(f"built {LIB}, {SYNTH}, {IO}, {CLI}")
 
 
 if __name__ == "__main__":

@@ -124,3 +124,24 @@ def generate(cfg: SeriesConfig) -> Series:
         raise ValueError("mfs_generate rejected the configuration")
     return Series(stack, tilts, sx, sy, float(axis.value), float(scale.value),
                   cfg)
+
+
+def write_series(series: Series, directory, name: str = "series",
+                 descending: bool = False):
+    """Writes the series as the command line reads it: an MRC stack (mode 1
+    int16 for count data, else mode 2 float32) and a .rawtlt list, in file
+    order (reversed when 'descending').  Returns (stack_path, tilt_path)."""
+    from pathlib import Path
+
+    from . import fileio
+
+    d = Path(directory)
+    d.mkdir(parents=True, exist_ok=True)
+    stack, tilts = series.stack, series.tilts
+    if descending:
+        stack, tilts = stack[::-1], tilts[::-1]
+    data = stack.astype(np.int16) if series.config.int16 else stack
+    mrc, tlt = d / f"{name}.mrc", d / f"{name}.rawtlt"
+    fileio.write_mrc(mrc, np.ascontiguousarray(data))
+    fileio.write_rawtlt(tlt, tilts)
+    return mrc, tlt
