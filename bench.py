@@ -11,8 +11,16 @@ final corrected stack in host memory.  Raw stacks are resident in HBM before
 the timed region starts (each step gets its own series).
 
 Prints one JSON line with the series/hour, the backprojection kernel's
-roofline (algorithmic bytes per launch over its HIP-event-timed duration)
-and a CPU baseline (the oracle on a bounded sample, extrapolated).
+roofline and a CPU baseline (the oracle on a bounded sample, extrapolated).
+
+Roofline: achieved = the backprojection's algorithmic bytes per target,
+4*(2bx*T*by) slab bytes + 4*n*bx*by sinogram bytes, over the average
+HIP-event-timed launch of the kernel that does the backprojection.  In the
+default fused mode that kernel (bprp_fused) also does the ray sums and never
+writes the slab, so 'achieved' is the rate at which it delivers the
+backprojection the materialised form would write to HBM; 'traffic' (from
+the committed FETCH_SIZE/WRITE_SIZE counter summary) shows the bytes it
+actually moves.  --mode materialized times bp_slab, which writes the slab.
 """
 from __future__ import annotations
 
@@ -29,6 +37,20 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+CALIB_BYTES = 1 << 30
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of 'kernel' from the committed counter summary
+    (profiles/traffic_<kernel>_series_1024x1024x61_T300.md, line
+    'traffic_bytes_per_launch: N'), or None when absent."""
+    p = ROOT / "profiles" / f"traffic_{kernel}_series_1024x1024x61_T300.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1]))
+    return None
 
 
 def pa_sweep_bytes(mf, tilts, bx, by, tb, nproj):
@@ -120,6 +142,16 @@ def main():
         s.run(opts)
         return s.correct(1)
 
+    # Counter calibration: one device-to-device copy of a known size (1 GiB,
+    # above the 256 MB last-level cache) outside the timed region; under
+    # `rocprofv3 --pmc FETCH_SIZE` / `WRITE_SIZE` its dispatch gives the
+    # counters' reading for known bytes.
+    calib = [ctx.empty((CALIB_BYTES // 4,)) for _ in range(2)]
+    calib[1].copy_from(calib[0])
+    ctx.synchronize()
+    for a in calib:
+        a.free()
+
     for s in pool[:args.warmup]:
         step(s)
     ctx.synchronize()
@@ -179,7 +211,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(kname),
             "avg_launch_ms": round(avg_ms, 4),
             "gvoxel_per_s": round(vox_per_launch / (avg_ms * 1e-3) / 1e9, 1),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),

@@ -124,6 +124,14 @@ class DeviceArray:
                                     self.nbytes))
         return self
 
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy of 'other' (same size) on the context's
+        stream."""
+        assert other.nbytes == self.nbytes
+        self.ctx._check(_lib.mf_d2d(self.ctx.h, self.ptr, other.ptr,
+                                    self.nbytes))
+        return self
+
     def download(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         self.ctx._check(_lib.mf_d2h(self.ctx.h, out.ctypes.data, self.ptr,

@@ -129,6 +129,31 @@ struct MomJob {
   int pitch;
 };
 
+struct NormJob {
+  const float* src;
+  float* dst;
+  int src_pitch;
+  int mom;  // index of the (sum, sum of squares) pair
+};
+
+// Job b from a device table, or for two-job launches (jobs == nullptr) from
+// the kernel arguments.  The argument structs are selected field by field:
+// selecting a whole by-value argument struct makes the compiler copy both
+// into per-thread scratch memory.
+__device__ __forceinline__ MomJob pick_job(const MomJob* jobs,
+                                           MomJob j0, MomJob j1,
+                                           unsigned b) {
+  if (jobs) return jobs[b];
+  return MomJob{b ? j1.src : j0.src, b ? j1.pitch : j0.pitch};
+}
+__device__ __forceinline__ NormJob pick_job(const NormJob* jobs,
+                                            NormJob j0,
+                                            NormJob j1, unsigned b) {
+  if (jobs) return jobs[b];
+  return NormJob{b ? j1.src : j0.src, b ? j1.dst : j0.dst,
+                 b ? j1.src_pitch : j0.src_pitch, b ? j1.mom : j0.mom};
+}
+
 // Partial (sum, sum of squares) of valid pixels over a band of rows: grid
 // (kParts, njobs).  Jobs come from a device table, or for two-job launches
 // from the kernel arguments (jobs == nullptr).  Each job's kParts partials
@@ -138,7 +163,7 @@ __global__ __launch_bounds__(256) void k_moments_part(const MomJob* jobs,
                                                       int w, int h,
                                                       double* part) {
   __shared__ double sh[16];
-  const MomJob j = jobs ? jobs[blockIdx.y] : (blockIdx.y ? j1 : j0);
+  const MomJob j = pick_job(jobs, j0, j1, blockIdx.y);
   const int rows = cdiv(h, kParts);
   const int ya = blockIdx.x * rows, yb = min(h, ya + rows);
   double s1 = 0.0, s2 = 0.0;
@@ -160,13 +185,6 @@ __global__ __launch_bounds__(256) void k_moments_part(const MomJob* jobs,
   }
 }
 
-struct NormJob {
-  const float* src;
-  float* dst;
-  int src_pitch;
-  int mom;  // index of the (sum, sum of squares) pair
-};
-
 // mNormalize / PANorm normalisation (GNormalize, CalcCC.cu:137-147) fused
 // with GRoundEdge (mask table): invalid pixels become 0.
 __global__ __launch_bounds__(256) void k_norm_mask(const NormJob* jobs,
@@ -176,7 +194,7 @@ __global__ __launch_bounds__(256) void k_norm_mask(const NormJob* jobs,
                                                    const float* mask,
                                                    int unit_std) {
   __shared__ float sh_ms[2];
-  const NormJob j = jobs ? jobs[blockIdx.y] : (blockIdx.y ? j1 : j0);
+  const NormJob j = pick_job(jobs, j0, j1, blockIdx.y);
   if (threadIdx.x < 64) {
     // kParts (= 64) partials of this job, summed in a fixed tree order.
     const double* p = part + ((size_t)j.mom * kParts + threadIdx.x) * 2;
