@@ -173,6 +173,18 @@ def tilt_axis(stack, tilts, sx, sy, rot, arange, num=100):
     return float(axis), scores, int(L.value)
 
 
+_sig("orc_offset_correlate", C.c_float, _fp, _fp, C.c_int, C.c_int,
+     C.c_float, C.c_float)
+
+
+def offset_correlate(ref, img, ref_tilt, tilt):
+    ref = np.ascontiguousarray(ref, np.float32)
+    img = np.ascontiguousarray(img, np.float32)
+    h, w = ref.shape
+    return float(_lib.orc_offset_correlate(_f(ref), _f(img), w, h, ref_tilt,
+                                           tilt))
+
+
 def find_offset(stack, tilts, sx, sy, rot):
     stack = np.ascontiguousarray(stack, np.float32)
     nz, ny, nx = stack.shape
@@ -205,6 +217,7 @@ _sig("orc_series_find_offset", C.c_float, _vp)
 _sig("orc_apply_offset", None, _vp)
 _sig("orc_projalign_setup", None, _vp, C.c_float, C.c_int, C.c_int)
 _sig("orc_projalign_iter", C.c_float, _vp)
+_sig("orc_pa_z0", C.c_float, _vp)
 _sig("orc_projalign_once", None, _vp)
 _sig("orc_correct_final", None, _vp, C.c_int, _fp, _ip)
 _sig("orc_run", None, _vp, C.POINTER(Options))
@@ -264,6 +277,9 @@ class Series:
 
     def find_offset(self):
         return float(_lib.orc_series_find_offset(self.h))
+
+    def pa_z0(self):
+        return float(_lib.orc_pa_z0(self.h))
 
     def apply_offset(self):
         _lib.orc_apply_offset(self.h)

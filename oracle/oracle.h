@@ -85,6 +85,9 @@ float orc_tilt_axis(const float* stack, int nx, int ny, int nz,
 float orc_find_offset(const float* stack, int nx, int ny, int nz,
                       const float* tilts, const float* shift_x,
                       const float* shift_y, const float* rotate, float* ccs);
+/* FindOffset::Correlate (FindOffset.cu:204-266) of one w x h pair. */
+float orc_offset_correlate(const float* ref, const float* img, int w, int h,
+                           float ref_tilt, float tilt);
 
 /* ---- whole pipeline (Process.cpp) ---- */
 typedef struct {
@@ -111,6 +114,8 @@ float orc_series_find_offset(orc_series*);
 void orc_apply_offset(orc_series*);
 void orc_projalign_setup(orc_series*, float zshift, int align_z, int nproj);
 float orc_projalign_iter(orc_series*);
+/* Rotation-centre height from the last FitRotCenterZ (ProjAlign.cpp:459-486). */
+float orc_pa_z0(const orc_series*);
 void orc_projalign_once(orc_series*);
 void orc_correct_final(orc_series*, int out_bin, float* out, int* out_size);
 void orc_run(orc_series*, const orc_options*);

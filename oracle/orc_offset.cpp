@@ -99,3 +99,9 @@ extern "C" float orc_find_offset(const float* stack, int nx, int ny, int nz,
                 false, corr, &s);
   return offset_search(corr.data(), s.bx, s.by, nz, tilts, ccs);
 }
+
+extern "C" float orc_offset_correlate(const float* ref, const float* img,
+                                      int w, int h, float ref_tilt,
+                                      float tilt) {
+  return offset_correlate(ref, img, w, h, ref_tilt, tilt);
+}

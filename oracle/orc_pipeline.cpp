@@ -289,6 +289,8 @@ extern "C" void orc_projalign_setup(orc_series* S, float zshift, int align_z,
            S->pa_sizes.by);
 }
 
+extern "C" float orc_pa_z0(const orc_series* S) { return S->pa_z0; }
+
 extern "C" float orc_projalign_iter(orc_series* S) {
   // ProjAlign::DoIt / DoItBin (ProjAlign.cpp:205-257).
   const int nz = S->nz;
