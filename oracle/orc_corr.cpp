@@ -54,6 +54,9 @@ void correct_one(const float* raw, const CorrSizes& s, float shift_x,
   std::vector<float> corr((size_t)s.pad * ny, 0.0f);
   const float hx = (float)s.cx * 0.5f, hy = (float)ny * 0.5f;
   const float ox = shift_x + (float)nx * 0.5f, oy = shift_y + (float)ny * 0.5f;
+  // Rows in parallel when called for one image; inside correct_stack's
+  // per-image loop the nested region runs on one thread.
+#pragma omp parallel for schedule(static)
   for (int y = 0; y < ny; y++) {
     for (int i = 0; i < s.cx; i++) {
       // mGCorrect (CorrStack.cu:275-312).
@@ -76,6 +79,7 @@ void correct_one(const float* raw, const CorrSizes& s, float shift_x,
   }
   // mGBinImage (CorrStack.cu:173-196): b x b sum, -1e30 if any invalid.
   std::vector<float> bin((size_t)s.padb * s.by, 0.0f);
+#pragma omp parallel for schedule(static)
   for (int y = 0; y < s.by; y++) {
     for (int x = 0; x < s.bx; x++) {
       float sum = 0.0f;
@@ -104,15 +108,19 @@ void correct_one(const float* raw, const CorrSizes& s, float shift_x,
       float r = (float)k / fn;
       filt[k] = 2.0f * r * (0.55f + 0.45f * fcos(6.2831852f * r));
     }
-    std::vector<float> cmp(2 * nc);
-    for (int y = 0; y < s.by; y++) {
-      float* row = bin.data() + (size_t)y * s.padb;
-      fft_r2c_line(row, n, cmp.data());
-      for (int k = 0; k < nc; k++) {
-        cmp[2 * k] = cmp[2 * k] / fn * filt[k];
-        cmp[2 * k + 1] = cmp[2 * k + 1] / fn * filt[k];
+#pragma omp parallel
+    {
+      std::vector<float> cmp(2 * nc);
+#pragma omp for schedule(static)
+      for (int y = 0; y < s.by; y++) {
+        float* row = bin.data() + (size_t)y * s.padb;
+        fft_r2c_line(row, n, cmp.data());
+        for (int k = 0; k < nc; k++) {
+          cmp[2 * k] = cmp[2 * k] / fn * filt[k];
+          cmp[2 * k + 1] = cmp[2 * k + 1] / fn * filt[k];
+        }
+        fft_c2r_line(cmp.data(), n, row);
       }
-      fft_c2r_line(cmp.data(), n, row);
     }
   }
   // CorrUtil::Unpad (CorrStack.cu:158-168).
