@@ -203,6 +203,53 @@ mf_status mf_correct(mf_series* s, int out_bin, float* out, int* bx, int* by);
  * alignment, axis and offset searches, projection matching. */
 mf_status mf_run(mf_series* s, const mf_options* o);
 
+/* ------------------------------------------- batched and served forms
+ * No counterpart in the reference, which aligns one series per process;
+ * these run many series on one GPU.  Each series' results equal what
+ * mf_run computes for it alone. */
+
+/* mf_run on n series of one context (sizes and tilt counts may differ),
+ * 'concurrency' at a time: each series has its own stream, so the kernels
+ * of different series overlap on the device. */
+mf_status mf_batch_run(mf_series** series, int n, const mf_options* o,
+                       int concurrency);
+
+/* Upper bound of the device bytes one series of this shape holds from
+ * mf_series_create through mf_correct (the served form's admission
+ * charge). */
+size_t mf_series_hbm_bytes(int nx, int ny, int nz, const mf_options* o);
+
+/* Request loop: series arrive with mf_submit, wait in arrival order, and
+ * start when a worker is free and their byte bound fits next to the
+ * running series within 'hbm_budget' (0: the context's budget, else the
+ * device memory free at creation less 1/16).  Raw stacks are copied at
+ * submission; results (parameters and the OutBin aligned stack) stay in
+ * host memory until released. */
+typedef struct mf_server mf_server;
+enum { MF_REQ_QUEUED = 0, MF_REQ_RUNNING = 1, MF_REQ_DONE = 2,
+       MF_REQ_FAILED = 3 };
+mf_status mf_server_create(mf_ctx* ctx, int workers, size_t hbm_budget,
+                           int queue_cap, mf_server** out);
+/* Waits for running series; queued ones fail with "server stopped". */
+void mf_server_destroy(mf_server* sv);
+/* MF_EBUSY: 'queue_cap' requests already wait; MF_ENOMEM: the series could
+ * never fit the budget. */
+mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
+                    const float* stack_zyx, const float* tilts,
+                    const mf_options* o, long long* id);
+mf_status mf_poll(mf_server* sv, long long id, int* state);
+mf_status mf_wait(mf_server* sv, long long id, int* state);
+/* Results of a finished request (any pointer may be NULL); times3 =
+ * submit, start, finish in seconds since the server started.  A failed
+ * request returns MF_EDEVICE with its message in err. */
+mf_status mf_fetch(mf_server* sv, long long id, double* shift_x,
+                   double* shift_y, float* rotate, float* angle_offset,
+                   float* aligned, int* bx, int* by, double* times3,
+                   char* err, size_t err_size);
+mf_status mf_release(mf_server* sv, long long id);
+mf_status mf_server_stats(mf_server* sv, size_t* budget, size_t* reserved,
+                          int* queued, int* running, long long* completed);
+
 #ifdef __cplusplus
 }
 #endif

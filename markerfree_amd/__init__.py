@@ -14,7 +14,10 @@ from pathlib import Path
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "lib" / "libmarkerfree_hip.so"
+# MARKERFREE_HIP_LIB selects another build of the same library (e.g. a
+# kernel variant under development); the default is the in-tree build.
+LIB_PATH = Path(os.environ.get("MARKERFREE_HIP_LIB",
+                               PKG / "lib" / "libmarkerfree_hip.so"))
 SYNTH_PATH = PKG / "lib" / "libmf_synth.so"
 CLI_PATH = PKG / "bin" / "markerfree"
 
@@ -409,3 +412,140 @@ class Series:
     def run(self, options: Options = None, **kw):
         o = options if options is not None else default_options(**kw)
         self._check(_lib.mf_run(self.h, C.byref(o)))
+
+
+# ------------------------------------------------ batched and served forms
+
+_sig("mf_batch_run", C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(Options),
+     C.c_int)
+_sig("mf_series_hbm_bytes", C.c_size_t, C.c_int, C.c_int, C.c_int,
+     C.POINTER(Options))
+_sig("mf_server_create", C.c_int, _vp, C.c_int, C.c_size_t, C.c_int,
+     C.POINTER(_vp))
+_sig("mf_server_destroy", None, _vp)
+_sig("mf_submit", C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
+     C.POINTER(C.c_float), C.POINTER(Options), C.POINTER(C.c_longlong))
+_sig("mf_poll", C.c_int, _vp, C.c_longlong, C.POINTER(C.c_int))
+_sig("mf_wait", C.c_int, _vp, C.c_longlong, C.POINTER(C.c_int))
+_sig("mf_fetch", C.c_int, _vp, C.c_longlong, C.POINTER(C.c_double),
+     C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_float),
+     C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_int),
+     C.POINTER(C.c_double), C.c_char_p, C.c_size_t)
+_sig("mf_release", C.c_int, _vp, C.c_longlong)
+_sig("mf_server_stats", C.c_int, _vp, C.POINTER(C.c_size_t),
+     C.POINTER(C.c_size_t), C.POINTER(C.c_int), C.POINTER(C.c_int),
+     C.POINTER(C.c_longlong))
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(4)
+
+
+def series_hbm_bytes(nx, ny, nz, options: Options = None, **kw) -> int:
+    o = options if options is not None else default_options(**kw)
+    return int(_lib.mf_series_hbm_bytes(nx, ny, nz, C.byref(o)))
+
+
+def batch_run(series, options: Options = None, concurrency=4, **kw):
+    """mf_batch_run: mf_run on every series, 'concurrency' at a time."""
+    o = options if options is not None else default_options(**kw)
+    arr = (_vp * len(series))(*[s.h for s in series])
+    st = _lib.mf_batch_run(arr, len(series), C.byref(o), concurrency)
+    if st:
+        msgs = [_lib.mf_series_last_error(s.h).decode() for s in series]
+        raise MFError(st, "; ".join(m for m in msgs if m))
+
+
+class Result:
+    def __init__(self, shift_x, shift_y, rotate, offset, aligned, times):
+        self.shift_x, self.shift_y, self.rotate = shift_x, shift_y, rotate
+        self.offset, self.aligned = offset, aligned
+        self.t_submit, self.t_start, self.t_done = times
+
+    @property
+    def latency(self):
+        return self.t_done - self.t_submit
+
+
+class Server:
+    """mf_server: series submitted as they arrive, admitted in order against
+    an HBM budget, run concurrently on 'workers' streams."""
+
+    def __init__(self, ctx: Context, workers=2, hbm_budget=0, queue_cap=64):
+        h = _vp()
+        ctx._check(_lib.mf_server_create(ctx.h, workers, hbm_budget, queue_cap,
+                                         C.byref(h)))
+        self.ctx, self.h = ctx, h.value
+        self._shapes = {}
+        ctx._series.add(self)
+
+    def submit(self, stack, tilts, options: Options = None, **kw) -> int:
+        o = options if options is not None else default_options(**kw)
+        stack = np.ascontiguousarray(stack, np.float32)
+        t = np.ascontiguousarray(tilts, np.float32)
+        nz, ny, nx = stack.shape
+        rid = C.c_longlong()
+        st = _lib.mf_submit(self.h, nx, ny, nz, f32p(stack), f32p(t),
+                            C.byref(o), C.byref(rid))
+        if st:
+            raise MFError(st, {MF_EBUSY: "queue full",
+                               MF_ENOMEM: "series exceeds the HBM budget"}
+                          .get(st, "submit failed"))
+        self._shapes[rid.value] = nz
+        return rid.value
+
+    def poll(self, rid) -> int:
+        s = C.c_int()
+        self.ctx._check(_lib.mf_poll(self.h, rid, C.byref(s)))
+        return s.value
+
+    def wait(self, rid) -> int:
+        s = C.c_int()
+        self.ctx._check(_lib.mf_wait(self.h, rid, C.byref(s)))
+        return s.value
+
+    def fetch(self, rid, aligned=True) -> Result:
+        nz = self._shapes[rid]
+        sx, sy = np.empty(nz), np.empty(nz)
+        rot = np.empty(nz, np.float32)
+        off = C.c_float()
+        bx, by = C.c_int(), C.c_int()
+        times = np.empty(3)
+        err = C.create_string_buffer(512)
+        fp64 = C.POINTER(C.c_double)
+        st = _lib.mf_fetch(self.h, rid, sx.ctypes.data_as(fp64),
+                           sy.ctypes.data_as(fp64), f32p(rot), C.byref(off),
+                           None, C.byref(bx), C.byref(by),
+                           times.ctypes.data_as(fp64), err, 512)
+        if st:
+            raise MFError(st, err.value.decode() or "fetch failed")
+        out = None
+        if aligned:
+            out = np.empty((nz, by.value, bx.value), np.float32)
+            st = _lib.mf_fetch(self.h, rid, None, None, None, None, f32p(out),
+                               None, None, None, err, 512)
+            if st:
+                raise MFError(st, err.value.decode())
+        return Result(sx, sy, rot, off.value, out, tuple(times))
+
+    def release(self, rid):
+        self.ctx._check(_lib.mf_release(self.h, rid))
+        self._shapes.pop(rid, None)
+
+    def stats(self):
+        b, r = C.c_size_t(), C.c_size_t()
+        q, n, c = C.c_int(), C.c_int(), C.c_longlong()
+        self.ctx._check(_lib.mf_server_stats(self.h, C.byref(b), C.byref(r),
+                                             C.byref(q), C.byref(n),
+                                             C.byref(c)))
+        return {"budget": b.value, "reserved": r.value, "queued": q.value,
+                "running": n.value, "completed": c.value}
+
+    def close(self):
+        if self.h:
+            _lib.mf_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

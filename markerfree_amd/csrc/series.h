@@ -61,4 +61,8 @@ void series_projalign_once(mf_series* s);
 void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by);
 void series_run(mf_series* s, const mf_options* o);
 
+// Upper bound of the device bytes one series holds through series_run and
+// series_correct (admission control of the served form).
+size_t series_hbm_bytes(int nx, int ny, int nz, const mf_options* o);
+
 }  // namespace mf

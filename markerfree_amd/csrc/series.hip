@@ -491,6 +491,33 @@ void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by) {
   MF_HIP(hipStreamSynchronize(s->st));
 }
 
+size_t series_hbm_bytes(int nx, int ny, int nz, const mf_options* o) {
+  const size_t raw = (size_t)4 * nx * ny * nz;
+  // Coarse alignment: corrected stack at the Transform bin plus the CC work
+  // of all adjacent pairs.
+  const int tb = std::max(std::max((int)(nx / 1024.0f + 0.1f),
+                                   (int)(ny / 1024.0f + 0.1f)), 1);
+  const CorrGeom gc = CorrGeom::make(nx, ny, tb);
+  const size_t coarse = (size_t)4 * gc.bx * gc.by * nz +
+                        stretch_cc_work_floats(gc.bx, gc.by, nz) * 4;
+  // Projection matching: corrected and R-weighted stacks, slab if
+  // materialised.
+  const int pb = std::max(std::max((int)(nx / 2048.0f + 0.5f),
+                                   (int)(ny / 2048.0f + 0.5f)), 1);
+  const CorrGeom gp = CorrGeom::make(nx, ny, pb);
+  size_t pa = (size_t)4 * nz * gp.by * (gp.bx + pad_x(gp.bx)) +
+              (size_t)4 * 3 * gp.by * pad_x(gp.bx);
+  if (o->reproj_mode == 0)
+    pa += (size_t)4 * 2 * gp.bx * gp.by * (o->align_z / pb / 2 * 2);
+  // Common lines (lines and spectra of every image and candidate), offset
+  // scan at bin 4, final correction, small tables.
+  const size_t axis = (size_t)8 * nz * 100 * (std::min(nx, ny) + 2);
+  const CorrGeom go = CorrGeom::make(nx, ny, std::max(o->out_bin, 1));
+  const size_t out = (size_t)4 * go.bx * go.by * nz;
+  // Stage buffers stay allocated for the life of the series: sum them.
+  return 2 * raw + coarse + pa + axis + out + raw / 8 + ((size_t)64 << 20);
+}
+
 void series_run(mf_series* s, const mf_options* o) {
   // Process::DoIt (Process.cpp:34-92) minus file I/O.
   s->verbose = o->verbose;

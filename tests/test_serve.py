@@ -1,0 +1,104 @@
+"""Batched and served forms: several series of different shapes run
+concurrently on one GPU, and every result is identical to the result of the
+same series run alone (deterministic kernels, no device state shared between
+series).  Admission and queue limits are exercised from the host."""
+import numpy as np
+import pytest
+
+
+def _mix():
+    from markerfree_amd import synth
+    shapes = [(256, 256, 41, 3.0, 85.3), (320, 224, 35, 3.5, 80.0),
+              (224, 288, 29, 4.0, 92.0)]
+    out = []
+    for i, (nx, ny, nz, step, axis) in enumerate(shapes):
+        cfg = synth.SeriesConfig(nx=nx, ny=ny, nz=nz, vol=(nx, ny, 64),
+                                 nblobs=250, tilt_step=step,
+                                 tilt_min=-step * (nz // 2),
+                                 axis=(axis, axis), shift_max=8.0, snr=0.5,
+                                 seeds=(40 + i, 50 + i, 60 + i))
+        out.append(synth.generate(cfg))
+    return out
+
+
+OPTS = dict(align_z=64, nproj=10)
+
+
+def _alone(mf, ctx, s):
+    G = mf.Series(ctx, s.stack, s.tilts)
+    G.run(**OPTS)
+    res = G.params(), G.correct(1)
+    G.close()
+    return res
+
+
+def _same(a, b):
+    (sx, sy, rot, off), ali = a
+    (sx2, sy2, rot2, off2), ali2 = b
+    np.testing.assert_array_equal(sx, sx2)
+    np.testing.assert_array_equal(sy, sy2)
+    np.testing.assert_array_equal(rot, rot2)
+    assert off == off2
+    np.testing.assert_array_equal(ali, ali2)
+
+
+@pytest.mark.gpu
+def test_gpu_batch_equals_alone(ctx, mf):
+    mix = _mix()
+    alone = [_alone(mf, ctx, s) for s in mix]
+    series = [mf.Series(ctx, s.stack, s.tilts) for s in mix] * 1
+    mf.batch_run(series, concurrency=3, **OPTS)
+    for G, ref in zip(series, alone):
+        _same((G.params(), G.correct(1)), ref)
+        G.close()
+
+
+@pytest.mark.gpu
+def test_gpu_server_mix_equals_alone(ctx, mf):
+    """Requests submitted back to back overlap on the device (three
+    workers); each fetched result equals the series run alone."""
+    mix = _mix()
+    alone = [_alone(mf, ctx, s) for s in mix]
+    with mf.Server(ctx, workers=3) as sv:
+        ids = [sv.submit(s.stack, s.tilts, **OPTS) for s in mix * 2]
+        for rid in ids:
+            assert sv.wait(rid) == mf.REQ_DONE
+        for k, rid in enumerate(ids):
+            r = sv.fetch(rid)
+            _same(((r.shift_x, r.shift_y, r.rotate, r.offset), r.aligned),
+                  alone[k % 3])
+            assert r.t_submit <= r.t_start <= r.t_done
+            sv.release(rid)
+        st = sv.stats()
+        assert st["completed"] == 6 and st["reserved"] == 0
+
+
+@pytest.mark.gpu
+def test_gpu_server_admission_and_limits(ctx, mf):
+    """With a budget that holds one series at a time the requests run one
+    after another (never two at once) and still give the same results; a
+    series larger than the budget is refused at submission, and a full
+    queue answers MF_EBUSY."""
+    mix = _mix()
+    s0 = mix[0]
+    need = mf.series_hbm_bytes(256, 256, 41, **OPTS)
+    ref = _alone(mf, ctx, s0)
+    with mf.Server(ctx, workers=3, hbm_budget=int(need * 1.5),
+                   queue_cap=2) as sv:
+        ids = [sv.submit(s0.stack, s0.tilts, **OPTS) for _ in range(2)]
+        with pytest.raises(mf.MFError) as e:
+            for _ in range(4):      # the queue holds at most 2 waiting
+                ids.append(sv.submit(s0.stack, s0.tilts, **OPTS))
+        assert e.value.code == mf.MF_EBUSY
+        big = np.zeros((41, 1024, 1024), np.float32)
+        with pytest.raises(mf.MFError) as e:
+            sv.submit(big, s0.tilts, **OPTS)
+        assert e.value.code == mf.MF_ENOMEM
+        for rid in ids:
+            assert sv.wait(rid) == mf.REQ_DONE
+        rs = [sv.fetch(rid) for rid in ids]
+        for r in rs:
+            _same(((r.shift_x, r.shift_y, r.rotate, r.offset), r.aligned), ref)
+        spans = sorted((r.t_start, r.t_done) for r in rs)
+        for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
+            assert b0 >= a1 - 1e-6        # serialised by the budget
