@@ -140,6 +140,134 @@ def cpu_baseline(series, pa_targets, pa_sweeps, n_targets=20):
                        f"{per_series:.0f} s/series")}
 
 
+def mixture(synth, n, seed=7):
+    """Series of the batched configuration's mixture: sizes from {1024^2,
+    2048^2}, tilt counts U{35..61}, ranges +-U[50, 60] degrees, thickness
+    {150, 300}, NPROJ 10, seeded per series.  (The 4092x5760 size of the
+    mixture is left out of this bounded run.)"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        side = int(rng.choice([1024, 2048]))
+        nz = int(rng.integers(35, 62))
+        rmax = float(rng.uniform(50, 60))
+        step = 2 * rmax / (nz - 1)
+        cfg = synth.SeriesConfig(nx=side, ny=side, nz=nz,
+                                 vol=(side, side, 300),
+                                 nblobs=3000 * (side // 1024) ** 2,
+                                 tilt_min=-rmax, tilt_step=step,
+                                 axis=(80.0, 95.0), shift_max=40.0, snr=0.05,
+                                 int16=True, seeds=(100 + i, 200 + i, 300 + i))
+        out.append((synth.generate(cfg), int(rng.choice([150, 300]))))
+    return out
+
+
+def bench_batched(mf, synth, args):
+    """The batched form on a mixture: every series aligned and its stack
+    downloaded, 'concurrency' series at a time (mf_batch_run), against the
+    same series one after another (the plain per-series form)."""
+    mix = mixture(synth, args.batch)
+    ctx = mf.Context(0)
+    groups = {}
+    for s, tb in mix:
+        groups.setdefault(tb, []).append(s)
+
+    def run(concurrency):
+        pool = [(mf.Series(ctx, s.stack, s.tilts), tb) for s, tb in mix]
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for tb in sorted(groups):
+            ser = [g for g, t in pool if t == tb]
+            mf.batch_run(ser, concurrency=concurrency, align_z=tb, nproj=10)
+        for g, _ in pool:
+            g.correct(1)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        params = [g.params() for g, _ in pool]
+        for g, _ in pool:
+            g.close()
+        return dt, params
+
+    run(1)                                  # warm-up (tables, plans)
+    t_plain, p_plain = run(1)
+    t_batch, p_batch = run(args.concurrency)
+    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+               for a, b in zip(p_plain, p_batch))
+    ctx.close()
+    n = len(mix)
+    out = {
+        "metric": "tilt series aligned/hour (batched form)",
+        "value": round(3600.0 * n / t_batch, 2), "unit": "series/hour",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic mixture (seeded)",
+        "config": {"workload": f"{n} series, sizes {{1024^2, 2048^2}}, "
+                               "tilts U{35..61}, range +-U[50,60], T "
+                               "{150,300}, NPROJ 10",
+                   "concurrency": args.concurrency,
+                   "sizes": [[s.stack.shape[2], s.stack.shape[1],
+                              s.stack.shape[0], tb] for s, tb in mix]},
+        "plain_same_mix_series_per_hour": round(3600.0 * n / t_plain, 2),
+        "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "results_identical_to_plain": bool(same),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_served(mf, synth, args):
+    """The served form: 1024^2 x 61 series (T=300, NPROJ 10) arriving as a
+    Poisson stream at each rate; latency = submission to results in host
+    memory (parameters and aligned stack)."""
+    cfg = synth.PRESETS["single"]
+    base = synth.generate(cfg)
+    opts = dict(align_z=300, nproj=10)
+    ctx = mf.Context(0)
+    rows = []
+    with mf.Server(ctx, workers=args.concurrency,
+                   queue_cap=args.requests + 1) as sv:
+        rid = sv.submit(base.stack, base.tilts, **opts)   # warm-up
+        sv.wait(rid)
+        sv.release(rid)
+        for lam in [float(r) for r in args.rates.split(",")]:
+            rng = np.random.default_rng(int(lam * 100))
+            gaps = rng.exponential(1.0 / lam, args.requests)
+            t0 = time.perf_counter()
+            due = t0
+            ids = []
+            for g in gaps:
+                due += g
+                dt = due - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids.append(sv.submit(base.stack, base.tilts, **opts))
+            res = []
+            for r in ids:
+                sv.wait(r)
+                res.append(sv.fetch(r, aligned=False))
+                sv.release(r)
+            lat = np.array([x.latency for x in res])
+            span = max(x.t_done for x in res) - min(x.t_submit for x in res)
+            rows.append({"rate_per_s": lam, "requests": len(res),
+                         "throughput_series_per_hour":
+                             round(3600.0 * len(res) / span, 1),
+                         "p50_ms": round(1000 * float(np.percentile(lat, 50)), 1),
+                         "p99_ms": round(1000 * float(np.percentile(lat, 99)), 1)})
+            print(json.dumps({"served_rate": rows[-1]}), file=sys.stderr,
+                  flush=True)
+    ctx.close()
+    best = max(r["throughput_series_per_hour"] for r in rows)
+    out = {
+        "metric": "tilt series aligned/hour + p50/p99 latency (served form)",
+        "value": best, "unit": "series/hour", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic (seeded 3000-blob phantom, int16 counts)",
+        "config": {"workload": "Poisson arrivals of 1024x1024x61 series, "
+                               "T=300, NPROJ 10",
+                   "workers": args.concurrency},
+        "rates": rows,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,6 +276,14 @@ def main():
     ap.add_argument("--mode", choices=["materialized", "fused"],
                     default="fused")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=["plain", "batched", "served"],
+                    default="plain",
+                    help="plain: the per-series form (the reported metric); "
+                         "batched / served: the multi-series forms")
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--concurrency", type=int, default=3)
+    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--rates", default="0.5,1,2,4,8")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the alignment path runs one process "
@@ -157,6 +293,11 @@ def main():
 
     import markerfree_amd as mf
     from markerfree_amd import synth
+
+    if args.form == "batched":
+        return bench_batched(mf, synth, args)
+    if args.form == "served":
+        return bench_served(mf, synth, args)
 
     cfg = synth.PRESETS["single"]
     series = synth.generate(cfg)
