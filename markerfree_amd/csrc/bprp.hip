@@ -20,6 +20,7 @@
 // the per-projection cos/sin/weight tables sit in scalar registers; larger
 // counts use the runtime-n instantiation (N = 0).
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -487,6 +488,174 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_fused(const BpParams p,
   }
 }
 
+// Table-driven fused form for n <= kTabMaxN reference projections.
+//
+// Two per-target tables are built in LDS once per workgroup:
+//   ztab[iz][i] = fma(iz + 0.5 - tb/2, sin_i, bx/2), the z half of every
+//     backprojection coordinate (shared by all voxels of slab row iz), so a
+//     lookup costs fma, cvt, shift, read, fma;
+//   wtab[m] = {W(m), RN(1/W(m))}, W(m) the reference-order FP32 sum of the
+//     weights of the projections whose bits are set in m, so a sample that
+//     needs the range tests accumulates a validity mask with one add-carry
+//     per projection and reads its weight sum and reciprocal at the end.
+// Samples inside the wave's common fast interval take no tests; the others
+// test every projection without branches.  Every voxel value, and the ray
+// sum, is computed with the same operations in the same order as the
+// reference-order loop (CalcReproj.cu:37-63, 97-124), so the output equals
+// k_bprp_fused and the materialised path bit for bit.
+constexpr int kTabMaxN = 10;
+
+__host__ __device__ constexpr int ztab_stride(int n) { return (n + 3) & ~3; }
+
+size_t tab_lds_bytes(int n, int bx, int tb) {
+  return (size_t)tb * ztab_stride(n) * 4 + ((size_t)8 << n) +
+         (size_t)n * ((size_t)bx + 1) * 4;
+}
+
+template <int N>
+__device__ __forceinline__ void load_z(const float* ztab, int iz, float* z) {
+  constexpr int ZS = ztab_stride(N);
+  const float4* r =
+      reinterpret_cast<const float4*>(ztab) + __umul24((unsigned)iz, ZS / 4);
+#pragma unroll
+  for (int k = 0; k < ZS / 4; k++) {
+    const float4 v = r[k];
+    z[4 * k] = v.x;
+    z[4 * k + 1] = v.y;
+    z[4 * k + 2] = v.z;
+    z[4 * k + 3] = v.w;
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(kFuThreads) void k_bprp_tab(const BpParams p,
+                                                         const RpParams q,
+                                                         int per) {
+  extern __shared__ float4 smem4[];
+  constexpr int ZS = ztab_stride(N);
+  float* ztab = reinterpret_cast<float*>(smem4);
+  float2* wtab = reinterpret_cast<float2*>(ztab + p.tb * ZS);
+  float* sv = reinterpret_cast<float*>(wtab + (1 << N));
+  const int bx = p.bx, rowp = bx + 1;
+  const int vx = 2 * bx;
+  const float cx = (float)bx * 0.5f;
+  const float hvx = (float)vx * 0.5f, htb = (float)p.tb * 0.5f;
+  const float iex = (float)(vx - 1), iez = (float)(q.tb - 1);
+  const unsigned iend_bits = __float_as_uint((float)(bx - 1));
+  const float ox = 0.5f - hvx, oz = 0.5f - htb;
+  const bool fast_ok = p.cnt_all >= 0.001f;
+
+  for (int k = threadIdx.x; k < p.tb * ZS; k += blockDim.x) {
+    const int iz = k / ZS, i = k - iz * ZS;
+    float v = 0.f;
+#pragma unroll
+    for (int j = 0; j < N; j++)
+      if (i == j) v = fmaf((float)iz + oz, p.s[j], cx);
+    ztab[k] = v;
+  }
+  for (int m = threadIdx.x; m < (1 << N); m += blockDim.x) {
+    float w = 0.f;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      if ((m >> (N - 1 - i)) & 1) w += p.w[i];
+    wtab[m] = make_float2(w, 1.0f / w);
+  }
+
+  const int nch = cdiv(bx, kFuThreads);
+  const int units = p.by * nch;
+  const int ua = blockIdx.x * per, ub = min(units, ua + per);
+  int ycur = -1, any_bad = 0;
+  for (int un = ua; un < ub; un++) {
+    const int y = un / nch;
+    if (y != ycur) {
+      __syncthreads();  // tables built; all waves done with the previous row
+      any_bad = load_sino_rows(p, N, y, sv);
+      ycur = y;
+    }
+    const int u = (un - y * nch) * kFuThreads + threadIdx.x;
+    const bool live = u < bx;
+    const int uu = live ? u : bx - 1;  // idle lanes shadow a real ray
+    const float xp = (float)uu + 0.5f - (float)bx * 0.5f;
+    const float tx = fmaf(xp, q.c, (float)vx * 0.5f);
+    const float tz = fmaf(xp, q.s, (float)q.tb * 0.5f);
+    const float z0 = (-xp * q.s) / q.c - 0.5f * (float)q.L;
+    int t0, t1;
+    rp_sample_range(q, tx, tz, z0, &t0, &t1);
+    int fa, fb;
+    fast_interval(p, q, N, tx, tz, z0, &fa, &fb);
+    fa = max(fa, t0);
+    fb = min(fb, t1);
+    const int ta = wave_max(fa), tb = wave_min(fb);
+    const bool has_fast = fast_ok && !any_bad && ta < tb;
+    const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
+    double sum = 0.0;
+    int cnt = 0;
+    // Samples that take the reference's tests.  A skipped sinogram sample
+    // (stored as -inf) clears its projection's validity bit when the row
+    // holds one.
+    auto general = [&](int from, int to, auto bad) {
+      for (int t = from; t < to; t++) {
+        const float fz = (float)t + z0;
+        const float X = fmaf(-fz, q.s, tx);
+        const float Z = fmaf(fz, q.c, tz);
+        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const float vfx = (inb ? truncf(X) : 0.f) + ox;
+        float zs[ZS];
+        load_z<N>(ztab, inb ? (int)Z : 0, zs);
+        float acc = 0.f;
+        unsigned m = 0;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          const float r = fmaf(vfx, p.c[i], zs[i]);
+          // 0 <= r <= bx-1 as one unsigned compare of the bit patterns.
+          // r is finite and never -0: a round-to-nearest fma returns -0 only
+          // when its product and addend are both -0, and zs[i] is not -0
+          // because its own addend, bx/2, is positive.
+          bool ok = __float_as_uint(r) <= iend_bits;
+          float val = sv[i * rowp + (ok ? (int)r : bx)];
+          if (decltype(bad)::value) {
+            ok = ok && (val != -INFINITY);
+            val = ok ? val : 0.f;
+          }
+          acc = fmaf(val, p.w[i], acc);
+          m = 2 * m + (ok ? 1u : 0u);
+        }
+        const float2 wr = wtab[m];
+        const float v =
+            (wr.x < 0.001f) ? kInvalid : div_by_rcp(acc, wr.x, wr.y);
+        const bool ok = inb && (v >= -1e10f);
+        sum += ok ? (double)v : 0.0;
+        cnt += ok ? 1 : 0;
+      }
+    };
+    if (any_bad) {
+      general(t0, t1, std::true_type{});
+    } else {
+      general(t0, a, std::false_type{});
+      if (has_fast) {
+        for (int t = a; t < b; t++) {
+          const float fz = (float)t + z0;
+          const float vfx = truncf(fmaf(-fz, q.s, tx)) + ox;
+          float zs[ZS];
+          load_z<N>(ztab, (int)fmaf(fz, q.c, tz), zs);
+          float acc = 0.f;
+#pragma unroll
+          for (int i = 0; i < N; i++)
+            acc = fmaf(sv[i * rowp + (int)fmaf(vfx, p.c[i], zs[i])], p.w[i],
+                       acc);
+          sum += (double)div_by_rcp(acc, p.cnt_all, p.rcp_all);
+        }
+        cnt += b - a;
+      }
+      general(b, t1, std::false_type{});
+    }
+    if (live)
+      q.out[(size_t)y * bx + u] = ((float)cnt < 0.8f * (float)q.L)
+                                      ? kInvalid
+                                      : (float)(sum / (double)cnt);
+  }
+}
+
 __global__ void k_fill(float* p, size_t n, float v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -510,9 +679,25 @@ struct KernelTable<0> {
   static void fill(FusedKernel* fu) { fu[0] = k_bprp_fused<0>; }
 };
 
+template <int N>
+struct TabKernelTable {
+  static void fill(FusedKernel* tk) {
+    tk[N] = k_bprp_tab<N>;
+    TabKernelTable<N - 1>::fill(tk);
+  }
+};
+template <>
+struct TabKernelTable<0> {
+  static void fill(FusedKernel* tk) { tk[0] = nullptr; }
+};
+
 struct Kernels {
   FusedKernel fu[kMaxFixedN + 1];
-  Kernels() { KernelTable<kMaxFixedN>::fill(fu); }
+  FusedKernel tab[kTabMaxN + 1];
+  Kernels() {
+    KernelTable<kMaxFixedN>::fill(fu);
+    TabKernelTable<kTabMaxN>::fill(tab);
+  }
 };
 
 const Kernels& kernels() {
@@ -607,7 +792,7 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   const size_t lds = bp_lds_bytes(n, bx);
   MF_REQUIRE(lds <= 160 * 1024, "sinogram rows exceed LDS");
   const int kid = n <= kMaxFixedN ? n : 0;
-  if (mode == 1) {
+  if (mode != 0) {
     if (n == 0) {
       hipLaunchKernelGGL(k_fill, dim3(cdiv((long long)bx * by, 256)),
                          dim3(256), 0, st, d_reproj, (size_t)bx * by,
@@ -616,19 +801,25 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
       return;
     }
     FusedKernel k = kernels().fu[kid];
-    if (lds > 64 * 1024)
+    size_t flds = lds;
+    const size_t tlds = tab_lds_bytes(n, bx, tb);
+    if (mode == 1 && n <= kTabMaxN && tlds <= 160 * 1024) {
+      k = kernels().tab[n];
+      flds = tlds;
+    }
+    if (flds > 64 * 1024)
       MF_HIP(hipFuncSetAttribute((const void*)k,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds));
+                                 (int)flds));
     // One resident wave of workgroups over the (row, ray chunk) units.
     int occ = 0;
     MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k,
-                                                        kFuThreads, lds));
+                                                        kFuThreads, flds));
     const long long units = (long long)by * cdiv(bx, kFuThreads);
     const long long slots = (long long)std::max(occ, 1) * std::max(ctx->num_cu, 1);
     const int per = cdiv(units, std::min(units, slots));
     TimedLaunch tl(ctx->timer, "bprp_fused", st);
-    hipLaunchKernelGGL(k, dim3(cdiv(units, per)), dim3(kFuThreads), lds, st,
+    hipLaunchKernelGGL(k, dim3(cdiv(units, per)), dim3(kFuThreads), flds, st,
                        bp, rp, per);
     MF_HIP(hipGetLastError());
     return;

@@ -193,7 +193,7 @@ def _check_target(ctx, orc, stack, tilts, skip, target, nproj, tb, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("bx,by,nz,step,tb", [
     (128, 16, 21, 3.0, 40),
     (256, 8, 41, 3.0, 100),   # small-series binned geometry, reduced rows
@@ -206,12 +206,14 @@ def test_gpu_reproject_parity(ctx, orc, mode, bx, by, nz, step, tb):
     for target in [z0 + 1, z0 - 3, nz - 1, 0]:
         skip = (rng.random(nz) < 0.3).astype(np.uint8)
         skip[target] = 1
-        for nproj in (1, 10):
+        # 14 neighbours exceed the table-driven kernel's limit (mode 1 then
+        # runs the table-free form).
+        for nproj in (1, 10, 14):
             _check_target(ctx, orc, stack, tilts, skip, target, nproj, tb, mode)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_gpu_reproject_bitexact_fp32(ctx, orc, mode):
     """Against the oracle's FP32 mode (backprojection accumulated as
     fma(v*cos, w, acc) in reference order, ray sums in FP64) the HIP
@@ -235,7 +237,7 @@ def test_gpu_reproject_bitexact_fp32(ctx, orc, mode):
 @pytest.mark.parametrize("bx,by,tb", [(512, 6, 150), (1024, 3, 300)])
 def test_gpu_fast_paths_bitexact_wide(ctx, orc, bx, by, tb):
     """Production widths, where most slab blocks and ray samples take the
-    kernels' test-free fast paths: the slab and both reprojection modes are
+    kernels' test-free fast paths: the slab and every reprojection mode are
     bit-identical to the oracle's FP32 mode, for targets across the tilt
     range (full, partial and single-neighbour ranges)."""
     stack, tilts = _series(bx, by, 61, 2.0, seed=bx)
@@ -255,9 +257,10 @@ def test_gpu_fast_paths_bitexact_wide(ctx, orc, bx, by, tb):
                           d_out, mode=0, d_slab=d_slab)
             np.testing.assert_array_equal(d_slab.download(), slab)
             np.testing.assert_array_equal(d_out.download(), ref)
-            ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
-                          d_out, mode=1)
-            np.testing.assert_array_equal(d_out.download(), ref)
+            for mode in (1, 2):
+                ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
+                              d_out, mode=mode)
+                np.testing.assert_array_equal(d_out.download(), ref)
     for d in (d_stack, d_out, d_slab):
         d.free()
 
@@ -271,7 +274,7 @@ def test_gpu_reproject_invalid_samples(ctx, orc):
     stack[4, 1, :] = INVALID
     skip = np.zeros(21, np.uint8)
     skip[15] = 1
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         _check_target(ctx, orc, stack, tilts, skip, 15, 10, 30, mode)
 
 
@@ -281,7 +284,7 @@ def test_gpu_reproject_empty_range(ctx, orc):
     skip = np.ones(11, np.uint8)
     d_stack = ctx.array(stack)
     d_out = ctx.empty((3, 64))
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         ctx.reproject(d_stack, 64, 3, 11, 20, tilts, skip, 5, 10, d_out,
                       mode=mode)
         assert np.all(d_out.download() == np.float32(INVALID))
