@@ -297,7 +297,7 @@ mf_status mf_stretch_cc(mf_ctx* ctx, const float* d_ref, const float* d_img,
     double ds = std::cos(kD2R * ref_tilt) / std::cos(kD2R * tilt);
     stretch_matrix((float)ds, axis_deg, p.m);
     float* w = (float*)work.ensure(ctx, stretch_cc_work_floats(bx, by, 1) * 4);
-    double* m = (double*)mom.ensure(ctx, cc_scratch_doubles(1) * sizeof(double));
+    double* m = (double*)mom.ensure(ctx, cc_scratch_doubles(1, by) * sizeof(double));
     float* dsh = (float*)sh.ensure(ctx, 2 * sizeof(float));
     stretch_cc_pairs(ctx, ctx->stream, s2, bx, by, {p}, bfactor, w, m, tab, dsh);
     MF_HIP(hipMemcpy(shift2, dsh, 2 * sizeof(float), hipMemcpyDeviceToHost));
@@ -314,7 +314,7 @@ mf_status mf_pa_measure(mf_ctx* ctx, const float* d_reproj,
     DeviceScope d(ctx);
     DevBuf work, mom, sh;
     float* w = (float*)work.ensure(ctx, 2 * (size_t)by * pad_x(bx) * 4);
-    double* m = (double*)mom.ensure(ctx, cc_scratch_doubles(1) * sizeof(double));
+    double* m = (double*)mom.ensure(ctx, cc_scratch_doubles(1, by) * sizeof(double));
     float* dsh = (float*)sh.ensure(ctx, 2 * sizeof(float));
     pa_measure(ctx, ctx->stream, d_reproj, d_proj, bx, by, w, m, dsh);
     MF_HIP(hipMemcpyAsync(shift2, dsh, 2 * sizeof(float),

@@ -134,8 +134,10 @@ __global__ __launch_bounds__(kLineThreads) void k_line_fft(
   float2* a = sm2;
   float2* b = a + nl * ls;
   float* raw = (float*)b;  // staging: nl * L floats (fits in nl * ls float2)
+  float2* tw = b + nl * ls;
   const int r0 = blockIdx.x * nl;
   const int nv = min(nl, nlines - r0);
+  fft_load_tw(P, tw);
   for (int i = threadIdx.x; i < nl * L; i += blockDim.x) {
     int l = i / L, k = i - l * L;
     raw[i] = l < nv ? lines[(size_t)(r0 + l) * pitch + k] : 0.0f;
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_fft(
     a[l * ls + k] = make_float2(v0, v1);
   }
   __syncthreads();
-  float2* Z = fft_lines(a, b, P, nl, false);
+  float2* Z = fft_lines(a, b, P, nl, false, tw);
   for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
     int l = i / (N + 1), k = i - l * (N + 1);
     if (l >= nv) continue;
@@ -374,7 +376,7 @@ AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   const int ls = fft_lds_len(P);
   const size_t per = (size_t)2 * ls * sizeof(float2);
   const int nl = std::max(1, std::min(16, (int)((64 * 1024) / per)));
-  const size_t lds = nl * per;
+  const size_t lds = nl * per + (size_t)ls * sizeof(float2);
   MF_REQUIRE(lds + 512 <= 160 * 1024, "common line exceeds LDS");
   if (lds > 60 * 1024)
     MF_HIP(hipFuncSetAttribute((const void*)k_line_fft,

@@ -25,7 +25,7 @@ const float* cc_mask(mf_ctx* ctx, int w, int h, bool pa);
 
 // Batched CalcCC::DoIt for the pairs of w x h images from 'stack'
 // (contiguous [n][h][w]).  Workspace: stretch_cc_work_floats() floats and
-// cc_scratch_doubles(npair) doubles of reduction partials.  d_shift receives
+// cc_scratch_doubles(npair, h) doubles of reduction partials.  d_shift receives
 // float2 shifts (binned pixels, before the bin factor); returns after the
 // stream has drained.
 void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
@@ -34,12 +34,12 @@ void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
                       DevBuf& tab, float* d_shift);
 
 // MeaAlignProj: d_reproj and d_proj are w x h; writes the float2 ishift.
-// Workspace: 2*h*pad_x(w) floats + cc_scratch_doubles(1) doubles.
+// Workspace: 2*h*pad_x(w) floats + cc_scratch_doubles(1, h) doubles.
 void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
                 const float* d_proj, int w, int h, float* d_work,
                 double* d_mom, float* d_ishift);
 
 size_t stretch_cc_work_floats(int w, int h, int npair);
-size_t cc_scratch_doubles(int npair);
+size_t cc_scratch_doubles(int npair, int h);
 
 }  // namespace mf

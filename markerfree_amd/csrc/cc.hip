@@ -6,6 +6,7 @@
 // (CalcCC.cu:237-271, 366-379), then scans the map on the host.  Here every
 // pair of a coarse-alignment sweep is processed in the same launches, the map
 // never leaves HBM and the peak search and parabolic fit run on the device.
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -13,7 +14,7 @@
 #include "cc.h"
 #include "context.h"
 #include "devutil.h"
-#include "fft.h"
+#include "fft_dev.h"
 
 namespace mf {
 
@@ -62,9 +63,10 @@ size_t stretch_cc_work_floats(int w, int h, int npair) {
 
 namespace {
 
-// Partial blocks per moment job and per peak map; the combining step uses
-// one 64-lane wave, one partial per lane.
-constexpr int kParts = 64;
+constexpr int kCcThreads = 256;
+// Partial (sum, sum of squares) blocks per image; their combination is a
+// fixed-order tree, so every consumer computes the same moments.
+constexpr int kMomParts = 256;
 
 // Pixel of an unpadded w-wide image read through the padded pitch the
 // reference uses (pad columns hold zero here; the reference leaves them
@@ -124,125 +126,148 @@ __global__ __launch_bounds__(256) void k_stretch_pairs(
   out[(size_t)blockIdx.y * h * pitch + (size_t)y * pitch + x] = v;
 }
 
-struct MomJob {
+// One input image of a correlation: its pixels (row pitch in floats) and the
+// padded work image [h][pad_x(w)] it is normalised and transformed in.
+struct CcImg {
   const float* src;
   int pitch;
-};
-
-struct NormJob {
-  const float* src;
   float* dst;
-  int src_pitch;
-  int mom;  // index of the (sum, sum of squares) pair
 };
 
-// Job b from a device table, or for two-job launches (jobs == nullptr) from
-// the kernel arguments.  The argument structs are selected field by field:
-// selecting a whole by-value argument struct makes the compiler copy both
-// into per-thread scratch memory.
-__device__ __forceinline__ MomJob pick_job(const MomJob* jobs,
-                                           MomJob j0, MomJob j1,
-                                           unsigned b) {
-  if (jobs) return jobs[b];
-  return MomJob{b ? j1.src : j0.src, b ? j1.pitch : j0.pitch};
-}
-__device__ __forceinline__ NormJob pick_job(const NormJob* jobs,
-                                            NormJob j0,
-                                            NormJob j1, unsigned b) {
-  if (jobs) return jobs[b];
-  return NormJob{b ? j1.src : j0.src, b ? j1.dst : j0.dst,
-                 b ? j1.src_pitch : j0.src_pitch, b ? j1.mom : j0.mom};
+// Images of a launch: a device table (TAB), or for two-image launches the
+// kernel argument itself, so the per-target measurement needs no upload.  The
+// two inline images are flat scalar fields selected one by one, and the
+// kernels are instantiated per form: a runtime choice between the table and
+// the argument makes the compiler copy the argument into scratch memory.
+struct CcImgs {
+  const CcImg* tab;
+  const float *src0, *src1;
+  float *dst0, *dst1;
+  int pitch0, pitch1;
+};
+template <bool TAB>
+__device__ __forceinline__ CcImg cc_img(const CcImgs& j, unsigned b) {
+  if (TAB) return j.tab[b];
+  return CcImg{b ? j.src1 : j.src0, b ? j.pitch1 : j.pitch0,
+               b ? j.dst1 : j.dst0};
 }
 
-// Partial (sum, sum of squares) of valid pixels over a band of rows: grid
-// (kParts, njobs).  Jobs come from a device table, or for two-job launches
-// from the kernel arguments (jobs == nullptr).  Each job's kParts partials
-// are combined in a fixed order by k_norm_mask, so results are reproducible.
-__global__ __launch_bounds__(256) void k_moments_part(const MomJob* jobs,
-                                                      MomJob j0, MomJob j1,
-                                                      int w, int h,
-                                                      double* part) {
+// Partial (sum, sum of squares) of the valid pixels of a band of rows,
+// grid (kMomParts, nimg).  FP64 accumulation (the reference's K7/K8 FP32
+// tree order is not reproduced).
+template <bool TAB>
+__global__ __launch_bounds__(kCcThreads) void k_cc_moments(const CcImgs imgs,
+                                                           int w, int h,
+                                                           double* part) {
   __shared__ double sh[16];
-  const MomJob j = pick_job(jobs, j0, j1, blockIdx.y);
-  const int rows = cdiv(h, kParts);
-  const int ya = blockIdx.x * rows, yb = min(h, ya + rows);
+  const CcImg im = cc_img<TAB>(imgs, blockIdx.y);
+  const int rows = cdiv(h, kMomParts);
+  const int ya = min(h, (int)blockIdx.x * rows), yb = min(h, ya + rows);
+  const int n = (yb - ya) * w;
   double s1 = 0.0, s2 = 0.0;
-  for (int y = ya; y < yb; y++) {
-    const float* row = j.src + (size_t)y * j.pitch;
-    for (int x = threadIdx.x; x < w; x += blockDim.x) {
-      const float v = row[x];
-      if (v < -1e10f) continue;
-      s1 += v;
-      s2 += (double)(v * v);
-    }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int y = ya + i / w, x = i - (y - ya) * w;
+    const float v = im.src[(size_t)y * im.pitch + x];
+    const bool ok = v >= -1e10f;
+    s1 += ok ? (double)v : 0.0;
+    s2 += ok ? (double)(v * v) : 0.0;
   }
   s1 = block_reduce(s1, sh, [](double a, double b) { return a + b; }, 0.0);
   s2 = block_reduce(s2, sh, [](double a, double b) { return a + b; }, 0.0);
   if (threadIdx.x == 0) {
-    double* o = part + ((size_t)blockIdx.y * kParts + blockIdx.x) * 2;
+    double* o = part + ((size_t)blockIdx.y * kMomParts + blockIdx.x) * 2;
     o[0] = s1;
     o[1] = s2;
   }
 }
 
-// mNormalize / PANorm normalisation (GNormalize, CalcCC.cu:137-147) fused
-// with GRoundEdge (mask table): invalid pixels become 0.
-__global__ __launch_bounds__(256) void k_norm_mask(const NormJob* jobs,
-                                                   NormJob j0, NormJob j1,
-                                                   int w, int h,
-                                                   const double* part,
-                                                   const float* mask,
-                                                   int unit_std) {
+// Mean and standard deviation of image b from its kMomParts partials, in a
+// fixed order (lane l sums partials l, l+64, ...; then a shuffle tree).  Call
+// from one whole wave; the result is valid in lane 0.
+__device__ void cc_mean_sd(const double* part, unsigned b, int w, int h,
+                           int unit_std, float* mean, float* sd) {
+  const int lane = threadIdx.x & 63;
+  const double* p = part + (size_t)b * kMomParts * 2;
+  double s1 = 0.0, s2 = 0.0;
+  for (int k = lane; k < kMomParts; k += 64) {
+    s1 += p[2 * k];
+    s2 += p[2 * k + 1];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s1 += __shfl_down(s1, o, 64);
+    s2 += __shfl_down(s2, o, 64);
+  }
+  const double npx = (double)w * (double)h;
+  *mean = (float)(s1 / npx);
+  *sd = 1.0f;
+  if (!unit_std) {
+    const float m2 = (float)(s2 / npx);
+    const float var = m2 - *mean * *mean;
+    *sd = var <= 0.f ? 0.0f : sqrtf(var);
+  }
+}
+
+// mNormalize / PANorm (GNormalize, CalcCC.cu:137-147; ProjAlign.cpp:272-312)
+// fused with GRoundEdge (mask table) and the row pass of the 2-D R2C: nl rows
+// of image blockIdx.y per workgroup are normalised (invalid pixels and masked
+// ones become 0), transformed and unpacked into the padded work image.
+template <bool TAB>
+__global__ __launch_bounds__(kCcThreads) void k_cc_rows_fwd(
+    const CcImgs imgs, int w, int h, const double* part, const float* mask,
+    int unit_std, FftDevPlan P, int nl) {
+  extern __shared__ float2 sm2[];
   __shared__ float sh_ms[2];
-  const NormJob j = pick_job(jobs, j0, j1, blockIdx.y);
+  const int N = P.n, ls = fft_lds_len(P);
+  float2* a = sm2;
+  float2* b = a + nl * ls;
+  float2* tw = b + nl * ls;
+  const CcImg im = cc_img<TAB>(imgs, blockIdx.y);
+  const int r0 = blockIdx.x * nl;
+  const int nv = min(nl, h - r0);
+  fft_load_tw(P, tw);
   if (threadIdx.x < 64) {
-    // kParts (= 64) partials of this job, summed in a fixed tree order.
-    const double* p = part + ((size_t)j.mom * kParts + threadIdx.x) * 2;
-    double s1 = p[0], s2 = p[1];
-    for (int o = 32; o > 0; o >>= 1) {
-      s1 += __shfl_down(s1, o, 64);
-      s2 += __shfl_down(s2, o, 64);
-    }
+    float mean, sd;
+    cc_mean_sd(part, blockIdx.y, w, h, unit_std, &mean, &sd);
     if (threadIdx.x == 0) {
-      const double npx = (double)w * (double)h;
-      const float mean = (float)(s1 / npx);
-      float sd = 1.0f;
-      if (!unit_std) {
-        const float m2 = (float)(s2 / npx);
-        const float var = m2 - mean * mean;
-        sd = var <= 0.f ? 0.0f : sqrtf(var);
-      }
       sh_ms[0] = mean;
       sh_ms[1] = sd;
     }
   }
   __syncthreads();
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= w * h) return;
-  const int y = idx / w, x = idx - y * w;
   const float mean = sh_ms[0], sd = sh_ms[1];
-  const float v = j.src[(size_t)y * j.src_pitch + x];
-  const float mk = mask[idx];
-  float o = 0.0f;
-  if (v >= -1e10f) {
-    const float nv = (v - mean) / sd;
-    if (!(nv < -1e10f) && mk != 0.0f) o = nv * mk;
+  auto norm = [&](int y, int x) {
+    const float v = im.src[(size_t)y * im.pitch + x];
+    const float mk = mask[(size_t)y * w + x];
+    float o = 0.0f;
+    if (v >= -1e10f) {
+      const float nv_ = (v - mean) / sd;
+      if (!(nv_ < -1e10f) && mk != 0.0f) o = nv_ * mk;
+    }
+    return o;
+  };
+  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+    const int l = i / N, k = i - l * N;
+    float2 v = make_float2(0.f, 0.f);
+    if (l < nv) v = make_float2(norm(r0 + l, 2 * k), norm(r0 + l, 2 * k + 1));
+    a[l * ls + k] = v;
   }
-  j.dst[(size_t)y * pad_x(w) + x] = o;
+  __syncthreads();
+  float2* Z = fft_lines(a, b, P, nl, false, tw);
+  const int cp = w / 2 + 1;  // complex row pitch of the work image
+  for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
+    const int l = i / (N + 1), k = i - l * (N + 1);
+    if (l >= nv) continue;
+    float2* out = reinterpret_cast<float2*>(im.dst) + (size_t)(r0 + l) * cp;
+    r2c_unpack(Z + l * ls, N, P.half, 1.0f, k,
+               [&](int kk, float2 v) { out[kk] = v; });
+  }
 }
 
-// GConv + GWiener + GCenterOrigin (CalcCC.cu:95-135): b = conj(a) * b,
-// times exp(-2B (k^2+y^2)/(W^2+H^2)) / sqrt(|b| + 0.01), times (-1)^(k+y).
-__global__ __launch_bounds__(256) void k_spectral(const float2* a, float2* b,
-                                                  int w, int h,
-                                                  size_t img_stride_c,
-                                                  float bfactor) {
-  const int nc = w / 2 + 1;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nc * h) return;
-  const int y = idx / nc, k = idx - y * nc;
-  const size_t i = (size_t)blockIdx.y * img_stride_c + idx;
-  const float2 r = a[i], g = b[i];
+// GConv + GWiener + GCenterOrigin (CalcCC.cu:95-135) for spectral element
+// (k, y): conj(r) * g, times exp(-2B (k^2+yy^2)/(W^2+H^2)) / sqrt(|.| + 0.01),
+// times (-1)^(k+y).
+__device__ __forceinline__ float2 cc_spectral(float2 r, float2 g, int k, int y,
+                                              int nc, int h, float bfactor) {
   float re = r.x * g.x + r.y * g.y;
   float im = r.x * g.y - r.y * g.x;
   const int yy = (y > h / 2) ? y - h : y;
@@ -257,7 +282,53 @@ __global__ __launch_bounds__(256) void k_spectral(const float2* a, float2* b,
     re = -re;
     im = -im;
   }
-  b[i] = make_float2(re, im);
+  return make_float2(re, im);
+}
+
+// Column pass of both forward transforms of a pair (times 'scale'), the
+// spectral product, and the column pass of the inverse transform, for nl
+// adjacent spectral columns per workgroup; the product's columns return to
+// the pair's second work image.
+template <bool TAB>
+__global__ __launch_bounds__(kCcThreads) void k_cc_cols(
+    const CcImgs imgs, int w, int h, int ngroups, FftDevPlan P, int nl,
+    float scale, float bfactor) {
+  extern __shared__ float2 sm2[];
+  const int ls = fft_lds_len(P);
+  float2* x = sm2;
+  float2* y = x + 2 * nl * ls;
+  float2* tw = y + 2 * nl * ls;
+  const int nc = w / 2 + 1;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int pair = blk / ngroups;
+  const int c0 = (blk - pair * ngroups) * nl;
+  const int nv = min(nl, nc - c0);
+  const CcImg ia = cc_img<TAB>(imgs, 2 * pair), ib = cc_img<TAB>(imgs, 2 * pair + 1);
+  const float2* A = reinterpret_cast<const float2*>(ia.dst);
+  float2* B = reinterpret_cast<float2*>(ib.dst);
+  fft_load_tw(P, tw);
+  for (int i = threadIdx.x; i < 2 * nl * h; i += blockDim.x) {
+    const int r = i / nl, l = i - r * nl;  // r = image * h + row
+    const int img = r >= h, row = r - img * h;
+    float2 v = make_float2(0.f, 0.f);
+    if (l < nv) v = (img ? B : A)[(size_t)row * nc + c0 + l];
+    x[(img * nl + l) * ls + row] = v;
+  }
+  __syncthreads();
+  float2* z = fft_lines(x, y, P, 2 * nl, false, tw);
+  float2* o = z == x ? y : x;
+  for (int i = threadIdx.x; i < nl * h; i += blockDim.x) {
+    const int l = i / h, row = i - l * h;
+    const float2 r = cscale(z[l * ls + row], scale);
+    const float2 g = cscale(z[(nl + l) * ls + row], scale);
+    z[(nl + l) * ls + row] = cc_spectral(r, g, c0 + l, row, nc, h, bfactor);
+  }
+  __syncthreads();
+  float2* q = fft_lines(z + nl * ls, o + nl * ls, P, nl, true, tw);
+  for (int i = threadIdx.x; i < nl * h; i += blockDim.x) {
+    const int row = i / nl, l = i - row * nl;
+    if (l < nv) B[(size_t)row * nc + c0 + l] = q[l * ls + row];
+  }
 }
 
 struct Best {
@@ -292,50 +363,91 @@ struct PeakWin {
   }
 };
 
-// Wave argmax with the smallest row-major index winning ties.
-__device__ __forceinline__ Best wave_best(Best b) {
+// Argmax with the smallest row-major index winning ties (the reference's
+// first maximum), over a whole workgroup; valid in thread 0.
+__device__ Best block_best(Best b, Best* sh) {
   for (int o = 32; o > 0; o >>= 1) {
     Best c{__shfl_down(b.v, o, 64), __shfl_down(b.i, o, 64)};
     b = best_of(b, c);
   }
-  return b;
-}
-
-// Stage 1: argmax over a band of the window's rows, grid (kParts, nmaps).
-__global__ __launch_bounds__(256) void k_peak_part(const float* maps, int w,
-                                                   int h, size_t img_stride,
-                                                   int mode, Best* part) {
-  __shared__ Best sh[4];
-  const int pitch = pad_x(w);
-  const float* cc = maps + (size_t)blockIdx.y * img_stride;
-  const PeakWin win(w, h, mode);
-  const int rows = cdiv(win.y1 - win.y0, kParts);
-  const int ya = win.y0 + blockIdx.x * rows, yb = min(win.y1, ya + rows);
-  Best b{win.init, 0x7fffffff};
-  for (int y = ya; y < yb; y++)
-    for (int x = win.x0 + threadIdx.x; x < win.x1; x += blockDim.x) {
-      const float v = cc[(size_t)y * pitch + x];
-      if (v > b.v) b = Best{v, y * w + x};
-    }
-  b = wave_best(b);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) sh[wv] = b;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int k = 1; k < (int)(blockDim.x >> 6); k++) b = best_of(b, sh[k]);
-  part[(size_t)blockIdx.y * kParts + blockIdx.x] = b;
+  if (threadIdx.x == 0)
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) b = best_of(b, sh[k]);
+  return b;
 }
 
-// Stage 2 (one wave per map): combine the partials, then the parabolic
-// sub-pixel fit of the reference (FP32 terms in mode 0, FP64 in mode 1).
-__global__ __launch_bounds__(64) void k_peak_fit(const float* maps, int w,
-                                                 int h, size_t img_stride,
-                                                 int mode, const Best* part,
-                                                 float* out) {
-  const int pitch = pad_x(w);
-  const float* cc = maps + (size_t)blockIdx.x * img_stride;
+// Row pass of the inverse 2-D transform (cufftExecC2R, unnormalised) of nl
+// rows of the pair's correlation map per workgroup, written back in place,
+// with the argmax of these rows inside the peak window as one partial.
+template <bool TAB>
+__global__ __launch_bounds__(kCcThreads) void k_cc_rows_inv(
+    const CcImgs imgs, int w, int h, FftDevPlan P, int nl, int mode,
+    Best* part) {
+  extern __shared__ float2 sm2[];
+  __shared__ Best sh[16];
+  const int N = P.n, ls = fft_lds_len(P);
+  float2* a = sm2;
+  float2* b = a + nl * ls;  // nl * (ls + 1) entries
+  float2* tw = b + nl * (ls + 1);
+  const CcImg im = cc_img<TAB>(imgs, 2 * blockIdx.y + 1);
+  const int r0 = blockIdx.x * nl;
+  const int nv = min(nl, h - r0);
+  const int bs = ls + 1, cp = w / 2 + 1;
+  float2* map = reinterpret_cast<float2*>(im.dst);
+  fft_load_tw(P, tw);
+  for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
+    const int l = i / (N + 1), k = i - l * (N + 1);
+    float2 v = make_float2(0.f, 0.f);
+    if (l < nv) v = map[(size_t)(r0 + l) * cp + k];
+    b[l * bs + k] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+    const int l = i / N, k = i - l * N;
+    a[l * ls + k] = c2r_pack(b[l * bs + k], b[l * bs + N - k], P.half[k]);
+  }
+  __syncthreads();
+  float2* z = fft_lines(a, b, P, nl, true, tw);
   const PeakWin win(w, h, mode);
-  Best b = wave_best(part[(size_t)blockIdx.x * kParts + threadIdx.x]);
+  Best best{win.init, 0x7fffffff};
+  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+    const int l = i / N, k = i - l * N;
+    if (l >= nv) continue;
+    const float2 v = z[l * ls + k];
+    map[(size_t)(r0 + l) * cp + k] = v;
+    const int y = r0 + l;
+    // A thread visits its pixels in increasing row-major order, so a
+    // strict '>' keeps its first maximum; block_best breaks ties by index.
+    if (y >= win.y0 && y < win.y1) {
+      const int xa = 2 * k, xb = 2 * k + 1;
+      if (xa >= win.x0 && xa < win.x1 && v.x > best.v)
+        best = Best{v.x, y * w + xa};
+      if (xb >= win.x0 && xb < win.x1 && v.y > best.v)
+        best = Best{v.y, y * w + xb};
+    }
+  }
+  best = block_best(best, sh);
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = best;
+}
+
+// Combines a map's row-group partials, then the parabolic sub-pixel fit of
+// the reference (FP32 terms in mode 0, FP64 in mode 1); one workgroup per
+// map.  Writes the shift of the peak from the map centre.
+template <bool TAB>
+__global__ __launch_bounds__(kCcThreads) void k_cc_fit(const CcImgs imgs,
+                                                       int w, int h, int mode,
+                                                       const Best* part,
+                                                       int nparts, float* out) {
+  __shared__ Best sh[16];
+  const int pitch = pad_x(w);
+  const float* cc = cc_img<TAB>(imgs, 2 * blockIdx.x + 1).dst;
+  const PeakWin win(w, h, mode);
+  Best b{win.init, 0x7fffffff};
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x)
+    b = best_of(b, part[(size_t)blockIdx.x * nparts + k]);
+  b = block_best(b, sh);
   if (threadIdx.x != 0) return;
   int px = mode == 0 ? 0 : win.x0, py = mode == 0 ? 0 : win.y0;
   if (b.i != 0x7fffffff) {
@@ -367,14 +479,84 @@ __global__ __launch_bounds__(64) void k_peak_fit(const float* maps, int w,
   out[2 * blockIdx.x + 1] = fpy - (float)(h / 2);
 }
 
-void find_peaks(mf_ctx* ctx, hipStream_t st, const float* maps, int w, int h,
-                size_t img_stride, int nmaps, int mode, Best* part,
-                float* out) {
-  TimedLaunch tl(ctx->timer, "cc_peak", st);
-  hipLaunchKernelGGL(k_peak_part, dim3(kParts, nmaps), dim3(256), 0, st, maps,
-                     w, h, img_stride, mode, part);
-  hipLaunchKernelGGL(k_peak_fit, dim3(nmaps), dim3(64), 0, st, maps, w, h,
-                     img_stride, mode, (const Best*)part, out);
+void set_lds(const void* k, size_t bytes) {
+  MF_REQUIRE(bytes <= 160 * 1024, "correlation lines exceed LDS");
+  if (bytes > 64 * 1024)
+    MF_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+}
+
+// Lines per workgroup: as many as the LDS budget allows (per_line bytes
+// each, 'fixed' once), but no more than keeps about two workgroups per CU
+// over 'total' lines.
+int cc_lines(mf_ctx* ctx, long long total, size_t per_line, size_t fixed,
+             int cap) {
+  const long long want = total / (2LL * std::max(ctx->num_cu, 1));
+  const long long fit = (long long)((96 * 1024 - fixed) / per_line);
+  return (int)std::max(1LL, std::min<long long>({want, fit, (long long)cap}));
+}
+
+template <bool TAB>
+void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
+               int w, int h, int unit_std, float scale, float bfactor,
+               int mode, const float* mask, double* part, float* d_shift);
+
+// The correlation of npair pairs (images 2p, 2p+1 of 'imgs'): moments,
+// normalisation, forward transforms, spectral product, inverse transform,
+// peak search and fit.  'part' holds cc_scratch_doubles(npair, h) doubles.
+void cc_run(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair, int w,
+            int h, int unit_std, float scale, float bfactor, int mode,
+            const float* mask, double* part, float* d_shift) {
+  MF_REQUIRE(w % 2 == 0 && w >= 2 && h >= 1, "bad correlation shape");
+  if (imgs.tab)
+    cc_launch<true>(ctx, st, imgs, npair, w, h, unit_std, scale, bfactor,
+                    mode, mask, part, d_shift);
+  else
+    cc_launch<false>(ctx, st, imgs, npair, w, h, unit_std, scale, bfactor,
+                     mode, mask, part, d_shift);
+}
+
+template <bool TAB>
+void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
+               int w, int h, int unit_std, float scale, float bfactor,
+               int mode, const float* mask, double* part, float* d_shift) {
+  const int nimg = 2 * npair;
+  TimedLaunch tl(ctx->timer, "cc", st);
+  hipLaunchKernelGGL(k_cc_moments<TAB>, dim3(kMomParts, nimg), dim3(kCcThreads), 0,
+                     st, imgs, w, h, part);
+  MF_HIP(hipGetLastError());
+  const FftDevPlan& PR = ctx->fft->plan(w / 2);
+  const size_t lr = fft_lds_len(PR);
+  {
+    const int nl = cc_lines(ctx, (long long)nimg * h, 2 * lr * 8, lr * 8, 16);
+    const size_t lds = (2 * nl * lr + lr) * sizeof(float2);
+    set_lds((const void*)k_cc_rows_fwd<TAB>, lds);
+    hipLaunchKernelGGL(k_cc_rows_fwd<TAB>, dim3(cdiv(h, nl), nimg), dim3(kCcThreads),
+                       lds, st, imgs, w, h, (const double*)part, mask, unit_std,
+                       PR, nl);
+    MF_HIP(hipGetLastError());
+  }
+  {
+    const FftDevPlan& PC = ctx->fft->plan(h);
+    const size_t lc = fft_lds_len(PC);
+    const int nc = w / 2 + 1;
+    const int nl = cc_lines(ctx, (long long)npair * nc, 4 * lc * 8, lc * 8, 8);
+    const size_t lds = (4 * nl * lc + lc) * sizeof(float2);
+    set_lds((const void*)k_cc_cols<TAB>, lds);
+    const int ngroups = cdiv(nc, nl);
+    hipLaunchKernelGGL(k_cc_cols<TAB>, dim3(ngroups * npair), dim3(kCcThreads), lds,
+                       st, imgs, w, h, ngroups, PC, nl, scale, bfactor);
+    MF_HIP(hipGetLastError());
+  }
+  Best* peaks = (Best*)(part + (size_t)2 * nimg * kMomParts);
+  const int nl = cc_lines(ctx, (long long)npair * h, (2 * lr + 1) * 8, lr * 8, 16);
+  const size_t lds = ((2 * lr + 1) * nl + lr) * sizeof(float2);
+  set_lds((const void*)k_cc_rows_inv<TAB>, lds);
+  const int nparts = cdiv(h, nl);
+  hipLaunchKernelGGL(k_cc_rows_inv<TAB>, dim3(nparts, npair), dim3(kCcThreads), lds,
+                     st, imgs, w, h, PR, nl, mode, peaks);
+  hipLaunchKernelGGL(k_cc_fit<TAB>, dim3(npair), dim3(kCcThreads), 0, st, imgs, w, h,
+                     mode, (const Best*)peaks, nparts, d_shift);
   MF_HIP(hipGetLastError());
 }
 
@@ -388,79 +570,53 @@ void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   if (npair <= 0) return;
   const int pitch = pad_x(w);
   const size_t img = (size_t)h * pitch;
-  float* A = d_work;                    // normalised, masked references
-  float* B = d_work + npair * img;      // normalised, masked stretched images
+  float* A = d_work;                    // normalised references, spectra
+  float* B = d_work + npair * img;      // normalised stretched images, maps
   float* S = d_work + 2 * npair * img;  // stretched images
-  std::vector<MomJob> mj(2 * npair);
-  std::vector<NormJob> nj(2 * npair);
+  std::vector<CcImg> im(2 * npair);
   for (int p = 0; p < npair; p++) {
-    mj[p] = MomJob{d_stack + (size_t)pairs[p].ref * w * h, w};
-    mj[npair + p] = MomJob{S + p * img, pitch};
-    nj[p] = NormJob{mj[p].src, A + p * img, w, p};
-    nj[npair + p] = NormJob{S + p * img, B + p * img, pitch, npair + p};
+    im[2 * p] = CcImg{d_stack + (size_t)pairs[p].ref * w * h, w, A + p * img};
+    im[2 * p + 1] = CcImg{S + p * img, pitch, B + p * img};
   }
-  const size_t b0 = npair * sizeof(PairDesc), b1 = mj.size() * sizeof(MomJob),
-               b2 = nj.size() * sizeof(NormJob);
-  char* dt = (char*)tab.ensure(ctx, b0 + b1 + b2);
+  const size_t b0 = npair * sizeof(PairDesc), b1 = im.size() * sizeof(CcImg);
+  char* dt = (char*)tab.ensure(ctx, b0 + b1);
   MF_HIP(hipMemcpyAsync(dt, pairs.data(), b0, hipMemcpyHostToDevice, st));
-  MF_HIP(hipMemcpyAsync(dt + b0, mj.data(), b1, hipMemcpyHostToDevice, st));
-  MF_HIP(hipMemcpyAsync(dt + b0 + b1, nj.data(), b2, hipMemcpyHostToDevice, st));
+  MF_HIP(hipMemcpyAsync(dt + b0, im.data(), b1, hipMemcpyHostToDevice, st));
   const PairDesc* d_pairs = (const PairDesc*)dt;
   const dim3 g2(cdiv((long long)w * h, 256), npair);
   hipLaunchKernelGGL(k_stretch_pairs, g2, dim3(256), 0, st, d_stack, w, h,
                      d_pairs, S, 1);
   MF_HIP(hipGetLastError());
-  const MomJob m0{};
-  const NormJob n0{};
-  hipLaunchKernelGGL(k_moments_part, dim3(kParts, 2 * npair), dim3(256), 0, st,
-                     (const MomJob*)(dt + b0), m0, m0, w, h, d_mom);
-  const float* mask = cc_mask(ctx, w, h, false);
-  hipLaunchKernelGGL(k_norm_mask, dim3(cdiv((long long)w * h, 256), 2 * npair),
-                     dim3(256), 0, st, (const NormJob*)(dt + b0 + b1), n0, n0,
-                     w, h, d_mom, mask, 0);
-  MF_HIP(hipGetLastError());
-  // A and B are adjacent: one batched forward transform, scaled 1/(w h)
-  // (cufft2D::ForwardFFT with bNorm).
-  fft_r2c_2d(ctx, st, A, w, h, 2 * npair, img, (float)(1.0 / w / h));
-  hipLaunchKernelGGL(k_spectral, dim3(cdiv((long long)(w / 2 + 1) * h, 256), npair),
-                     dim3(256), 0, st, (const float2*)A, (float2*)B, w, h,
-                     img / 2, bfactor);
-  MF_HIP(hipGetLastError());
-  fft_c2r_2d(ctx, st, B, w, h, npair, img);
-  find_peaks(ctx, st, B, w, h, img, npair, 0,
-             (Best*)(d_mom + (size_t)4 * npair * kParts), d_shift);
+  CcImgs imgs{};
+  imgs.tab = (const CcImg*)(dt + b0);
+  // Transform's correlation normalises the forward FFT by 1/(w h)
+  // (cufft2D::ForwardFFT with bNorm, CalcCC.cu:257,261).
+  cc_run(ctx, st, imgs, npair, w, h, 0, (float)(1.0 / w / h), bfactor, 0,
+         cc_mask(ctx, w, h, false), d_mom, d_shift);
   MF_HIP(hipStreamSynchronize(st));  // 'tab' is reusable after return
 }
 
-size_t cc_scratch_doubles(int npair) {
-  // Moment partials: 2 jobs x kParts x (sum, sum of squares) per pair; peak
-  // partials: kParts x (value, index) per pair.
-  return (size_t)5 * npair * kParts;
+size_t cc_scratch_doubles(int npair, int h) {
+  // Moment partials: 2 images x kMomParts x (sum, sum of squares) per pair;
+  // peak partials: at most one (value, index) per map row.
+  return (size_t)npair * (4 * kMomParts + h);
 }
 
 void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
                 const float* d_proj, int w, int h, float* d_work,
                 double* d_mom, float* d_ishift) {
-  const int pitch = pad_x(w);
-  const size_t img = (size_t)h * pitch;
-  float* A = d_work;
-  float* B = d_work + img;
-  const MomJob m0{d_reproj, w}, m1{d_proj, w};
-  const NormJob n0{d_reproj, A, w, 0}, n1{d_proj, B, w, 1};
-  hipLaunchKernelGGL(k_moments_part, dim3(kParts, 2), dim3(256), 0, st,
-                     nullptr, m0, m1, w, h, d_mom);
-  const float* mask = cc_mask(ctx, w, h, true);
-  hipLaunchKernelGGL(k_norm_mask, dim3(cdiv((long long)w * h, 256), 2),
-                     dim3(256), 0, st, nullptr, n0, n1, w, h, d_mom, mask, 1);
-  MF_HIP(hipGetLastError());
-  fft_r2c_2d(ctx, st, A, w, h, 2, img, 1.0f);
-  hipLaunchKernelGGL(k_spectral, dim3(cdiv((long long)(w / 2 + 1) * h, 256), 1),
-                     dim3(256), 0, st, (const float2*)A, (float2*)B, w, h,
-                     img / 2, 500.0f);
-  MF_HIP(hipGetLastError());
-  fft_c2r_2d(ctx, st, B, w, h, 1, img);
-  find_peaks(ctx, st, B, w, h, img, 1, 1, (Best*)(d_mom + (size_t)4 * kParts),
-             d_ishift);
+  const size_t img = (size_t)h * pad_x(w);
+  CcImgs imgs{};
+  imgs.src0 = d_reproj;
+  imgs.pitch0 = w;
+  imgs.dst0 = d_work;
+  imgs.src1 = d_proj;
+  imgs.pitch1 = w;
+  imgs.dst1 = d_work + img;
+  // PANorm: std := 1 (ProjAlign.cpp:272-293); unnormalised transforms and
+  // B = 500 (:258-271); PAFindPeak window (:325-375).
+  cc_run(ctx, st, imgs, 1, w, h, 1, 1.0f, 500.0f, 1, cc_mask(ctx, w, h, true),
+         d_mom, d_ishift);
 }
 
 }  // namespace mf

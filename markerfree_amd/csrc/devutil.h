@@ -23,4 +23,15 @@ __device__ T block_reduce(T v, T* sh, Op op, T identity) {
   return v;
 }
 
+// Workgroup index remapped so that consecutive logical blocks run on the same
+// XCD: the dispatcher hands workgroup g to XCD g mod 8, so logical blocks
+// [x*per, (x+1)*per) (per = nb/8, the first nb mod 8 XCDs taking one more)
+// all land on XCD x.  A bijection on [0, nb); only locality depends on it.
+__device__ __forceinline__ int xcd_block(int g, int nb) {
+  constexpr int kXcd = 8;
+  const int x = g % kXcd, slot = g / kXcd;
+  const int per = nb / kXcd, rem = nb % kXcd;
+  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + slot;
+}
+
 }  // namespace mf

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "context.h"
+#include "devutil.h"
 #include "fft_dev.h"
 
 namespace mf {
@@ -172,12 +173,14 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_rows(
   const int N = P.n, ls = fft_lds_len(P);
   float2* a = sm2;
   float2* b = sm2 + nl * ls;
+  float2* tw = b + nl * ls;
   const int r0 = blockIdx.x * nl;
   const int nv = min(nl, nrows - r0);
   auto row_ptr = [&](int r) {
     int img = r / rows_per_img, rr = r - img * rows_per_img;
     return buf + img * img_stride + (size_t)rr * pitch;
   };
+  fft_load_tw(P, tw);
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     int l = i / N, k = i - l * N;
     float2 v = make_float2(0.f, 0.f);
@@ -185,7 +188,7 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_rows(
     a[l * ls + k] = v;
   }
   __syncthreads();
-  float2* Z = fft_lines(a, b, P, nl, false);
+  float2* Z = fft_lines(a, b, P, nl, false, tw);
   for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
     int l = i / (N + 1), k = i - l * (N + 1);
     if (l >= nv) continue;
@@ -202,6 +205,7 @@ __global__ __launch_bounds__(kFftThreads) void k_c2r_rows(
   const int N = P.n, ls = fft_lds_len(P);
   float2* a = sm2;
   float2* b = sm2 + nl * ls;  // holds nl * (ls + 1) entries
+  float2* tw = b + nl * (ls + 1);
   const int r0 = blockIdx.x * nl;
   const int nv = min(nl, nrows - r0);
   auto row_ptr = [&](int r) {
@@ -210,6 +214,7 @@ __global__ __launch_bounds__(kFftThreads) void k_c2r_rows(
   };
   // Stage X_0..X_N in b (line stride ls + 1), then pack Z' into a.
   const int bs = ls + 1;
+  fft_load_tw(P, tw);
   for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
     int l = i / (N + 1), k = i - l * (N + 1);
     float2 v = make_float2(0.f, 0.f);
@@ -222,7 +227,7 @@ __global__ __launch_bounds__(kFftThreads) void k_c2r_rows(
     a[l * ls + k] = c2r_pack(b[l * bs + k], b[l * bs + N - k], P.half[k]);
   }
   __syncthreads();
-  float2* z = fft_lines(a, b, P, nl, true);
+  float2* z = fft_lines(a, b, P, nl, true, tw);
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     int l = i / N, k = i - l * N;
     if (l >= nv) continue;
@@ -231,7 +236,9 @@ __global__ __launch_bounds__(kFftThreads) void k_c2r_rows(
 }
 
 // Complex transform along columns of images of ncols x nrows complex values
-// (row pitch 'cpitch' complex), nl adjacent columns per workgroup.
+// (row pitch 'cpitch' complex), nl adjacent columns per workgroup.  Adjacent
+// column groups run on the same XCD (xcd_block), so the rows' cache lines
+// they share are fetched into one L2.
 __global__ __launch_bounds__(kFftThreads) void k_c2c_cols(
     float2* buf, int ncols, int cpitch, size_t img_stride_c, FftDevPlan P,
     int nl, int inv, float scale) {
@@ -239,10 +246,13 @@ __global__ __launch_bounds__(kFftThreads) void k_c2c_cols(
   const int N = P.n, ls = fft_lds_len(P);
   float2* a = sm2;
   float2* b = sm2 + nl * ls;
+  float2* tw = b + nl * ls;
   const int groups = (ncols + nl - 1) / nl;
-  const int img = blockIdx.x / groups;
-  const int c0 = (blockIdx.x - img * groups) * nl;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int img = blk / groups;
+  const int c0 = (blk - img * groups) * nl;
   float2* base = buf + img * img_stride_c;
+  fft_load_tw(P, tw);
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     int y = i / nl, l = i - y * nl;
     float2 v = make_float2(0.f, 0.f);
@@ -250,7 +260,7 @@ __global__ __launch_bounds__(kFftThreads) void k_c2c_cols(
     a[l * ls + y] = v;
   }
   __syncthreads();
-  float2* z = fft_lines(a, b, P, nl, inv != 0);
+  float2* z = fft_lines(a, b, P, nl, inv != 0, tw);
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     int y = i / nl, l = i - y * nl;
     if (c0 + l < ncols)
@@ -270,7 +280,8 @@ void launch_rows(mf_ctx* ctx, hipStream_t st, float* buf, int n, int nrows,
   MF_REQUIRE(n % 2 == 0 && n >= 2, "real transform length must be even");
   const FftDevPlan& P = ctx->fft->plan(n / 2);
   const int nl = fft_lines_per_group(P, kRowLds, 16);
-  const size_t lds = (size_t)nl * (2 * fft_lds_len(P) + 1) * sizeof(float2);
+  const size_t lds =
+      ((size_t)nl * (2 * fft_lds_len(P) + 1) + fft_lds_len(P)) * sizeof(float2);
   MF_REQUIRE(lds <= 160 * 1024, "FFT row exceeds LDS");
   dim3 grid(cdiv(nrows, nl));
   TimedLaunch tl(ctx->timer, "fft_rows", st);
@@ -290,8 +301,13 @@ void launch_cols(mf_ctx* ctx, hipStream_t st, float* buf, int nx, int ny,
                  int nimg, size_t img_stride, bool inv, float scale) {
   const FftDevPlan& P = ctx->fft->plan(ny);
   const int ncols = nx / 2 + 1;
-  const int nl = fft_lines_per_group(P, kColLds, 16);
-  const size_t lds = (size_t)2 * nl * fft_lds_len(P) * sizeof(float2);
+  // Columns per workgroup: as many as LDS allows, but no more than keeps
+  // about two workgroups per CU busy.
+  const long long want = (long long)ncols * nimg / (2LL * std::max(ctx->num_cu, 1));
+  const int nl = std::max(1, std::min<int>(fft_lines_per_group(P, kColLds, 16),
+                                           (int)std::max(1LL, want)));
+  const size_t lds =
+      ((size_t)2 * nl * fft_lds_len(P) + fft_lds_len(P)) * sizeof(float2);
   MF_REQUIRE(lds <= 160 * 1024, "FFT column exceeds LDS");
   dim3 grid(cdiv(ncols, nl) * nimg);
   set_lds((const void*)k_c2c_cols, lds);

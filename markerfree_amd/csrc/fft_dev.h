@@ -174,15 +174,23 @@ __host__ __device__ __forceinline__ int fft_lds_len(const FftDevPlan& P) {
   return P.blue ? P.m : P.n;
 }
 
+// Copies the twiddle table of P (fft_lds_len(P) entries) into LDS; the
+// caller synchronises before the first transform.
+__device__ __forceinline__ void fft_load_tw(const FftDevPlan& P, float2* tw) {
+  const int L = fft_lds_len(P);
+  for (int i = threadIdx.x; i < L; i += blockDim.x) tw[i] = P.tw[i];
+}
+
 // Complex transform of nl lines (length P.n, stride fft_lds_len(P)) held in
-// buf; tmp is scratch of the same size.  Input in the first n entries of each
-// line; returns the buffer holding the n outputs.  Syncs internally; the
-// caller must sync after loading.
+// buf; tmp is scratch of the same size; tw is the LDS copy of the plan's
+// twiddles (fft_load_tw).  Input in the first n entries of each line;
+// returns the buffer holding the n outputs.  Syncs internally; the caller
+// must sync after loading.
 __device__ __forceinline__ float2* fft_lines(float2* buf, float2* tmp,
                                              const FftDevPlan& P, int nl,
-                                             bool inv) {
+                                             bool inv, const float2* tw) {
   if (!P.blue)
-    return fft_stockham(buf, tmp, P.n, P.n, P.nrad, P.rad, P.tw, nl, inv);
+    return fft_stockham(buf, tmp, P.n, P.n, P.nrad, P.rad, tw, nl, inv);
   const int n = P.n, m = P.m;
   for (int i = threadIdx.x; i < nl * m; i += blockDim.x) {
     const int l = i / m, k = i - l * m;
@@ -194,7 +202,7 @@ __device__ __forceinline__ float2* fft_lines(float2* buf, float2* tmp,
     buf[l * m + k] = v;
   }
   __syncthreads();
-  float2* r = fft_stockham(buf, tmp, m, m, P.nrad, P.rad, P.tw, nl, false);
+  float2* r = fft_stockham(buf, tmp, m, m, P.nrad, P.rad, tw, nl, false);
   float2* o = (r == buf) ? tmp : buf;
   const float2* f = inv ? P.bi : P.bf;
   for (int i = threadIdx.x; i < nl * m; i += blockDim.x) {
@@ -202,7 +210,7 @@ __device__ __forceinline__ float2* fft_lines(float2* buf, float2* tmp,
     r[i] = cmul(r[i], f[k]);
   }
   __syncthreads();
-  r = fft_stockham(r, o, m, m, P.nrad, P.rad, P.tw, nl, true);
+  r = fft_stockham(r, o, m, m, P.nrad, P.rad, tw, nl, true);
   for (int i = threadIdx.x; i < nl * m; i += blockDim.x) {
     const int l = i / m, k = i - l * m;
     if (k < n) {

@@ -217,7 +217,7 @@ float series_coarse_iter(mf_series* s) {
   float* work = (float*)s->tr_work.ensure(
       s->ctx, stretch_cc_work_floats(g.bx, g.by, std::max(np, 1)) * sizeof(float));
   double* mom = (double*)s->tr_mom.ensure(
-      s->ctx, cc_scratch_doubles(std::max(np, 1)) * sizeof(double));
+      s->ctx, cc_scratch_doubles(std::max(np, 1), g.by) * sizeof(double));
   float* dsh = (float*)s->tr_shift.ensure(s->ctx, 2 * (np + 1) * sizeof(float));
   stretch_cc_pairs(s->ctx, s->st, corr, g.bx, g.by, pairs, 400.0f, work, mom,
                    s->tr_tab, dsh);
@@ -345,7 +345,7 @@ void series_projalign_setup(mf_series* s, float zshift, int align_z,
   s->pa_pad.ensure(s->ctx, (size_t)g.by * padb * s->nz * sizeof(float));
   s->pa_reproj.ensure(s->ctx, plane * sizeof(float));
   s->pa_work.ensure(s->ctx, 2 * (size_t)g.by * padb * sizeof(float));
-  s->pa_mom.ensure(s->ctx, cc_scratch_doubles(1) * sizeof(double));
+  s->pa_mom.ensure(s->ctx, cc_scratch_doubles(1, g.by) * sizeof(double));
   s->pa_ready = true;
   if (s->verbose)
     printf("bin:%d   cmpsize:%d,%d\nbinsize:%d, %d\n", s->pa_bin, g.bx / 2 + 1,

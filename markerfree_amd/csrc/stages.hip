@@ -178,8 +178,10 @@ __global__ __launch_bounds__(kRowThreads) void k_rweight(
   float2* a = sm2;
   float2* b = a + nl * ls;
   float2* X = b + nl * ls;
+  float2* tw = X + nl * xs;
   const int r0 = blockIdx.x * nl;
   const int nv = min(nl, nrows - r0);
+  fft_load_tw(P, tw);
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     int l = i / N, k = i - l * N;
     float2 v = make_float2(0.f, 0.f);
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(kRowThreads) void k_rweight(
     a[l * ls + k] = v;
   }
   __syncthreads();
-  float2* Z = fft_lines(a, b, P, nl, false);
+  float2* Z = fft_lines(a, b, P, nl, false, tw);
   for (int i = threadIdx.x; i < nl * xs; i += blockDim.x) {
     int l = i / xs, k = i - l * xs;
     r2c_unpack(Z + l * ls, N, P.half, 1.0f, k, [&](int kk, float2 v) {
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(kRowThreads) void k_rweight(
     a[l * ls + k] = c2r_pack(X[l * xs + k], X[l * xs + N - k], P.half[k]);
   }
   __syncthreads();
-  float2* z = fft_lines(a, b, P, nl, true);
+  float2* z = fft_lines(a, b, P, nl, true, tw);
   for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
     int l = i / N, k = i - l * N;
     if (l >= nv) continue;
@@ -333,7 +335,7 @@ void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
   const int ls = fft_lds_len(P);
   const size_t per = (size_t)(2 * ls + P.n + 1) * sizeof(float2);
   const int nl = std::max(1, std::min(16, (int)((64 * 1024) / per)));
-  const size_t lds = nl * per;
+  const size_t lds = nl * per + (size_t)ls * sizeof(float2);
   MF_REQUIRE(lds <= 160 * 1024, "R-weighting row exceeds LDS");
   if (lds > 64 * 1024)
     MF_HIP(hipFuncSetAttribute((const void*)k_rweight,
