@@ -325,7 +325,9 @@ def main():
     for s in pool[:args.warmup]:
         step(s)
     ctx.synchronize()
-    ctx.timing(True)
+    kname = "bp_slab" if mode == 0 else "bprp_fused"
+    # HIP events around the backprojection launches only.
+    ctx.timing(True, only=kname)
     ctx.timing_reset()
     lat = []
     t0 = time.perf_counter()
@@ -336,7 +338,6 @@ def main():
     ctx.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
-    kname = "bp_slab" if mode == 0 else "bprp_fused"
     k_ms, k_n = ctx.kernel_time(kname)
     ctx.timing(False)
 

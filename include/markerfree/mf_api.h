@@ -61,8 +61,12 @@ mf_status mf_d2d(mf_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
 /* When enabled, every launch of a named kernel is bracketed by HIP events on
  * the stream it runs on; mf_kernel_time sums the elapsed times (ms) and counts
  * launches.  Names: "bp_slab", "rp_slab", "bprp_fused", "fft_rows",
- * "fft_cols", "correct", "radon", "cc_peak". */
+ * "fft_cols", "correct", "rweight", "correct_rweight", "radon", "cc" (the
+ * launches of one correlation).  mf_timing_select restricts timing to the
+ * launches of one name (NULL or "": all), so that timing one kernel adds no
+ * events around the others.  No counterpart in the reference. */
 mf_status mf_timing_enable(mf_ctx* ctx, int enable);
+mf_status mf_timing_select(mf_ctx* ctx, const char* name);
 mf_status mf_timing_reset(mf_ctx* ctx);
 mf_status mf_kernel_time(mf_ctx* ctx, const char* name, double* total_ms,
                          long long* launches);

@@ -58,6 +58,7 @@ _sig("mf_d2h", C.c_int, _vp, _vp, _vp, C.c_size_t)
 _sig("mf_d2d", C.c_int, _vp, _vp, _vp, C.c_size_t)
 _sig("mf_timing_enable", C.c_int, _vp, C.c_int)
 _sig("mf_timing_reset", C.c_int, _vp)
+_sig("mf_timing_select", C.c_int, _vp, C.c_char_p)
 _sig("mf_kernel_time", C.c_int, _vp, C.c_char_p, C.POINTER(C.c_double),
      C.POINTER(C.c_longlong))
 _sig("mf_find_proj_range", C.c_int, _fp, C.c_int, _u8p, C.c_int, C.c_int, _ip)
@@ -187,7 +188,10 @@ class Context:
         return DeviceArray(self, shape, dtype)
 
     # ---- timing
-    def timing(self, on: bool = True):
+    def timing(self, on: bool = True, only: str | None = None):
+        """Kernel timing on or off; 'only' restricts it to one launch name."""
+        self._check(_lib.mf_timing_select(
+            self.h, only.encode() if only else None))
         self._check(_lib.mf_timing_enable(self.h, int(on)))
 
     def timing_reset(self):

@@ -158,6 +158,11 @@ mf_status mf_timing_enable(mf_ctx* ctx, int enable) {
   return MF_OK;
 }
 
+mf_status mf_timing_select(mf_ctx* ctx, const char* name) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] { ctx->timer.select(name ? name : ""); });
+}
+
 mf_status mf_timing_reset(mf_ctx* ctx) {
   if (!ctx) return MF_EINVAL;
   return guard(ctx, [&] {
@@ -260,15 +265,12 @@ mf_status mf_correct_proj(mf_ctx* ctx, const float* d_raw, int nx, int ny,
     p.shift_y = shift_y;
     p.cos_a = hcos(th);
     p.sin_a = hsin(th);
-    DevBuf pb, padb;
+    DevBuf pb;
     CorrProj* dp = (CorrProj*)pb.ensure(ctx, sizeof(CorrProj));
     MF_HIP(hipMemcpyAsync(dp, &p, sizeof(p), hipMemcpyHostToDevice, ctx->stream));
     if (rweight) {
-      const int pitch = pad_x(g.bx);
-      float* pad = (float*)padb.ensure(ctx, (size_t)pitch * g.by * sizeof(float));
-      correct_bin(ctx, ctx->stream, d_raw, g, dp, 0, 1, random_fill != 0, pad,
-                  pitch, (size_t)pitch * g.by);
-      rweight_rows(ctx, ctx->stream, pad, g.bx, g.by, d_out);
+      correct_rweight(ctx, ctx->stream, d_raw, g, dp, 0, 1, random_fill != 0,
+                      d_out);
     } else {
       correct_bin(ctx, ctx->stream, d_raw, g, dp, 0, 1, random_fill != 0,
                   d_out, g.bx, (size_t)g.bx * g.by);

@@ -13,6 +13,7 @@
 #include "axis.h"
 #include "cc.h"
 #include "context.h"
+#include "devutil.h"
 #include "fft_dev.h"
 #include "stages.h"
 
@@ -85,14 +86,21 @@ struct RadonImg {
 
 // GRadon (Rotate.cu:83-136): mean of the nearest non-negative pixels along
 // the candidate direction over the cos(tilt)-cropped common region; the sum
-// is sequential in x exactly as the reference thread sums it.
+// is sequential in x exactly as the reference thread sums it.  One-dimensional
+// grid over (line block, candidate, image), image slowest, remapped so that
+// each XCD works through its own range of images and their pixels stay in
+// its L2.
 __global__ __launch_bounds__(256) void k_radon(const float* stack, int nx,
                                                int ny, int num, int L,
                                                const float* cs, const int* reg,
                                                const RadonImg* imgs,
                                                float* lines, int pitch) {
-  const int y = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y, z = blockIdx.z;
+  const int yblocks = cdiv(L, 256);
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int z = blk / (yblocks * num);
+  const int rem = blk - z * yblocks * num;
+  const int k = rem / yblocks;
+  const int y = (rem - k * yblocks) * blockDim.x + threadIdx.x;
   if (y >= L) return;
   const float* img = stack + (size_t)z * nx * ny;
   const RadonImg im = imgs[z];
@@ -103,18 +111,21 @@ __global__ __launch_bounds__(256) void k_radon(const float* stack, int nx,
   int cnt = 0;
   const int left = (int)((float)r[0] * im.ct + 0.5f);
   const int right = (int)((float)r[1] * im.ct + 0.5f);
-  for (int pass = 0; pass < 2; pass++) {
-    const int xa = pass == 0 ? left : 0, xb = pass == 0 ? 0 : right;
-    for (int x = xa; x < xb; x++) {
-      float X = (float)x * c - fy * s + im.ox;
-      float Y = (float)x * s + fy * c + im.oy;
-      if (X < 0.f || X >= (float)nx) continue;
-      if (Y < 0.f || Y >= (float)ny) continue;
-      float v = img[(size_t)(int)Y * nx + (int)X];
-      if (v < 0.f) continue;
-      sum += v;
-      cnt++;
-    }
+  // x runs over [left, 0) then [0, right) as the reference's two loops do;
+  // left <= 0 <= right.  Branch-free so that the loads of several samples
+  // are in flight at once: a skipped sample adds +0.0f to a sum that is
+  // never -0, which leaves it unchanged.
+  const float fys = fy * s, fyc = fy * c;
+  const float fnx = (float)nx, fny = (float)ny;
+#pragma unroll 4
+  for (int x = left; x < right; x++) {
+    const float X = (float)x * c - fys + im.ox;
+    const float Y = (float)x * s + fyc + im.oy;
+    const bool in = X >= 0.f && X < fnx && Y >= 0.f && Y < fny;
+    const float v = img[in ? (int)Y * nx + (int)X : 0];
+    const bool ok = in && !(v < 0.f);
+    sum += ok ? v : 0.0f;
+    cnt += ok ? 1 : 0;
   }
   lines[((size_t)z * num + k) * pitch + y] = cnt == 0 ? kInvalid : sum / (float)cnt;
 }
@@ -368,7 +379,7 @@ AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
                      (float)ny * 0.5f + (float)sy[z0], d_cs, reg);
   {
     TimedLaunch tl(ctx->timer, "radon", st);
-    hipLaunchKernelGGL(k_radon, dim3(cdiv(L, 256), num, nz), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_radon, dim3(cdiv(L, 256) * num * nz), dim3(256), 0, st,
                        d_stack, nx, ny, num, L, d_cs, reg, d_ri, lines, pitch);
   }
   MF_HIP(hipGetLastError());

@@ -12,12 +12,31 @@ void throw_hip(hipError_t e, const char* what, const char* file, int line) {
   throw Error(code, buf);
 }
 
+KernelTimer::~KernelTimer() {
+  for (Rec* r : pending_) free_.push_back(r);
+  for (Rec* r : free_) {
+    (void)hipEventDestroy(r->ev[0]);
+    (void)hipEventDestroy(r->ev[1]);
+    delete r;
+  }
+}
+
 hipEvent_t* KernelTimer::begin(const char* name, hipStream_t s) {
-  if (!enabled_) return nullptr;
-  Rec* r = new Rec;
+  if (!enabled_ || (!only_.empty() && only_ != name)) return nullptr;
+  Rec* r = nullptr;
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    if (!free_.empty()) {
+      r = free_.back();
+      free_.pop_back();
+    }
+  }
+  if (!r) {
+    r = new Rec;
+    MF_HIP(hipEventCreate(&r->ev[0]));
+    MF_HIP(hipEventCreate(&r->ev[1]));
+  }
   r->name = name;
-  MF_HIP(hipEventCreate(&r->ev[0]));
-  MF_HIP(hipEventCreate(&r->ev[1]));
   MF_HIP(hipEventRecord(r->ev[0], s));
   std::lock_guard<std::mutex> g(mu_);
   pending_.push_back(r);
@@ -37,9 +56,7 @@ void KernelTimer::drain() {
     auto& d = done_[r->name];
     d.first += ms;
     d.second += 1;
-    (void)hipEventDestroy(r->ev[0]);
-    (void)hipEventDestroy(r->ev[1]);
-    delete r;
+    free_.push_back(r);
   }
   pending_.clear();
 }

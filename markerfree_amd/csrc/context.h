@@ -18,7 +18,10 @@ class FftCache;
 // Sums HIP-event-timed durations of named kernels (bench/roofline support).
 class KernelTimer {
  public:
+  ~KernelTimer();
   void enable(bool on) { enabled_ = on; }
+  // Times only launches named 'name' (empty: all).
+  void select(const std::string& name) { only_ = name; }
   bool enabled() const { return enabled_; }
   // Returns an event pair to record around one launch, or null when off.
   hipEvent_t* begin(const char* name, hipStream_t s);
@@ -32,7 +35,9 @@ class KernelTimer {
     hipEvent_t ev[2];
   };
   bool enabled_ = false;
+  std::string only_;
   std::vector<Rec*> pending_;
+  std::vector<Rec*> free_;  // event pairs reused across launches
   std::map<std::string, std::pair<double, long long>> done_;
   std::mutex mu_;
   void drain();

@@ -39,7 +39,7 @@ struct mf_series {
   float pa_z0 = 0.0f;
   std::vector<float> pa_angles;
   mf::CorrGeom pa_geom{};
-  mf::DevBuf pa_corr, pa_pad, pa_reproj, pa_work, pa_mom, pa_dev, pa_slab;
+  mf::DevBuf pa_corr, pa_reproj, pa_work, pa_mom, pa_dev, pa_slab;
   std::string last_error;
   ~mf_series();
 };

@@ -21,7 +21,7 @@ mf_series::~mf_series() {
   if (st) (void)hipStreamSynchronize(st);
   for (mf::DevBuf* b : {&raw, &stack, &tmp, &tr_corr, &tr_work, &tr_mom,
                         &tr_shift, &tr_tab, &cproj, &off_corr, &pa_corr,
-                        &pa_pad, &pa_reproj, &pa_work, &pa_mom, &pa_dev,
+                        &pa_reproj, &pa_work, &pa_mom, &pa_dev,
                         &pa_slab, &axw.lines, &axw.reg, &axw.small, &offw.buf})
     b->release();
   if (st) (void)hipStreamDestroy(st);
@@ -342,7 +342,6 @@ void series_projalign_setup(mf_series* s, float zshift, int align_z,
   const CorrGeom& g = s->pa_geom;
   const size_t plane = (size_t)g.bx * g.by, padb = pad_x(g.bx);
   s->pa_corr.ensure(s->ctx, plane * s->nz * sizeof(float));
-  s->pa_pad.ensure(s->ctx, (size_t)g.by * padb * s->nz * sizeof(float));
   s->pa_reproj.ensure(s->ctx, plane * sizeof(float));
   s->pa_work.ensure(s->ctx, 2 * (size_t)g.by * padb * sizeof(float));
   s->pa_mom.ensure(s->ctx, cc_scratch_doubles(1, g.by) * sizeof(double));
@@ -359,15 +358,11 @@ float series_projalign_iter(mf_series* s) {
   const int nz = s->nz;
   const CorrGeom& g = s->pa_geom;
   const size_t plane = (size_t)g.bx * g.by;
-  const int padb = pad_x(g.bx);
-  const size_t pimg = (size_t)g.by * padb;
   float* corr = (float*)s->pa_corr.get();
-  float* pad = (float*)s->pa_pad.get();
   // Correct + bin + R-weight the whole stack with the current parameters.
   CorrProj* cp = upload_corr(s, corr_params(s, false));
-  correct_bin(ctx, s->st, (const float*)s->stack.get(), g, cp, 0, nz, true,
-              pad, padb, pimg);
-  rweight_rows(ctx, s->st, pad, g.bx, nz * g.by, corr);
+  correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, 0, nz, true,
+                  corr);
   // FitRotCenterZ (ProjAlign.cpp:459-486) with dA starting at 0 (quirk Q13),
   // then RemoveOffsetZ(-1) (:488-501).
   const std::vector<float>& ang = s->pa_angles;
@@ -430,9 +425,8 @@ float series_projalign_iter(mf_series* s) {
     hipLaunchKernelGGL(k_pa_update, dim3(1), dim3(1), 0, s->st, d, t, binx,
                        biny, cp);
     MF_HIP(hipGetLastError());
-    correct_bin(ctx, s->st, (const float*)s->stack.get(), g, cp, t, 1, true,
-                pad + t * pimg, padb, pimg);
-    rweight_rows(ctx, s->st, pad + t * pimg, g.bx, g.by, corr + t * plane);
+    correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, t, 1,
+                    true, corr + t * plane);
     skip[t] = 0;
     order.push_back(t);
   };
@@ -500,12 +494,12 @@ size_t series_hbm_bytes(int nx, int ny, int nz, const mf_options* o) {
   const CorrGeom gc = CorrGeom::make(nx, ny, tb);
   const size_t coarse = (size_t)4 * gc.bx * gc.by * nz +
                         stretch_cc_work_floats(gc.bx, gc.by, nz) * 4;
-  // Projection matching: corrected and R-weighted stacks, slab if
-  // materialised.
+  // Projection matching: the corrected, R-weighted stack and the
+  // measurement's work images; the slab if materialised.
   const int pb = std::max(std::max((int)(nx / 2048.0f + 0.5f),
                                    (int)(ny / 2048.0f + 0.5f)), 1);
   const CorrGeom gp = CorrGeom::make(nx, ny, pb);
-  size_t pa = (size_t)4 * nz * gp.by * (gp.bx + pad_x(gp.bx)) +
+  size_t pa = (size_t)4 * nz * gp.by * gp.bx +
               (size_t)4 * 3 * gp.by * pad_x(gp.bx);
   if (o->reproj_mode == 0)
     pa += (size_t)4 * 2 * gp.bx * gp.by * (o->align_z / pb / 2 * 2);

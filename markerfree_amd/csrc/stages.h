@@ -54,6 +54,12 @@ void correct_bin(mf_ctx* ctx, hipStream_t st, const float* d_raw,
 // pad_x(bx)) is R2C'd, ramp-filtered, C2R'd and written unpadded to d_out.
 void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
                   int nrows, float* d_out);
+// correct_bin with random fill choice, then rweight_rows, in one pass: the
+// R-weighted binned projections z0 .. z0+count-1 go to d_out as
+// [count][by][bx] (unpadded; d_out holds projection z0 first).
+void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
+                     const CorrGeom& g, const CorrProj* d_proj, int z0,
+                     int count, bool random_fill, float* d_out);
 
 // ------------------------------------------------------- correlation
 // Per-image FP64 moments sum(v) and sum(v^2) over valid pixels of w x h

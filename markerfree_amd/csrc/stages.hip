@@ -5,6 +5,7 @@
 // one projection per call after a disk read and H2D copy of it; here the
 // preprocessed stack stays in HBM and one launch corrects any number of
 // projections.
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -123,16 +124,12 @@ struct CorrArgs {
   size_t out_stride;
 };
 
-// One binned output pixel per thread: each of its bin x bin corrected pixels
-// is the nearest raw pixel at R(axis) p + shift + centre (mGCorrect), random
-// filled or invalid outside; the bin sum is invalid if any is (mGBinImage).
-__global__ __launch_bounds__(256) void k_correct_bin(const CorrArgs a) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.bx * a.by) return;
-  const int y = idx / a.bx, x = idx - y * a.bx;
-  const int z = a.z0 + blockIdx.y;
-  const CorrProj pr = a.proj[z];
-  const float* raw = a.raw + (size_t)z * a.nx * a.ny;
+// Binned output pixel (x, y) of projection z: each of its bin x bin corrected
+// pixels is the nearest raw pixel at R(axis) p + shift + centre (mGCorrect),
+// random filled or invalid outside; the bin sum is invalid if any is
+// (mGBinImage).
+__device__ float corr_pixel(const CorrArgs& a, const CorrProj& pr,
+                            const float* raw, int x, int y) {
   const float hx = (float)a.cx * 0.5f, hy = (float)a.ny * 0.5f;
   const float ox = pr.shift_x + (float)a.nx * 0.5f;
   const float oy = pr.shift_y + (float)a.ny * 0.5f;
@@ -161,8 +158,19 @@ __global__ __launch_bounds__(256) void k_correct_bin(const CorrArgs a) {
       sum += v;
     }
   }
+  return bad ? kInvalid : sum;
+}
+
+// One binned output pixel per thread.
+__global__ __launch_bounds__(256) void k_correct_bin(const CorrArgs a) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.bx * a.by) return;
+  const int y = idx / a.bx, x = idx - y * a.bx;
+  const int z = a.z0 + blockIdx.y;
+  const CorrProj pr = a.proj[z];
+  const float* raw = a.raw + (size_t)z * a.nx * a.ny;
   a.out[(size_t)blockIdx.y * a.out_stride + (size_t)y * a.out_pitch + x] =
-      bad ? kInvalid : sum;
+      corr_pixel(a, pr, raw, x, y);
 }
 
 constexpr int kRowThreads = 256;
@@ -209,6 +217,58 @@ __global__ __launch_bounds__(kRowThreads) void k_rweight(
     int l = i / N, k = i - l * N;
     if (l >= nv) continue;
     reinterpret_cast<float2*>(out + (size_t)(r0 + l) * 2 * N)[k] = z[l * ls + k];
+  }
+}
+
+// Correction + binning (k_correct_bin) fused with the R-weighting of the
+// rows it produces (k_rweight): nl binned rows of projection z0 + blockIdx.y
+// per workgroup go from the raw image to the unpadded R-weighted output
+// without passing through HBM.  Same operations, same results.
+__global__ __launch_bounds__(kRowThreads) void k_correct_rweight(
+    const CorrArgs ca, FftDevPlan P, int nl, const float* filt) {
+  extern __shared__ float2 sm2[];
+  const int N = P.n, ls = fft_lds_len(P), xs = N + 1;
+  const float fn = 2.0f * ((float)(N + 1) - 1.0f);
+  float2* a = sm2;
+  float2* b = a + nl * ls;
+  float2* X = b + nl * ls;
+  float2* tw = X + nl * xs;
+  const int r0 = blockIdx.x * nl;
+  const int nv = min(nl, ca.by - r0);
+  const int z = ca.z0 + blockIdx.y;
+  const CorrProj pr = ca.proj[z];
+  const float* raw = ca.raw + (size_t)z * ca.nx * ca.ny;
+  fft_load_tw(P, tw);
+  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+    int l = i / N, k = i - l * N;
+    float2 v = make_float2(0.f, 0.f);
+    if (l < nv)
+      v = make_float2(corr_pixel(ca, pr, raw, 2 * k, r0 + l),
+                      corr_pixel(ca, pr, raw, 2 * k + 1, r0 + l));
+    a[l * ls + k] = v;
+  }
+  __syncthreads();
+  float2* Z = fft_lines(a, b, P, nl, false, tw);
+  for (int i = threadIdx.x; i < nl * xs; i += blockDim.x) {
+    int l = i / xs, k = i - l * xs;
+    r2c_unpack(Z + l * ls, N, P.half, 1.0f, k, [&](int kk, float2 v) {
+      const float f = filt[kk];
+      X[l * xs + kk] = make_float2((v.x / fn) * f, (v.y / fn) * f);
+    });
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+    int l = i / N, k = i - l * N;
+    a[l * ls + k] = c2r_pack(X[l * xs + k], X[l * xs + N - k], P.half[k]);
+  }
+  __syncthreads();
+  float2* zz = fft_lines(a, b, P, nl, true, tw);
+  float* out = ca.out + (size_t)blockIdx.y * ca.out_stride;
+  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
+    int l = i / N, k = i - l * N;
+    if (l >= nv) continue;
+    reinterpret_cast<float2*>(out + (size_t)(r0 + l) * 2 * N)[k] =
+        zz[l * ls + k];
   }
 }
 
@@ -345,6 +405,47 @@ void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
   TimedLaunch tl(ctx->timer, "rweight", st);
   hipLaunchKernelGGL(k_rweight, dim3(cdiv(nrows, nl)), dim3(kRowThreads), lds,
                      st, d_pad, pad_x(bx), d_out, nrows, P, nl, filt);
+  MF_HIP(hipGetLastError());
+}
+
+void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
+                     const CorrGeom& g, const CorrProj* d_proj, int z0,
+                     int count, bool random_fill, float* d_out) {
+  if (count <= 0) return;
+  MF_REQUIRE(g.bx % 2 == 0 && g.bx >= 2, "R-weighting needs an even row length");
+  CorrArgs a;
+  a.raw = d_raw;
+  a.proj = d_proj;
+  a.out = d_out;
+  a.nx = g.nx;
+  a.ny = g.ny;
+  a.cx = g.cx;
+  a.bx = g.bx;
+  a.by = g.by;
+  a.binx = g.binx;
+  a.biny = g.biny;
+  a.z0 = z0;
+  a.fill = random_fill ? 1 : 0;
+  a.out_pitch = g.bx;
+  a.out_stride = (size_t)g.bx * g.by;
+  const FftDevPlan& P = ctx->fft->plan(g.bx / 2);
+  const int ls = fft_lds_len(P);
+  const size_t per = (size_t)(2 * ls + P.n + 1) * sizeof(float2);
+  // Rows per workgroup: within 48 KB of LDS, and no more than keeps about
+  // two workgroups per CU busy.
+  const long long want = (long long)g.by * count / (2LL * std::max(ctx->num_cu, 1));
+  const int fit = (int)((48 * 1024 - (size_t)ls * sizeof(float2)) / per);
+  const int nl = (int)std::max(1LL, std::min<long long>({want, (long long)fit, 16LL}));
+  const size_t lds = nl * per + (size_t)ls * sizeof(float2);
+  MF_REQUIRE(lds <= 160 * 1024, "R-weighting row exceeds LDS");
+  if (lds > 64 * 1024)
+    MF_HIP(hipFuncSetAttribute((const void*)k_correct_rweight,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+  const float* filt = rweight_filter(ctx, g.bx);
+  TimedLaunch tl(ctx->timer, "correct_rweight", st);
+  hipLaunchKernelGGL(k_correct_rweight, dim3(cdiv(g.by, nl), count),
+                     dim3(kRowThreads), lds, st, a, P, nl, filt);
   MF_HIP(hipGetLastError());
 }
 
