@@ -66,6 +66,19 @@ mf_status mf_d2d(mf_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
  * launches of one name (NULL or "": all), so that timing one kernel adds no
  * events around the others.  No counterpart in the reference. */
 mf_status mf_timing_enable(mf_ctx* ctx, int enable);
+/* Launch-shape tunables (workgroup size, lines per workgroup) keyed by kernel
+ * family and shape, e.g. "cc_rows/1024x1024/threads"; a key that is not set
+ * takes the built-in heuristic.  Written by the autotuner
+ * (markerfree_amd.autotune) and read back from its JSON cache.  No
+ * counterpart in the reference.  mf_tune_set with value 0 removes the key;
+ * mf_tune_get returns MF_EINVAL for a key that is not set. */
+mf_status mf_tune_set(mf_ctx* ctx, const char* key, int value);
+mf_status mf_tune_get(mf_ctx* ctx, const char* key, int* value);
+/* Sets every tunable of a JSON object {"key": positive integer, ...} read
+ * from 'path' (the autotuner's cache); *count receives how many were set.
+ * MF_EIO if the file cannot be read, MF_EINVAL if it is not such an
+ * object. */
+mf_status mf_tune_load(mf_ctx* ctx, const char* path, int* count);
 mf_status mf_timing_select(mf_ctx* ctx, const char* name);
 mf_status mf_timing_reset(mf_ctx* ctx);
 mf_status mf_kernel_time(mf_ctx* ctx, const char* name, double* total_ms,
@@ -133,6 +146,17 @@ mf_status mf_stretch_cc(mf_ctx* ctx, const float* d_ref, const float* d_img,
  * reprojection with a corrected projection; shift2 (host) = ishift. */
 mf_status mf_pa_measure(mf_ctx* ctx, const float* d_reproj,
                         const float* d_proj, int bx, int by, float* shift2);
+
+/* CalcTIltAxis::Radon (method/Rotate.cu:397-428, GRadon :83-136) for the
+ * candidate grid of CLsetup (:339-374) around axis0 over 'range' degrees:
+ * the mean line values of every image of d_stack [nz][ny][nx] for every
+ * candidate, as [nz][num][L] floats into d_lines (device), L = *line_len.
+ * tilts and per-image shifts on the host; the common region follows the
+ * zero-tilt image's shift.  d_lines NULL: only *line_len is set. */
+mf_status mf_common_lines(mf_ctx* ctx, const float* d_stack, int nx, int ny,
+                          int nz, const float* tilts, const float* shift_x,
+                          const float* shift_y, float axis0, float range,
+                          int num, float* d_lines, int* line_len);
 
 /* ------------------------------------------------------------- a series */
 

@@ -19,6 +19,8 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = Path(os.environ.get("MARKERFREE_HIP_LIB",
                                PKG / "lib" / "libmarkerfree_hip.so"))
 SYNTH_PATH = PKG / "lib" / "libmf_synth.so"
+# Launch shapes chosen by the autotuner (markerfree_amd.autotune) on MI355X.
+DEFAULT_TUNING = PKG / "tuned" / "gfx950.json"
 CLI_PATH = PKG / "bin" / "markerfree"
 
 if not LIB_PATH.exists():
@@ -59,6 +61,11 @@ _sig("mf_d2d", C.c_int, _vp, _vp, _vp, C.c_size_t)
 _sig("mf_timing_enable", C.c_int, _vp, C.c_int)
 _sig("mf_timing_reset", C.c_int, _vp)
 _sig("mf_timing_select", C.c_int, _vp, C.c_char_p)
+_sig("mf_tune_set", C.c_int, _vp, C.c_char_p, C.c_int)
+_sig("mf_tune_get", C.c_int, _vp, C.c_char_p, C.POINTER(C.c_int))
+_sig("mf_tune_load", C.c_int, _vp, C.c_char_p, C.POINTER(C.c_int))
+_sig("mf_common_lines", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _fp, _fp,
+     _fp, C.c_float, C.c_float, C.c_int, _vp, C.POINTER(C.c_int))
 _sig("mf_kernel_time", C.c_int, _vp, C.c_char_p, C.POINTER(C.c_double),
      C.POINTER(C.c_longlong))
 _sig("mf_find_proj_range", C.c_int, _fp, C.c_int, _u8p, C.c_int, C.c_int, _ip)
@@ -151,7 +158,11 @@ class DeviceArray:
 class Context:
     """mf_ctx: one device, one stream, an HBM budget and kernel timers."""
 
-    def __init__(self, device: int = 0, hbm_budget: int = 0):
+    def __init__(self, device: int = 0, hbm_budget: int = 0,
+                 tuning: str | None = "default"):
+        """'tuning': path of an autotuner JSON cache to apply, "default" for
+        the committed cache of this GPU family (if present), None for the
+        built-in launch-shape heuristics."""
         import weakref
         h = _vp()
         st = _lib.mf_ctx_create(device, hbm_budget, C.byref(h))
@@ -160,6 +171,26 @@ class Context:
         self.h = h.value
         # Series must be destroyed before their context (C-ABI rule).
         self._series = weakref.WeakSet()
+        if tuning == "default":
+            tuning = str(DEFAULT_TUNING) if DEFAULT_TUNING.exists() else None
+        if tuning:
+            self.tune_load(tuning)
+
+    # ---- launch-shape tunables (markerfree_amd.autotune)
+    def tune_set(self, key: str, value: int):
+        self._check(_lib.mf_tune_set(self.h, key.encode(), int(value)))
+
+    def tune_get(self, key: str):
+        """Value of a tunable, or None when it is not set."""
+        v = C.c_int()
+        if _lib.mf_tune_get(self.h, key.encode(), C.byref(v)):
+            return None
+        return int(v.value)
+
+    def tune_load(self, path) -> int:
+        n = C.c_int()
+        self._check(_lib.mf_tune_load(self.h, str(path).encode(), C.byref(n)))
+        return int(n.value)
 
     def _check(self, st):
         if st:
@@ -300,7 +331,26 @@ def _ctx_pa_measure(self, d_reproj, d_proj, bx, by):
     return float(out[0]), float(out[1])
 
 
+def _ctx_common_lines(self, d_stack, nx, ny, nz, tilts, sx, sy, axis0,
+                      arange, num=100):
+    """GRadon lines [nz][num][L] (host array) of a resident stack."""
+    a = [np.ascontiguousarray(v, np.float32) for v in (tilts, sx, sy)]
+    L = C.c_int()
+    self._check(_lib.mf_common_lines(self.h, d_stack.ptr, nx, ny, nz,
+                                     *map(f32p, a), axis0, arange, num, None,
+                                     C.byref(L)))
+    out = self.empty((nz, num, L.value))
+    try:
+        self._check(_lib.mf_common_lines(self.h, d_stack.ptr, nx, ny, nz,
+                                         *map(f32p, a), axis0, arange, num,
+                                         out.ptr, C.byref(L)))
+        return out.download()
+    finally:
+        out.free()
+
+
 Context.correct_proj = _ctx_correct_proj
+Context.common_lines = _ctx_common_lines
 Context.stretch_cc = _ctx_stretch_cc
 Context.pa_measure = _ctx_pa_measure
 

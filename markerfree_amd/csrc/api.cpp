@@ -1,6 +1,10 @@
 // extern "C" entry points: argument checks, exception-to-status translation.
+#include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <string>
 #include <new>
 
 #include "bprp.h"
@@ -156,6 +160,80 @@ mf_status mf_timing_enable(mf_ctx* ctx, int enable) {
   if (!ctx) return MF_EINVAL;
   ctx->timer.enable(enable != 0);
   return MF_OK;
+}
+
+mf_status mf_tune_set(mf_ctx* ctx, const char* key, int value) {
+  if (!ctx || !key) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(value >= 0, "tunable values are positive (0 removes the key)");
+    std::lock_guard<std::mutex> g(ctx->tune_mu);
+    if (value == 0)
+      ctx->tune.erase(key);
+    else
+      ctx->tune[key] = value;
+  });
+}
+
+mf_status mf_tune_get(mf_ctx* ctx, const char* key, int* value) {
+  if (!ctx || !key || !value) return MF_EINVAL;
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  auto it = ctx->tune.find(key);
+  if (it == ctx->tune.end()) return MF_EINVAL;
+  *value = it->second;
+  return MF_OK;
+}
+
+mf_status mf_tune_load(mf_ctx* ctx, const char* path, int* count) {
+  if (!ctx || !path) return MF_EINVAL;
+  return guard(ctx, [&] {
+    FILE* f = fopen(path, "rb");
+    if (!f) throw Error(MF_EIO, std::string("cannot read ") + path);
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    fclose(f);
+    // A flat object of string keys and positive integer values.
+    std::map<std::string, int> kv;
+    size_t i = text.find('{');
+    MF_REQUIRE(i != std::string::npos, "tuning file is not a JSON object");
+    auto skip_ws = [&] {
+      while (i < text.size() && isspace((unsigned char)text[i])) i++;
+    };
+    i++;
+    skip_ws();
+    while (i < text.size() && text[i] != '}') {
+      MF_REQUIRE(text[i] == '"', "tuning file: expected a string key");
+      const size_t e = text.find('"', i + 1);
+      MF_REQUIRE(e != std::string::npos, "tuning file: unterminated key");
+      const std::string key = text.substr(i + 1, e - i - 1);
+      i = e + 1;
+      skip_ws();
+      MF_REQUIRE(i < text.size() && text[i] == ':', "tuning file: expected ':'");
+      i++;
+      skip_ws();
+      size_t used = 0;
+      long v = 0;
+      try {
+        v = std::stol(text.substr(i, 24), &used);
+      } catch (...) {
+        used = 0;
+      }
+      MF_REQUIRE(used > 0 && v > 0 && v < (1L << 30),
+                 "tuning file: values are positive integers");
+      kv[key] = (int)v;
+      i += used;
+      skip_ws();
+      if (i < text.size() && text[i] == ',') {
+        i++;
+        skip_ws();
+      }
+    }
+    MF_REQUIRE(i < text.size(), "tuning file: unterminated object");
+    std::lock_guard<std::mutex> g(ctx->tune_mu);
+    for (auto& p : kv) ctx->tune[p.first] = p.second;
+    if (count) *count = (int)kv.size();
+  });
 }
 
 mf_status mf_timing_select(mf_ctx* ctx, const char* name) {
@@ -321,6 +399,32 @@ mf_status mf_pa_measure(mf_ctx* ctx, const float* d_reproj,
     pa_measure(ctx, ctx->stream, d_reproj, d_proj, bx, by, w, m, dsh);
     MF_HIP(hipMemcpyAsync(shift2, dsh, 2 * sizeof(float),
                           hipMemcpyDeviceToHost, ctx->stream));
+    MF_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+mf_status mf_common_lines(mf_ctx* ctx, const float* d_stack, int nx, int ny,
+                          int nz, const float* tilts, const float* shift_x,
+                          const float* shift_y, float axis0, float range,
+                          int num, float* d_lines, int* line_len) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(d_stack && tilts && shift_x && shift_y && line_len && nx > 0 &&
+                   ny > 0 && nz > 0 && num >= 2,
+               "bad arguments");
+    DeviceScope d(ctx);
+    const AxisSetup a = AxisSetup::make(nx, ny, axis0, range, num);
+    MF_REQUIRE(a.L >= 4, "common-line length too small for this image size");
+    *line_len = a.L;
+    if (!d_lines) return;
+    std::vector<double> sx(shift_x, shift_x + nz), sy(shift_y, shift_y + nz);
+    AxisWork ws;
+    float* lines = common_lines(ctx, ctx->stream, d_stack, nx, ny, nz, tilts,
+                                sx.data(), sy.data(), a, ws);
+    MF_HIP(hipMemcpy2DAsync(d_lines, (size_t)a.L * 4, lines,
+                            (size_t)(a.L + 2) * 4, (size_t)a.L * 4,
+                            (size_t)nz * num, hipMemcpyDeviceToDevice,
+                            ctx->stream));
     MF_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

@@ -23,11 +23,22 @@ struct AxisResult {
 
 struct AxisWork {
   DevBuf lines, reg, small;
+  // Transposed copy of the stack for column-like candidates, and the stack
+  // it was made from (reset this when that stack's pixels change).
+  DevBuf tstack;
+  const void* t_of = nullptr;
 };
 
 struct OffsetWork {
   DevBuf buf;
 };
+
+// GRadon (Rotate.cu:397-428, 83-136): the mean line values of every image
+// for every candidate of 'a', in ws.lines as [nz][num] lines of a.L values at
+// a pitch of a.L + 2 floats (returned).  Asynchronous on 'st'.
+float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
+                    int ny, int nz, const float* tilts, const double* sx,
+                    const double* sy, const AxisSetup& a, AxisWork& ws);
 
 // Radon lines of every image for every candidate, their spectra and the
 // leave-one-out line correlation score (Rotate.cu:292-561).  Synchronous.

@@ -89,8 +89,14 @@ struct RadonImg {
 // is sequential in x exactly as the reference thread sums it.  One-dimensional
 // grid over (line block, candidate, image), image slowest, remapped so that
 // each XCD works through its own range of images and their pixels stay in
-// its L2.
-__global__ __launch_bounds__(256) void k_radon(const float* stack, int nx,
+// its L2.  The lanes of a wave trace parallel lines a pixel apart, stepping
+// by (-sin, cos); for candidates nearer the image's x axis than its y axis
+// that step crosses rows, so those read the transposed stack (tstack, [nz]
+// [nx][ny]), where it runs along them.  Samples inside the analytic
+// in-image interval of the line (with a one-pixel margin) skip the bounds
+// test; the others take it.
+__global__ __launch_bounds__(256) void k_radon(const float* stack,
+                                               const float* tstack, int nx,
                                                int ny, int num, int L,
                                                const float* cs, const int* reg,
                                                const RadonImg* imgs,
@@ -102,32 +108,91 @@ __global__ __launch_bounds__(256) void k_radon(const float* stack, int nx,
   const int k = rem / yblocks;
   const int y = (rem - k * yblocks) * blockDim.x + threadIdx.x;
   if (y >= L) return;
-  const float* img = stack + (size_t)z * nx * ny;
   const RadonImg im = imgs[z];
   const float c = cs[2 * k], s = cs[2 * k + 1];
+  const bool trans = tstack && fabsf(c) > fabsf(s);
+  const float* img = (trans ? tstack : stack) + (size_t)z * nx * ny;
+  // Element (ix, iy) is at iy*sy + ix*sx.
+  const int sxs = trans ? ny : 1, sys = trans ? 1 : nx;
   const float fy = (float)y - (float)L * 0.5f;
   const int* r = reg + 2 * ((size_t)k * L + y);
   float sum = 0.0f;
   int cnt = 0;
   const int left = (int)((float)r[0] * im.ct + 0.5f);
   const int right = (int)((float)r[1] * im.ct + 0.5f);
+  const float fys = fy * s, fyc = fy * c;
+  const float fnx = (float)nx, fny = (float)ny;
+  // In-image interval [xa, xb) from the exact linear forms with a one-pixel
+  // margin (the FP32 evaluation errs by far less).
+  double lo = -1e30, hi = 1e30;
+  auto clip = [&](double a0, double a1, double mx) {
+    // a0 + a1 x in [1, mx - 1]
+    if (fabs(a1) < 1e-12) {
+      if (a0 < 1.0 || a0 > mx - 1.0) {
+        lo = 1e30;
+        hi = -1e30;
+      }
+      return;
+    }
+    const double t1 = (1.0 - a0) / a1, t2 = (mx - 1.0 - a0) / a1;
+    lo = fmax(lo, fmin(t1, t2));
+    hi = fmin(hi, fmax(t1, t2));
+  };
+  clip((double)im.ox - (double)fys, c, nx);
+  clip((double)im.oy + (double)fyc, s, ny);
+  int xa = right, xb = right;
+  if (lo <= hi) {
+    xa = (int)fmax((double)left, fmin((double)right, ceil(lo)));
+    xb = (int)fmax((double)xa, fmin((double)right, floor(hi) + 1.0));
+  }
   // x runs over [left, 0) then [0, right) as the reference's two loops do;
   // left <= 0 <= right.  Branch-free so that the loads of several samples
   // are in flight at once: a skipped sample adds +0.0f to a sum that is
   // never -0, which leaves it unchanged.
-  const float fys = fy * s, fyc = fy * c;
-  const float fnx = (float)nx, fny = (float)ny;
+  auto tested = [&](int from, int to) {
 #pragma unroll 4
-  for (int x = left; x < right; x++) {
+    for (int x = from; x < to; x++) {
+      const float X = (float)x * c - fys + im.ox;
+      const float Y = (float)x * s + fyc + im.oy;
+      const bool in = X >= 0.f && X < fnx && Y >= 0.f && Y < fny;
+      const float v = img[in ? (int)Y * sys + (int)X * sxs : 0];
+      const bool ok = in && !(v < 0.f);
+      sum += ok ? v : 0.0f;
+      cnt += ok ? 1 : 0;
+    }
+  };
+  tested(left, xa);
+#pragma unroll 4
+  for (int x = xa; x < xb; x++) {
     const float X = (float)x * c - fys + im.ox;
     const float Y = (float)x * s + fyc + im.oy;
-    const bool in = X >= 0.f && X < fnx && Y >= 0.f && Y < fny;
-    const float v = img[in ? (int)Y * nx + (int)X : 0];
-    const bool ok = in && !(v < 0.f);
+    const float v = img[(int)Y * sys + (int)X * sxs];
+    const bool ok = !(v < 0.f);
     sum += ok ? v : 0.0f;
     cnt += ok ? 1 : 0;
   }
+  tested(xb, right);
   lines[((size_t)z * num + k) * pitch + y] = cnt == 0 ? kInvalid : sum / (float)cnt;
+}
+
+// [nz][ny][nx] -> [nz][nx][ny] in 32 x 32 tiles staged in LDS.
+__global__ __launch_bounds__(256) void k_transpose(const float* in, float* out,
+                                                   int nx, int ny) {
+  __shared__ float t[32][33];
+  const size_t plane = (size_t)nx * ny;
+  const float* src = in + blockIdx.z * plane;
+  float* dst = out + blockIdx.z * plane;
+  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int j = ty; j < 32; j += 8) {
+    const int x = x0 + tx, y = y0 + j;
+    if (x < nx && y < ny) t[j][tx] = src[(size_t)y * nx + x];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int y = y0 + tx, x = x0 + j;
+    if (x < nx && y < ny) dst[(size_t)x * ny + y] = t[tx][j];
+  }
 }
 
 constexpr int kLineThreads = 256;
@@ -149,9 +214,9 @@ __global__ __launch_bounds__(kLineThreads) void k_line_fft(
   const int r0 = blockIdx.x * nl;
   const int nv = min(nl, nlines - r0);
   fft_load_tw(P, tw);
-  for (int i = threadIdx.x; i < nl * L; i += blockDim.x) {
-    int l = i / L, k = i - l * L;
-    raw[i] = l < nv ? lines[(size_t)(r0 + l) * pitch + k] : 0.0f;
+  for (LineIter it(L); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
+    raw[l * L + k] = l < nv ? lines[(size_t)(r0 + l) * pitch + k] : 0.0f;
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -185,8 +250,8 @@ __global__ __launch_bounds__(kLineThreads) void k_line_fft(
     }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     const float m = smean[l];
     float v0 = raw[l * L + 2 * k], v1 = raw[l * L + 2 * k + 1];
     v0 = (v0 < 0.f ? 0.0f : v0 - m) * taper[2 * k];
@@ -195,8 +260,8 @@ __global__ __launch_bounds__(kLineThreads) void k_line_fft(
   }
   __syncthreads();
   float2* Z = fft_lines(a, b, P, nl, false, tw);
-  for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
-    int l = i / (N + 1), k = i - l * (N + 1);
+  for (LineIter it(N + 1); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     if (l >= nv) continue;
     float2* out = reinterpret_cast<float2*>(lines + (size_t)(r0 + l) * pitch);
     r2c_unpack(Z + l * ls, N, P.half, 1.0f, k,
@@ -311,10 +376,9 @@ __global__ __launch_bounds__(256) void k_offset_moments(const float* corr,
 
 }  // namespace
 
-AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
-                       int nx, int ny, int nz, const float* tilts,
-                       const double* sx, const double* sy, float axis0,
-                       float range, int num, AxisWork& ws) {
+float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
+                    int ny, int nz, const float* tilts, const double* sx,
+                    const double* sy, const AxisSetup& a, AxisWork& ws) {
   // GetFrameIdxFromTilt(0) (Util.cpp:8-20).
   int z0 = 0;
   {
@@ -327,9 +391,64 @@ AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
       }
     }
   }
+  const int num = a.num, L = a.L, nc = L / 2 + 1, pitch = L + 2;
+  MF_REQUIRE(L >= 4, "common-line length too small for this image size");
+  std::vector<RadonImg> ri(nz);
+  for (int z = 0; z < nz; z++) {
+    ri[z].ct = (float)std::cos(tilts[z] * kD2R);
+    ri[z].ox = (float)nx * 0.5f + (float)sx[z];
+    ri[z].oy = (float)ny * 0.5f + (float)sy[z];
+  }
+  // Workspace (the 'small' block also holds axis_search's score buffers).
+  const size_t lines_f = (size_t)nz * num * pitch;
+  float* lines = (float*)ws.lines.ensure(ctx, lines_f * sizeof(float));
+  int* reg = (int*)ws.reg.ensure(ctx, (size_t)num * L * 2 * sizeof(int));
+  char* small = (char*)ws.small.ensure(
+      ctx, a.cs.size() * 4 + nz * sizeof(RadonImg) + (size_t)num * nc * 8 +
+               (size_t)num * nz * 4 + num * 4 + 256);
+  float* d_cs = (float*)small;
+  RadonImg* d_ri = (RadonImg*)(small + a.cs.size() * 4);
+  MF_HIP(hipMemcpyAsync(d_cs, a.cs.data(), a.cs.size() * 4,
+                        hipMemcpyHostToDevice, st));
+  MF_HIP(hipMemcpyAsync(d_ri, ri.data(), nz * sizeof(RadonImg),
+                        hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_common_region, dim3(cdiv(L, 256), num), dim3(256), 0, st,
+                     num, L, nx, ny, (float)nx * 0.5f + (float)sx[z0],
+                     (float)ny * 0.5f + (float)sy[z0], d_cs, reg);
+  // Column-like candidates read a transposed copy of the stack, made once
+  // per stack.
+  bool colike = false;
+  for (int k = 0; k < num; k++)
+    colike = colike || std::fabs(a.cs[2 * k]) > std::fabs(a.cs[2 * k + 1]);
+  const float* tstack = nullptr;
+  if (colike) {
+    float* t = (float*)ws.tstack.ensure(ctx, (size_t)nx * ny * nz * sizeof(float));
+    if (ws.t_of != d_stack) {
+      hipLaunchKernelGGL(k_transpose, dim3(cdiv(nx, 32), cdiv(ny, 32), nz),
+                         dim3(256), 0, st, d_stack, t, nx, ny);
+      MF_HIP(hipGetLastError());
+      ws.t_of = d_stack;
+    }
+    tstack = t;
+  }
+  {
+    TimedLaunch tl(ctx->timer, "radon", st);
+    hipLaunchKernelGGL(k_radon, dim3(cdiv(L, 256) * num * nz), dim3(256), 0, st,
+                       d_stack, tstack, nx, ny, num, L, d_cs, reg, d_ri, lines,
+                       pitch);
+  }
+  MF_HIP(hipGetLastError());
+  return lines;
+}
+
+AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
+                       int nx, int ny, int nz, const float* tilts,
+                       const double* sx, const double* sy, float axis0,
+                       float range, int num, AxisWork& ws) {
   AxisSetup a = AxisSetup::make(nx, ny, axis0, range, num);
   const int L = a.L, nc = L / 2 + 1, pitch = L + 2;
-  MF_REQUIRE(L >= 4, "common-line length too small for this image size");
+  float* lines =
+      common_lines(ctx, st, d_stack, nx, ny, nz, tilts, sx, sy, a, ws);
   // Host tables: taper (GRemoveMean) and filter (GConv) depend on L only.
   const float* d_taper =
       (const float*)ctx_table(ctx, "taper/" + std::to_string(L), [&] {
@@ -352,36 +471,11 @@ AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   }
   const float* d_filt = (const float*)ctx_table(
       ctx, "linefilt/" + std::to_string(nc), [&] { return filt; });
-  std::vector<RadonImg> ri(nz);
-  for (int z = 0; z < nz; z++) {
-    ri[z].ct = (float)std::cos(tilts[z] * kD2R);
-    ri[z].ox = (float)nx * 0.5f + (float)sx[z];
-    ri[z].oy = (float)ny * 0.5f + (float)sy[z];
-  }
-  // Workspace.
-  const size_t lines_f = (size_t)nz * num * pitch;
-  float* lines = (float*)ws.lines.ensure(ctx, lines_f * sizeof(float));
-  int* reg = (int*)ws.reg.ensure(ctx, (size_t)num * L * 2 * sizeof(int));
-  char* small = (char*)ws.small.ensure(
-      ctx, a.cs.size() * 4 + nz * sizeof(RadonImg) + (size_t)num * nc * 8 +
-               (size_t)num * nz * 4 + num * 4 + 256);
-  float* d_cs = (float*)small;
+  char* small = (char*)ws.small.get();
   RadonImg* d_ri = (RadonImg*)(small + a.cs.size() * 4);
   float2* d_S = (float2*)((char*)d_ri + nz * sizeof(RadonImg));
   float* d_ccv = (float*)(d_S + (size_t)num * nc);
   float* d_score = d_ccv + (size_t)num * nz;
-  MF_HIP(hipMemcpyAsync(d_cs, a.cs.data(), a.cs.size() * 4,
-                        hipMemcpyHostToDevice, st));
-  MF_HIP(hipMemcpyAsync(d_ri, ri.data(), nz * sizeof(RadonImg),
-                        hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_common_region, dim3(cdiv(L, 256), num), dim3(256), 0, st,
-                     num, L, nx, ny, (float)nx * 0.5f + (float)sx[z0],
-                     (float)ny * 0.5f + (float)sy[z0], d_cs, reg);
-  {
-    TimedLaunch tl(ctx->timer, "radon", st);
-    hipLaunchKernelGGL(k_radon, dim3(cdiv(L, 256) * num * nz), dim3(256), 0, st,
-                       d_stack, nx, ny, num, L, d_cs, reg, d_ri, lines, pitch);
-  }
   MF_HIP(hipGetLastError());
   const FftDevPlan& P = ctx->fft->plan(L / 2);
   const int ls = fft_lds_len(P);

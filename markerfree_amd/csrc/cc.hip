@@ -163,11 +163,9 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_moments(const CcImgs imgs,
   const CcImg im = cc_img<TAB>(imgs, blockIdx.y);
   const int rows = cdiv(h, kMomParts);
   const int ya = min(h, (int)blockIdx.x * rows), yb = min(h, ya + rows);
-  const int n = (yb - ya) * w;
   double s1 = 0.0, s2 = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int y = ya + i / w, x = i - (y - ya) * w;
-    const float v = im.src[(size_t)y * im.pitch + x];
+  for (LineIter it(w); it.l < yb - ya; it.next()) {
+    const float v = im.src[(size_t)(ya + it.l) * im.pitch + it.k];
     const bool ok = v >= -1e10f;
     s1 += ok ? (double)v : 0.0;
     s2 += ok ? (double)(v * v) : 0.0;
@@ -245,8 +243,8 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_rows_fwd(
     }
     return o;
   };
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    const int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    const int l = it.l, k = it.k;
     float2 v = make_float2(0.f, 0.f);
     if (l < nv) v = make_float2(norm(r0 + l, 2 * k), norm(r0 + l, 2 * k + 1));
     a[l * ls + k] = v;
@@ -254,9 +252,8 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_rows_fwd(
   __syncthreads();
   float2* Z = fft_lines(a, b, P, nl, false, tw);
   const int cp = w / 2 + 1;  // complex row pitch of the work image
-  for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
-    const int l = i / (N + 1), k = i - l * (N + 1);
-    if (l >= nv) continue;
+  for (LineIter it(N + 1); it.l < nv; it.next()) {
+    const int l = it.l, k = it.k;
     float2* out = reinterpret_cast<float2*>(im.dst) + (size_t)(r0 + l) * cp;
     r2c_unpack(Z + l * ls, N, P.half, 1.0f, k,
                [&](int kk, float2 v) { out[kk] = v; });
@@ -307,8 +304,8 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_cols(
   const float2* A = reinterpret_cast<const float2*>(ia.dst);
   float2* B = reinterpret_cast<float2*>(ib.dst);
   fft_load_tw(P, tw);
-  for (int i = threadIdx.x; i < 2 * nl * h; i += blockDim.x) {
-    const int r = i / nl, l = i - r * nl;  // r = image * h + row
+  for (LineIter it(nl); it.l < 2 * h; it.next()) {
+    const int r = it.l, l = it.k;  // r = image * h + row
     const int img = r >= h, row = r - img * h;
     float2 v = make_float2(0.f, 0.f);
     if (l < nv) v = (img ? B : A)[(size_t)row * nc + c0 + l];
@@ -317,16 +314,16 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_cols(
   __syncthreads();
   float2* z = fft_lines(x, y, P, 2 * nl, false, tw);
   float2* o = z == x ? y : x;
-  for (int i = threadIdx.x; i < nl * h; i += blockDim.x) {
-    const int l = i / h, row = i - l * h;
+  for (LineIter it(h); it.l < nl; it.next()) {
+    const int l = it.l, row = it.k;
     const float2 r = cscale(z[l * ls + row], scale);
     const float2 g = cscale(z[(nl + l) * ls + row], scale);
     z[(nl + l) * ls + row] = cc_spectral(r, g, c0 + l, row, nc, h, bfactor);
   }
   __syncthreads();
   float2* q = fft_lines(z + nl * ls, o + nl * ls, P, nl, true, tw);
-  for (int i = threadIdx.x; i < nl * h; i += blockDim.x) {
-    const int row = i / nl, l = i - row * nl;
+  for (LineIter it(nl); it.l < h; it.next()) {
+    const int row = it.l, l = it.k;
     if (l < nv) B[(size_t)row * nc + c0 + l] = q[l * ls + row];
   }
 }
@@ -397,24 +394,23 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_rows_inv(
   const int bs = ls + 1, cp = w / 2 + 1;
   float2* map = reinterpret_cast<float2*>(im.dst);
   fft_load_tw(P, tw);
-  for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
-    const int l = i / (N + 1), k = i - l * (N + 1);
+  for (LineIter it(N + 1); it.l < nl; it.next()) {
+    const int l = it.l, k = it.k;
     float2 v = make_float2(0.f, 0.f);
     if (l < nv) v = map[(size_t)(r0 + l) * cp + k];
     b[l * bs + k] = v;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    const int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    const int l = it.l, k = it.k;
     a[l * ls + k] = c2r_pack(b[l * bs + k], b[l * bs + N - k], P.half[k]);
   }
   __syncthreads();
   float2* z = fft_lines(a, b, P, nl, true, tw);
   const PeakWin win(w, h, mode);
   Best best{win.init, 0x7fffffff};
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    const int l = i / N, k = i - l * N;
-    if (l >= nv) continue;
+  for (LineIter it(N); it.l < nv; it.next()) {
+    const int l = it.l, k = it.k;
     const float2 v = z[l * ls + k];
     map[(size_t)(r0 + l) * cp + k] = v;
     const int y = r0 + l;
@@ -486,14 +482,31 @@ void set_lds(const void* k, size_t bytes) {
                                (int)bytes));
 }
 
-// Lines per workgroup: as many as the LDS budget allows (per_line bytes
-// each, 'fixed' once), but no more than keeps about two workgroups per CU
-// over 'total' lines.
-int cc_lines(mf_ctx* ctx, long long total, size_t per_line, size_t fixed,
-             int cap) {
-  const long long want = total / (2LL * std::max(ctx->num_cu, 1));
+// Launch shape of one line-transform kernel: lines per workgroup and
+// workgroup size.  Tunables "<family>/<w>x<h>x<npair>/lines" and
+// ".../threads" override the heuristic: as many lines as keep about 'per_cu' workgroups
+// per CU over 'total' lines, within 96 KB of LDS (per_line bytes each,
+// 'fixed' once) and 'cap'; one thread per radix-8 butterfly of those lines,
+// 128 to kCcThreads.
+struct Shape {
+  int nl, threads;
+};
+Shape cc_shape(mf_ctx* ctx, const char* family, int w, int h, int npair,
+               long long total, int per_cu, size_t per_line, size_t fixed,
+               int cap, int n) {
+  const std::string key = std::string(family) + "/" + std::to_string(w) + "x" +
+                          std::to_string(h) + "x" + std::to_string(npair) + "/";
+  const long long want = total / ((long long)per_cu * std::max(ctx->num_cu, 1));
   const long long fit = (long long)((96 * 1024 - fixed) / per_line);
-  return (int)std::max(1LL, std::min<long long>({want, fit, (long long)cap}));
+  Shape s;
+  s.nl = (int)std::max(1LL, std::min<long long>({want, fit, (long long)cap}));
+  s.nl = std::max(1, std::min(tune_get(ctx, key + "lines", s.nl), (int)fit));
+  const int t = (s.nl * std::max(n / 8, 1) + 63) / 64 * 64;
+  s.threads = std::max(128, std::min(t, kCcThreads));
+  s.threads = tune_get(ctx, key + "threads", s.threads);
+  MF_REQUIRE(s.threads % 64 == 0 && s.threads <= kCcThreads,
+             "workgroup size must be a multiple of 64 up to 256");
+  return s;
 }
 
 template <bool TAB>
@@ -528,33 +541,37 @@ void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
   const FftDevPlan& PR = ctx->fft->plan(w / 2);
   const size_t lr = fft_lds_len(PR);
   {
-    const int nl = cc_lines(ctx, (long long)nimg * h, 2 * lr * 8, lr * 8, 16);
-    const size_t lds = (2 * nl * lr + lr) * sizeof(float2);
+    const Shape sh = cc_shape(ctx, "cc_rowsf", w, h, npair, (long long)nimg * h, 8,
+                              2 * lr * 8, lr * 8, 16, (int)lr);
+    const size_t lds = (2 * sh.nl * lr + lr) * sizeof(float2);
     set_lds((const void*)k_cc_rows_fwd<TAB>, lds);
-    hipLaunchKernelGGL(k_cc_rows_fwd<TAB>, dim3(cdiv(h, nl), nimg), dim3(kCcThreads),
-                       lds, st, imgs, w, h, (const double*)part, mask, unit_std,
-                       PR, nl);
+    hipLaunchKernelGGL(k_cc_rows_fwd<TAB>, dim3(cdiv(h, sh.nl), nimg),
+                       dim3(sh.threads), lds, st, imgs, w, h,
+                       (const double*)part, mask, unit_std, PR, sh.nl);
     MF_HIP(hipGetLastError());
   }
   {
     const FftDevPlan& PC = ctx->fft->plan(h);
     const size_t lc = fft_lds_len(PC);
     const int nc = w / 2 + 1;
-    const int nl = cc_lines(ctx, (long long)npair * nc, 4 * lc * 8, lc * 8, 8);
-    const size_t lds = (4 * nl * lc + lc) * sizeof(float2);
+    // Two lines (the pair's two columns) per column of the group.
+    const Shape sh = cc_shape(ctx, "cc_cols", w, h, npair, (long long)npair * nc, 2,
+                              4 * lc * 8, lc * 8, 8, 2 * (int)lc);
+    const size_t lds = (4 * sh.nl * lc + lc) * sizeof(float2);
     set_lds((const void*)k_cc_cols<TAB>, lds);
-    const int ngroups = cdiv(nc, nl);
-    hipLaunchKernelGGL(k_cc_cols<TAB>, dim3(ngroups * npair), dim3(kCcThreads), lds,
-                       st, imgs, w, h, ngroups, PC, nl, scale, bfactor);
+    const int ngroups = cdiv(nc, sh.nl);
+    hipLaunchKernelGGL(k_cc_cols<TAB>, dim3(ngroups * npair), dim3(sh.threads),
+                       lds, st, imgs, w, h, ngroups, PC, sh.nl, scale, bfactor);
     MF_HIP(hipGetLastError());
   }
   Best* peaks = (Best*)(part + (size_t)2 * nimg * kMomParts);
-  const int nl = cc_lines(ctx, (long long)npair * h, (2 * lr + 1) * 8, lr * 8, 16);
-  const size_t lds = ((2 * lr + 1) * nl + lr) * sizeof(float2);
+  const Shape sh = cc_shape(ctx, "cc_rowsi", w, h, npair, (long long)npair * h, 8,
+                            (2 * lr + 1) * 8, lr * 8, 16, (int)lr);
+  const size_t lds = ((2 * lr + 1) * sh.nl + lr) * sizeof(float2);
   set_lds((const void*)k_cc_rows_inv<TAB>, lds);
-  const int nparts = cdiv(h, nl);
-  hipLaunchKernelGGL(k_cc_rows_inv<TAB>, dim3(nparts, npair), dim3(kCcThreads), lds,
-                     st, imgs, w, h, PR, nl, mode, peaks);
+  const int nparts = cdiv(h, sh.nl);
+  hipLaunchKernelGGL(k_cc_rows_inv<TAB>, dim3(nparts, npair), dim3(sh.threads),
+                     lds, st, imgs, w, h, PR, sh.nl, mode, peaks);
   hipLaunchKernelGGL(k_cc_fit<TAB>, dim3(npair), dim3(kCcThreads), 0, st, imgs, w, h,
                      mode, (const Best*)peaks, nparts, d_shift);
   MF_HIP(hipGetLastError());

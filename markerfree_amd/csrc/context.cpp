@@ -116,6 +116,12 @@ const void* ctx_table(mf_ctx* ctx, const std::string& key,
   return d;
 }
 
+int tune_get(mf_ctx* ctx, const std::string& key, int dflt) {
+  std::lock_guard<std::mutex> g(ctx->tune_mu);
+  auto it = ctx->tune.find(key);
+  return it == ctx->tune.end() ? dflt : it->second;
+}
+
 DevBuf::~DevBuf() { release(); }
 
 void* DevBuf::ensure(mf_ctx* ctx, size_t bytes) {

@@ -80,6 +80,10 @@ void dev_free(mf_ctx* ctx, void* p, size_t bytes);
 const void* ctx_table(mf_ctx* ctx, const std::string& key,
                       const std::function<std::vector<float>()>& build);
 
+// Launch-shape tunable 'key' (for example "cc_rows/1024x1024/threads") as set
+// through mf_tune_set, else 'dflt' (the built-in heuristic).
+int tune_get(mf_ctx* ctx, const std::string& key, int dflt);
+
 }  // namespace mf
 
 struct mf_ctx {
@@ -94,6 +98,7 @@ struct mf_ctx {
   mf::DevBuf slab_ws;                   // materialised-slab workspace
   std::unique_ptr<mf::FftCache> fft;
   std::map<std::string, std::unique_ptr<mf::DevBuf>> tables;
-  std::mutex mu, table_mu;
+  std::map<std::string, int> tune;      // launch-shape tunables
+  std::mutex mu, table_mu, tune_mu;
   ~mf_ctx();
 };

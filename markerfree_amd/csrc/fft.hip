@@ -181,16 +181,16 @@ __global__ __launch_bounds__(kFftThreads) void k_r2c_rows(
     return buf + img * img_stride + (size_t)rr * pitch;
   };
   fft_load_tw(P, tw);
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     float2 v = make_float2(0.f, 0.f);
     if (l < nv) v = reinterpret_cast<const float2*>(row_ptr(r0 + l))[k];
     a[l * ls + k] = v;
   }
   __syncthreads();
   float2* Z = fft_lines(a, b, P, nl, false, tw);
-  for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
-    int l = i / (N + 1), k = i - l * (N + 1);
+  for (LineIter it(N + 1); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     if (l >= nv) continue;
     float2* out = reinterpret_cast<float2*>(row_ptr(r0 + l));
     r2c_unpack(Z + l * ls, N, P.half, scale, k,
@@ -215,21 +215,21 @@ __global__ __launch_bounds__(kFftThreads) void k_c2r_rows(
   // Stage X_0..X_N in b (line stride ls + 1), then pack Z' into a.
   const int bs = ls + 1;
   fft_load_tw(P, tw);
-  for (int i = threadIdx.x; i < nl * (N + 1); i += blockDim.x) {
-    int l = i / (N + 1), k = i - l * (N + 1);
+  for (LineIter it(N + 1); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     float2 v = make_float2(0.f, 0.f);
     if (l < nv) v = reinterpret_cast<const float2*>(row_ptr(r0 + l))[k];
     b[l * bs + k] = v;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     a[l * ls + k] = c2r_pack(b[l * bs + k], b[l * bs + N - k], P.half[k]);
   }
   __syncthreads();
   float2* z = fft_lines(a, b, P, nl, true, tw);
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     if (l >= nv) continue;
     reinterpret_cast<float2*>(row_ptr(r0 + l))[k] = cscale(z[l * ls + k], scale);
   }
@@ -253,16 +253,16 @@ __global__ __launch_bounds__(kFftThreads) void k_c2c_cols(
   const int c0 = (blk - img * groups) * nl;
   float2* base = buf + img * img_stride_c;
   fft_load_tw(P, tw);
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int y = i / nl, l = i - y * nl;
+  for (LineIter it(nl); it.l < N; it.next()) {
+    int y = it.l, l = it.k;
     float2 v = make_float2(0.f, 0.f);
     if (c0 + l < ncols) v = base[(size_t)y * cpitch + c0 + l];
     a[l * ls + y] = v;
   }
   __syncthreads();
   float2* z = fft_lines(a, b, P, nl, inv != 0, tw);
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int y = i / nl, l = i - y * nl;
+  for (LineIter it(nl); it.l < N; it.next()) {
+    int y = it.l, l = it.k;
     if (c0 + l < ncols)
       base[(size_t)y * cpitch + c0 + l] = cscale(z[l * ls + y], scale);
   }

@@ -6,6 +6,35 @@
 
 namespace mf {
 
+// Walks the (line, index) pairs l < nl, k < n of a workgroup's lines, one
+// pair per thread per step, without an integer division per step: the
+// block stride is split once into whole lines and a remainder.
+struct LineIter {
+  int l, k, dl, dk, n;
+  __device__ __forceinline__ explicit LineIter(int n_) : n(n_) {
+    l = (int)threadIdx.x / n;
+    k = (int)threadIdx.x - l * n;
+    dl = (int)blockDim.x / n;
+    dk = (int)blockDim.x - dl * n;
+  }
+  __device__ __forceinline__ void next() {
+    l += dl;
+    k += dk;
+    if (k >= n) {
+      k -= n;
+      l++;
+    }
+  }
+};
+
+// floor(a / d) for 0 <= a < 2^22 and d >= 1 from r = 1.0f / d: (a + 0.5)/d
+// lies at least 0.5/d from any integer, and the two roundings of the float
+// evaluation err by less than (a + 0.5)/d * 2^-23 < 0.5/d, so truncation
+// gives the exact quotient.
+__device__ __forceinline__ int qdiv(int a, float r) {
+  return (int)(((float)a + 0.5f) * r);
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -152,12 +181,12 @@ __device__ __forceinline__ float2* fft_stockham(float2* x, float2* y, int N,
     const int R = rad[s];
     const int NR = N / R;
     const int tstride = N / (Ns * R);
-    const int total = nl * NR;
-    for (int b = threadIdx.x; b < total; b += blockDim.x) {
-      const int l = b / NR, j = b - l * NR;
-      const int k = j % Ns;
-      const float2* src = x + l * ls + j;
-      float2* dst = y + l * ls + (j / Ns) * Ns * R + k;
+    const float rNs = 1.0f / (float)Ns;
+    for (LineIter it(NR); it.l < nl; it.next()) {
+      const int j = it.k;
+      const int q = qdiv(j, rNs), k = j - q * Ns;
+      const float2* src = x + it.l * ls + j;
+      float2* dst = y + it.l * ls + q * Ns * R + k;
       fft_butterfly(R, src, NR, dst, Ns, k * tstride, tw, N, inv);
     }
     __syncthreads();
@@ -192,8 +221,8 @@ __device__ __forceinline__ float2* fft_lines(float2* buf, float2* tmp,
   if (!P.blue)
     return fft_stockham(buf, tmp, P.n, P.n, P.nrad, P.rad, tw, nl, inv);
   const int n = P.n, m = P.m;
-  for (int i = threadIdx.x; i < nl * m; i += blockDim.x) {
-    const int l = i / m, k = i - l * m;
+  for (LineIter it(m); it.l < nl; it.next()) {
+    const int l = it.l, k = it.k;
     float2 v = make_float2(0.f, 0.f);
     if (k < n) {
       float2 c = P.chirp[k];
@@ -205,14 +234,12 @@ __device__ __forceinline__ float2* fft_lines(float2* buf, float2* tmp,
   float2* r = fft_stockham(buf, tmp, m, m, P.nrad, P.rad, tw, nl, false);
   float2* o = (r == buf) ? tmp : buf;
   const float2* f = inv ? P.bi : P.bf;
-  for (int i = threadIdx.x; i < nl * m; i += blockDim.x) {
-    const int k = i % m;
-    r[i] = cmul(r[i], f[k]);
-  }
+  for (LineIter it(m); it.l < nl; it.next())
+    r[it.l * m + it.k] = cmul(r[it.l * m + it.k], f[it.k]);
   __syncthreads();
   r = fft_stockham(r, o, m, m, P.nrad, P.rad, tw, nl, true);
-  for (int i = threadIdx.x; i < nl * m; i += blockDim.x) {
-    const int l = i / m, k = i - l * m;
+  for (LineIter it(m); it.l < nl; it.next()) {
+    const int l = it.l, k = it.k;
     if (k < n) {
       float2 c = P.chirp[k];
       r[l * m + k] = cmul(r[l * m + k], inv ? cconj(c) : c);
@@ -230,7 +257,7 @@ __device__ __forceinline__ void r2c_unpack(const float2* Z, int N,
                                            const float2* __restrict__ half,
                                            float scale, int k, Out out) {
   const float2 zk = Z[k == N ? 0 : k];
-  const float2 zn = cconj(Z[(N - k) % N]);
+  const float2 zn = cconj(Z[k == 0 ? 0 : N - k]);
   const float2 e = cscale(cadd(zk, zn), 0.5f);
   const float2 o = cscale(csub(zk, zn), 0.5f);
   // X_k = E + (-i) w_k O,  w_k = exp(-i pi k / N)

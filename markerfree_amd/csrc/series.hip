@@ -22,7 +22,8 @@ mf_series::~mf_series() {
   for (mf::DevBuf* b : {&raw, &stack, &tmp, &tr_corr, &tr_work, &tr_mom,
                         &tr_shift, &tr_tab, &cproj, &off_corr, &pa_corr,
                         &pa_reproj, &pa_work, &pa_mom, &pa_dev,
-                        &pa_slab, &axw.lines, &axw.reg, &axw.small, &offw.buf})
+                        &pa_slab, &axw.lines, &axw.reg, &axw.small,
+                        &axw.tstack, &offw.buf})
     b->release();
   if (st) (void)hipStreamDestroy(st);
 }
@@ -175,6 +176,7 @@ void series_preprocess(mf_series* s) {
   s->sy.assign(nz, 0.0);
   s->rot.assign(nz, 0.0f);
   s->offset = 0.0f;
+  s->axw.t_of = nullptr;  // the stack's pixels are new
   s->preprocessed = true;
 }
 
@@ -503,9 +505,9 @@ size_t series_hbm_bytes(int nx, int ny, int nz, const mf_options* o) {
               (size_t)4 * 3 * gp.by * pad_x(gp.bx);
   if (o->reproj_mode == 0)
     pa += (size_t)4 * 2 * gp.bx * gp.by * (o->align_z / pb / 2 * 2);
-  // Common lines (lines and spectra of every image and candidate), offset
-  // scan at bin 4, final correction, small tables.
-  const size_t axis = (size_t)8 * nz * 100 * (std::min(nx, ny) + 2);
+  // Common lines (lines and spectra of every image and candidate, the
+  // transposed stack), offset scan at bin 4, final correction, small tables.
+  const size_t axis = (size_t)8 * nz * 100 * (std::min(nx, ny) + 2) + raw;
   const CorrGeom go = CorrGeom::make(nx, ny, std::max(o->out_bin, 1));
   const size_t out = (size_t)4 * go.bx * go.by * nz;
   // Stage buffers stay allocated for the life of the series: sum them.

@@ -190,16 +190,16 @@ __global__ __launch_bounds__(kRowThreads) void k_rweight(
   const int r0 = blockIdx.x * nl;
   const int nv = min(nl, nrows - r0);
   fft_load_tw(P, tw);
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     float2 v = make_float2(0.f, 0.f);
     if (l < nv) v = reinterpret_cast<const float2*>(in + (size_t)(r0 + l) * pitch)[k];
     a[l * ls + k] = v;
   }
   __syncthreads();
   float2* Z = fft_lines(a, b, P, nl, false, tw);
-  for (int i = threadIdx.x; i < nl * xs; i += blockDim.x) {
-    int l = i / xs, k = i - l * xs;
+  for (LineIter it(xs); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     r2c_unpack(Z + l * ls, N, P.half, 1.0f, k, [&](int kk, float2 v) {
       // GRWeight (CorrStack.cu:564-579): x /= N, then x *= ramp.
       const float f = filt[kk];
@@ -207,14 +207,14 @@ __global__ __launch_bounds__(kRowThreads) void k_rweight(
     });
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     a[l * ls + k] = c2r_pack(X[l * xs + k], X[l * xs + N - k], P.half[k]);
   }
   __syncthreads();
   float2* z = fft_lines(a, b, P, nl, true, tw);
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     if (l >= nv) continue;
     reinterpret_cast<float2*>(out + (size_t)(r0 + l) * 2 * N)[k] = z[l * ls + k];
   }
@@ -239,8 +239,8 @@ __global__ __launch_bounds__(kRowThreads) void k_correct_rweight(
   const CorrProj pr = ca.proj[z];
   const float* raw = ca.raw + (size_t)z * ca.nx * ca.ny;
   fft_load_tw(P, tw);
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     float2 v = make_float2(0.f, 0.f);
     if (l < nv)
       v = make_float2(corr_pixel(ca, pr, raw, 2 * k, r0 + l),
@@ -249,23 +249,23 @@ __global__ __launch_bounds__(kRowThreads) void k_correct_rweight(
   }
   __syncthreads();
   float2* Z = fft_lines(a, b, P, nl, false, tw);
-  for (int i = threadIdx.x; i < nl * xs; i += blockDim.x) {
-    int l = i / xs, k = i - l * xs;
+  for (LineIter it(xs); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     r2c_unpack(Z + l * ls, N, P.half, 1.0f, k, [&](int kk, float2 v) {
       const float f = filt[kk];
       X[l * xs + kk] = make_float2((v.x / fn) * f, (v.y / fn) * f);
     });
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     a[l * ls + k] = c2r_pack(X[l * xs + k], X[l * xs + N - k], P.half[k]);
   }
   __syncthreads();
   float2* zz = fft_lines(a, b, P, nl, true, tw);
   float* out = ca.out + (size_t)blockIdx.y * ca.out_stride;
-  for (int i = threadIdx.x; i < nl * N; i += blockDim.x) {
-    int l = i / N, k = i - l * N;
+  for (LineIter it(N); it.l < nl; it.next()) {
+    int l = it.l, k = it.k;
     if (l >= nv) continue;
     reinterpret_cast<float2*>(out + (size_t)(r0 + l) * 2 * N)[k] =
         zz[l * ls + k];
@@ -432,10 +432,18 @@ void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
   const int ls = fft_lds_len(P);
   const size_t per = (size_t)(2 * ls + P.n + 1) * sizeof(float2);
   // Rows per workgroup: within 48 KB of LDS, and no more than keeps about
-  // two workgroups per CU busy.
-  const long long want = (long long)g.by * count / (2LL * std::max(ctx->num_cu, 1));
-  const int fit = (int)((48 * 1024 - (size_t)ls * sizeof(float2)) / per);
-  const int nl = (int)std::max(1LL, std::min<long long>({want, (long long)fit, 16LL}));
+  // eight workgroups per CU busy; 256 threads (the per-pixel correction
+  // gathers dominate).  Tunables "corr_rw/<bx>x<by>x<count>/lines" and
+  // ".../threads" override the heuristic.
+  const std::string key = "corr_rw/" + std::to_string(g.bx) + "x" +
+                          std::to_string(g.by) + "x" + std::to_string(count) + "/";
+  const long long want = (long long)g.by * count / (8LL * std::max(ctx->num_cu, 1));
+  const int fit = std::max(1, (int)((48 * 1024 - (size_t)ls * sizeof(float2)) / per));
+  int nl = (int)std::max(1LL, std::min<long long>({want, (long long)fit, 16LL}));
+  nl = std::max(1, std::min(tune_get(ctx, key + "lines", nl), fit));
+  int threads = tune_get(ctx, key + "threads", kRowThreads);
+  MF_REQUIRE(threads % 64 == 0 && threads <= kRowThreads,
+             "workgroup size must be a multiple of 64 up to 256");
   const size_t lds = nl * per + (size_t)ls * sizeof(float2);
   MF_REQUIRE(lds <= 160 * 1024, "R-weighting row exceeds LDS");
   if (lds > 64 * 1024)
@@ -445,7 +453,7 @@ void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
   const float* filt = rweight_filter(ctx, g.bx);
   TimedLaunch tl(ctx->timer, "correct_rweight", st);
   hipLaunchKernelGGL(k_correct_rweight, dim3(cdiv(g.by, nl), count),
-                     dim3(kRowThreads), lds, st, a, P, nl, filt);
+                     dim3(threads), lds, st, a, P, nl, filt);
   MF_HIP(hipGetLastError());
 }
 

@@ -173,6 +173,23 @@ def tilt_axis(stack, tilts, sx, sy, rot, arange, num=100):
     return float(axis), scores, int(L.value)
 
 
+_sig("orc_common_lines", C.c_int, _fp, C.c_int, C.c_int, C.c_int, _fp, _fp,
+     _fp, C.c_float, C.c_float, C.c_int, _fp)
+
+
+def common_lines(stack, tilts, sx, sy, axis0, arange, num=100):
+    """GRadon lines [nz][num][L] over the CLsetup grid around axis0."""
+    stack = np.ascontiguousarray(stack, np.float32)
+    nz, ny, nx = stack.shape
+    a = [np.ascontiguousarray(v, np.float32) for v in (tilts, sx, sy)]
+    L = _lib.orc_common_lines(_f(stack), nx, ny, nz, *map(_f, a), axis0,
+                              arange, num, None)
+    out = np.zeros((nz, num, L), np.float32)
+    _lib.orc_common_lines(_f(stack), nx, ny, nz, *map(_f, a), axis0, arange,
+                          num, _f(out))
+    return out
+
+
 _sig("orc_offset_correlate", C.c_float, _fp, _fp, C.c_int, C.c_int,
      C.c_float, C.c_float)
 

@@ -81,6 +81,14 @@ float orc_tilt_axis(const float* stack, int nx, int ny, int nz,
                     const float* shift_y, const float* rotate, float range,
                     int num, float* scores, int* line_size);
 
+/* CalcTIltAxis::Radon (Rotate.cu:397-428, GRadon :83-136) over the CLsetup
+ * grid around axis0: lines [nz][num][L] of mean values; returns L (lines
+ * NULL: L only). */
+int orc_common_lines(const float* stack, int nx, int ny, int nz,
+                     const float* tilts, const float* shift_x,
+                     const float* shift_y, float axis0, float range, int num,
+                     float* lines);
+
 /* ---- tilt-angle offset (FindOffset.cu) ---- */
 float orc_find_offset(const float* stack, int nx, int ny, int nz,
                       const float* tilts, const float* shift_x,

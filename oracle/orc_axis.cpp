@@ -83,16 +83,15 @@ std::vector<float> line_taper(int L) {
   return t;
 }
 
-// Radon + FFT1d for one image (Rotate.cu:397-451, GRadon 83-136): one
-// spectrum of L/2+1 complex values per candidate.
-void image_lines(const AxisSetup& a, const std::vector<int>& reg,
+// GRadon (Rotate.cu:83-136) of one image: the mean line values of candidate
+// k, L values into line.
+void radon_lines(const AxisSetup& a, const std::vector<int>& reg,
                  const float* img, int nx, int ny, float tilt, float sx,
-                 float sy, const std::vector<float>& taper, float* spec) {
-  const int num = (int)a.rot.size(), L = a.L, nc = L / 2 + 1;
+                 float sy, int k, float* line) {
+  const int L = a.L;
   const float ct = (float)std::cos(tilt * kD2R);
   const float ox = (float)nx * 0.5f + sx, oy = (float)ny * 0.5f + sy;
-  std::vector<float> line(L);
-  for (int k = 0; k < num; k++) {
+  {
     const float c = a.cosr[k], s = a.sinr[k];
     for (int y = 0; y < L; y++) {
       const float fy = (float)y - (float)L * 0.5f;
@@ -115,6 +114,18 @@ void image_lines(const AxisSetup& a, const std::vector<int>& reg,
       for (int x = 0; x < right; x++) sample(x);
       line[y] = cnt == 0 ? invalid() : sum / (float)cnt;
     }
+  }
+}
+
+// Radon + FFT1d for one image (Rotate.cu:397-451): one spectrum of L/2+1
+// complex values per candidate.
+void image_lines(const AxisSetup& a, const std::vector<int>& reg,
+                 const float* img, int nx, int ny, float tilt, float sx,
+                 float sy, const std::vector<float>& taper, float* spec) {
+  const int num = (int)a.rot.size(), L = a.L, nc = L / 2 + 1;
+  std::vector<float> line(L);
+  for (int k = 0; k < num; k++) {
+    radon_lines(a, reg, img, nx, ny, tilt, sx, sy, k, line.data());
     // GCalcMean + GRemoveMean (Rotate.cu:138-198).
     double ms = 0.0;
     int mc = 0;
@@ -135,6 +146,24 @@ void image_lines(const AxisSetup& a, const std::vector<int>& reg,
 }  // namespace orc
 
 using namespace orc;
+
+extern "C" int orc_common_lines(const float* stack, int nx, int ny, int nz,
+                                const float* tilts, const float* shift_x,
+                                const float* shift_y, float axis0, float range,
+                                int num, float* lines) {
+  const int z0 = zero_tilt_index(tilts, nz, 0.0f);
+  AxisSetup a = axis_setup(nx, ny, axis0, range, num);
+  if (!lines) return a.L;
+  std::vector<int> reg;
+  common_region(a, nx, ny, shift_x[z0], shift_y[z0], reg);
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+  for (int z = 0; z < nz; z++)
+    for (int k = 0; k < num; k++)
+      radon_lines(a, reg, stack + (size_t)z * nx * ny, nx, ny, tilts[z],
+                  shift_x[z], shift_y[z], k,
+                  lines + ((size_t)z * num + k) * a.L);
+  return a.L;
+}
 
 extern "C" float orc_tilt_axis(const float* stack, int nx, int ny, int nz,
                                const float* tilts, const float* shift_x,
