@@ -11,6 +11,7 @@
 // reference's intermediate buf.mrc is not written: the preprocessed stack
 // stays in device memory.
 #include <getopt.h>
+#include <unistd.h>
 #include <sys/time.h>
 
 #include <cstdio>
@@ -249,6 +250,29 @@ int run(const Options& o) {
   if (mf_ctx_create(o.gpu, 0, &ctx) != MF_OK) {
     printf("[Error] Cannot use GPU %d\n", o.gpu);
     return 1;
+  }
+  // Launch shapes from the autotuner's cache: MF_TUNING names a file ("" to
+  // keep the built-in heuristics); by default the cache installed beside the
+  // library, <bin>/../tuned/gfx950.json, when it exists.
+  {
+    std::string path;
+    const char* env = getenv("MF_TUNING");
+    if (env) {
+      path = env;
+    } else {
+      char exe[4096];
+      const ssize_t n = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+      if (n > 0) {
+        exe[n] = 0;
+        std::string dir(exe);
+        dir = dir.substr(0, dir.rfind('/'));
+        path = dir + "/../tuned/gfx950.json";
+        if (access(path.c_str(), R_OK) != 0) path.clear();
+      }
+    }
+    if (!path.empty() && mf_tune_load(ctx, path.c_str(), nullptr) != MF_OK)
+      printf("[Warning] tuning file %s not applied: %s\n", path.c_str(),
+             mf_last_error(ctx));
   }
   mf_series* s = nullptr;
   auto die = [&](const char* what) {
