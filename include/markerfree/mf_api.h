@@ -53,6 +53,14 @@ const char* mf_version(void);
 /* Plumbing for callers that own device buffers (tests, benchmarks). */
 mf_status mf_dev_malloc(mf_ctx* ctx, size_t bytes, void** d_out);
 mf_status mf_dev_free(mf_ctx* ctx, void* d_ptr);
+/* Page-locked host memory (hipHostMalloc): stacks uploaded from it and
+ * aligned stacks downloaded into it (mf_series_create, mf_correct,
+ * mf_fetch) move by DMA at the full PCIe rate instead of through staging
+ * copies.  Independent of any context.  MF_ENOMEM when the allocation fails;
+ * mf_host_free(NULL) is a no-op.  No counterpart in the reference (it reads
+ * and writes files). */
+mf_status mf_host_alloc(size_t bytes, void** h_out);
+mf_status mf_host_free(void* h_ptr);
 mf_status mf_h2d(mf_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
 mf_status mf_d2h(mf_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
 mf_status mf_d2d(mf_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
@@ -253,8 +261,9 @@ size_t mf_series_hbm_bytes(int nx, int ny, int nz, const mf_options* o);
  * start when a worker is free and their byte bound fits next to the
  * running series within 'hbm_budget' (0: the context's budget, else the
  * device memory free at creation less 1/16).  Raw stacks are copied at
- * submission; results (parameters and the OutBin aligned stack) stay in
- * host memory until released. */
+ * submission into page-locked host memory; results (parameters and the
+ * OutBin aligned stack, also page-locked) stay in host memory until
+ * released. */
 typedef struct mf_server mf_server;
 enum { MF_REQ_QUEUED = 0, MF_REQ_RUNNING = 1, MF_REQ_DONE = 2,
        MF_REQ_FAILED = 3 };
@@ -263,7 +272,7 @@ mf_status mf_server_create(mf_ctx* ctx, int workers, size_t hbm_budget,
 /* Waits for running series; queued ones fail with "server stopped". */
 void mf_server_destroy(mf_server* sv);
 /* MF_EBUSY: 'queue_cap' requests already wait; MF_ENOMEM: the series could
- * never fit the budget. */
+ * never fit the budget, or no page-locked host memory is left for its copy. */
 mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
                     const float* stack_zyx, const float* tilts,
                     const mf_options* o, long long* id);

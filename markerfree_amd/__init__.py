@@ -61,6 +61,8 @@ _sig("mf_d2d", C.c_int, _vp, _vp, _vp, C.c_size_t)
 _sig("mf_timing_enable", C.c_int, _vp, C.c_int)
 _sig("mf_timing_reset", C.c_int, _vp)
 _sig("mf_timing_select", C.c_int, _vp, C.c_char_p)
+_sig("mf_host_alloc", C.c_int, C.c_size_t, C.POINTER(_vp))
+_sig("mf_host_free", C.c_int, _vp)
 _sig("mf_tune_set", C.c_int, _vp, C.c_char_p, C.c_int)
 _sig("mf_tune_get", C.c_int, _vp, C.c_char_p, C.POINTER(C.c_int))
 _sig("mf_tune_load", C.c_int, _vp, C.c_char_p, C.POINTER(C.c_int))
@@ -104,6 +106,23 @@ def f32p(a: np.ndarray):
 def u8p(a: np.ndarray):
     assert a.dtype == np.uint8 and a.flags.c_contiguous
     return a.ctypes.data_as(_u8p)
+
+
+def pinned_empty(shape, dtype=np.float32) -> np.ndarray:
+    """A numpy array in page-locked host memory (mf_host_alloc), freed when
+    the array and its views are collected; uploads from it and downloads into
+    it run at the full PCIe rate."""
+    import weakref
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape))
+    n = count * dt.itemsize
+    p = _vp()
+    st = _lib.mf_host_alloc(n, C.byref(p))
+    if st:
+        raise MFError(st, f"mf_host_alloc({n}) failed")
+    buf = (C.c_char * max(n, 1)).from_address(p.value)
+    weakref.finalize(buf, _lib.mf_host_free, p.value)
+    return np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
 
 
 def find_proj_range(tilts, skip, target, nproj):
@@ -217,6 +236,9 @@ class Context:
 
     def empty(self, shape, dtype=np.float32) -> DeviceArray:
         return DeviceArray(self, shape, dtype)
+
+    def pinned_empty(self, shape, dtype=np.float32) -> np.ndarray:
+        return pinned_empty(shape, dtype)
 
     # ---- timing
     def timing(self, on: bool = True, only: str | None = None):
@@ -455,11 +477,17 @@ class Series:
     def projalign_once(self):
         self._check(_lib.mf_projalign_once(self.h))
 
-    def correct(self, out_bin=1):
+    def correct(self, out_bin=1, out=None):
+        """The aligned stack [nz][by][bx]; 'out', if given, receives it (for
+        example a Context.pinned_empty array reused across series)."""
         bx, by = C.c_int(), C.c_int()
         self._check(_lib.mf_correct(self.h, out_bin, None, C.byref(bx),
                                     C.byref(by)))
-        out = np.empty((self.nz, by.value, bx.value), np.float32)
+        shape = (self.nz, by.value, bx.value)
+        if out is None:
+            out = np.empty(shape, np.float32)
+        if out.shape != shape:
+            raise ValueError(f"out has shape {out.shape}, expected {shape}")
         self._check(_lib.mf_correct(self.h, out_bin, f32p(out), None, None))
         return out
 

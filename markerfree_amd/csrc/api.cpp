@@ -127,6 +127,21 @@ mf_status mf_dev_free(mf_ctx* ctx, void* p) {
   });
 }
 
+mf_status mf_host_alloc(size_t bytes, void** h_out) {
+  if (!h_out) return MF_EINVAL;
+  *h_out = nullptr;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess)
+    return MF_ENOMEM;
+  *h_out = p;
+  return MF_OK;
+}
+
+mf_status mf_host_free(void* p) {
+  if (!p) return MF_OK;
+  return hipHostFree(p) == hipSuccess ? MF_OK : MF_EINVAL;
+}
+
 mf_status mf_h2d(mf_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (!ctx || (!dst && bytes) || (!src && bytes)) return MF_EINVAL;
   return guard(ctx, [&] {

@@ -42,17 +42,50 @@ double seconds_since(Clock::time_point t0) {
   return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
+// Page-locked float buffer (mf_host_alloc): the request's stack copy and its
+// aligned stack move to and from the device by DMA at the PCIe rate.
+class PinnedFloats {
+ public:
+  PinnedFloats() = default;
+  PinnedFloats(const PinnedFloats&) = delete;
+  PinnedFloats& operator=(const PinnedFloats&) = delete;
+  ~PinnedFloats() { clear(); }
+  // Returns false when page-locked memory cannot be had.
+  bool resize(size_t n) {
+    clear();
+    void* q = nullptr;
+    if (n && mf_host_alloc(n * sizeof(float), &q) != MF_OK) return false;
+    p_ = (float*)q;
+    n_ = n;
+    return true;
+  }
+  void clear() {
+    if (p_) mf_host_free(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  float* data() { return p_; }
+  const float* data() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  float* p_ = nullptr;
+  size_t n_ = 0;
+};
+
 struct Request {
   long long id = 0;
   int nx = 0, ny = 0, nz = 0;
-  std::vector<float> stack, tilts;  // host copies (file order)
+  PinnedFloats stack;               // host copy (file order)
+  std::vector<float> tilts;
   mf_options opt{};
   size_t bytes = 0;                 // admission charge
   int state = MF_REQ_QUEUED;
   std::string error;
   // Results.
   std::vector<double> sx, sy;
-  std::vector<float> rot, aligned;
+  std::vector<float> rot;
+  PinnedFloats aligned;
   float offset = 0.0f;
   int bx = 0, by = 0;
   double t_submit = 0, t_start = 0, t_done = 0;  // seconds since server start
@@ -96,7 +129,12 @@ void process(mf_ctx* ctx, Request& r) {
   if (mf_run(s, &r.opt) != MF_OK) return fail();
   if (mf_correct(s, r.opt.out_bin, nullptr, &r.bx, &r.by) != MF_OK)
     return fail();
-  r.aligned.resize((size_t)r.bx * r.by * r.nz);
+  if (!r.aligned.resize((size_t)r.bx * r.by * r.nz)) {
+    mf_series_destroy(s);
+    r.error = "no page-locked host memory for the aligned stack";
+    r.state = MF_REQ_FAILED;
+    return;
+  }
   if (mf_correct(s, r.opt.out_bin, r.aligned.data(), &r.bx, &r.by) != MF_OK)
     return fail();
   r.sx.resize(r.nz);
@@ -146,7 +184,7 @@ void worker_loop(mf_server* sv) {
       sv->running--;
       sv->completed++;
       r->t_done = seconds_since(sv->t0);
-      std::vector<float>().swap(r->stack);
+      r->stack.clear();
     }
     sv->cv_work.notify_all();
     sv->cv_done.notify_all();
@@ -267,7 +305,9 @@ mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
     if (r->bytes > sv->budget) return MF_ENOMEM;  // could never be admitted
     if ((int)sv->queue.size() >= sv->queue_cap) return MF_EBUSY;
   }
-  r->stack.assign(stack_zyx, stack_zyx + (size_t)nx * ny * nz);
+  const size_t n = (size_t)nx * ny * nz;
+  if (!r->stack.resize(n)) return MF_ENOMEM;
+  std::memcpy(r->stack.data(), stack_zyx, n * sizeof(float));
   r->tilts.assign(tilts, tilts + nz);
   {
     std::lock_guard<std::mutex> lk(sv->mu);

@@ -102,3 +102,22 @@ def test_gpu_server_admission_and_limits(ctx, mf):
         spans = sorted((r.t_start, r.t_done) for r in rs)
         for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
             assert b0 >= a1 - 1e-6        # serialised by the budget
+
+
+@pytest.mark.gpu
+def test_gpu_pinned_host_buffers_same_results(ctx, mf):
+    """A stack uploaded from page-locked memory and an aligned stack
+    downloaded into it (mf_host_alloc) equal the pageable paths."""
+    s = _mix()[0]
+    ref = _alone(mf, ctx, s)
+    pin = mf.pinned_empty(s.stack.shape)
+    pin[:] = s.stack
+    G = mf.Series(ctx, pin, s.tilts)
+    G.run(**OPTS)
+    out = mf.pinned_empty(ref[1].shape)
+    got = G.correct(1, out=out)
+    assert got is out
+    _same(ref, (G.params(), got))
+    with pytest.raises(ValueError):
+        G.correct(1, out=np.empty((1, 2, 3), np.float32))
+    G.close()
