@@ -19,7 +19,7 @@ import sys
 
 import numpy as np
 
-from . import Context
+from . import Context, Series
 
 LINES = (1, 2, 4, 8)
 THREADS = (64, 128, 256)
@@ -90,6 +90,18 @@ def tune(size=(1024, 1024), nz=61, reps=20, log=print):
                                      1, True, dout), reps, log))
         for d in (da, db, draw, dout):
             d.free()
+        # Coarse alignment: the correlation of all adjacent pairs of an
+        # nz-image series at this size, one sweep per timing.
+        stack = (rng.random((nz, h, w)) + 0.5).astype(np.float32)
+        tilts = (-60.0 + 120.0 / (nz - 1) * np.arange(nz)).astype(np.float32)
+        s = Series(ctx, stack, tilts)
+        s.preprocess()
+        s.coarse_setup()
+        pre = f"/{w}x{h}x{nz - 1}/"
+        out.update(_descend(ctx, ["cc_rowsf" + pre, "cc_cols" + pre,
+                                  "cc_rowsi" + pre], "cc",
+                            lambda: s.coarse_iter(), max(2, reps // 10), log))
+        s.close()
     return out
 
 

@@ -95,6 +95,26 @@ struct RadonImg {
 // [nx][ny]), where it runs along them.  Samples inside the analytic
 // in-image interval of the line (with a one-pixel margin) skip the bounds
 // test; the others take it.
+// Pixel (ix, iy) of one image: row-major, or from the transposed copy.
+// 24-bit products (both factors are below 2^24 and the index below 2^32) and
+// a 32-bit byte offset from the image base keep the address arithmetic at
+// full rate.
+template <bool TRANS>
+__device__ __forceinline__ float radon_pix(const float* img, int nx, int ny,
+                                           int ix, int iy) {
+  const unsigned e = TRANS ? __umul24((unsigned)ix, (unsigned)ny) + (unsigned)iy
+                           : __umul24((unsigned)iy, (unsigned)nx) + (unsigned)ix;
+  return *reinterpret_cast<const float*>(
+      reinterpret_cast<const char*>(img) + (e << 2));
+}
+
+template <bool TRANS>
+__device__ __forceinline__ void radon_line(const float* img, int nx, int ny,
+                                           float c, float s, float fy,
+                                           const RadonImg& im, int left,
+                                           int right, float* sum_out,
+                                           int* cnt_out);
+
 __global__ __launch_bounds__(256) void k_radon(const float* stack,
                                                const float* tstack, int nx,
                                                int ny, int num, int L,
@@ -112,14 +132,29 @@ __global__ __launch_bounds__(256) void k_radon(const float* stack,
   const float c = cs[2 * k], s = cs[2 * k + 1];
   const bool trans = tstack && fabsf(c) > fabsf(s);
   const float* img = (trans ? tstack : stack) + (size_t)z * nx * ny;
-  // Element (ix, iy) is at iy*sy + ix*sx.
-  const int sxs = trans ? ny : 1, sys = trans ? 1 : nx;
   const float fy = (float)y - (float)L * 0.5f;
   const int* r = reg + 2 * ((size_t)k * L + y);
-  float sum = 0.0f;
-  int cnt = 0;
   const int left = (int)((float)r[0] * im.ct + 0.5f);
   const int right = (int)((float)r[1] * im.ct + 0.5f);
+  float sum;
+  int cnt;
+  if (trans)
+    radon_line<true>(img, nx, ny, c, s, fy, im, left, right, &sum, &cnt);
+  else
+    radon_line<false>(img, nx, ny, c, s, fy, im, left, right, &sum, &cnt);
+  lines[((size_t)z * num + k) * pitch + y] = cnt == 0 ? kInvalid : sum / (float)cnt;
+}
+
+// The sequential sum of one line over x in [left, right) (left <= 0 <=
+// right, as the reference's two loops [left, 0) and [0, right) run).
+template <bool TRANS>
+__device__ __forceinline__ void radon_line(const float* img, int nx, int ny,
+                                           float c, float s, float fy,
+                                           const RadonImg& im, int left,
+                                           int right, float* sum_out,
+                                           int* cnt_out) {
+  float sum = 0.0f;
+  int cnt = 0;
   const float fys = fy * s, fyc = fy * c;
   const float fnx = (float)nx, fny = (float)ny;
   // In-image interval [xa, xb) from the exact linear forms with a one-pixel
@@ -145,17 +180,17 @@ __global__ __launch_bounds__(256) void k_radon(const float* stack,
     xa = (int)fmax((double)left, fmin((double)right, ceil(lo)));
     xb = (int)fmax((double)xa, fmin((double)right, floor(hi) + 1.0));
   }
-  // x runs over [left, 0) then [0, right) as the reference's two loops do;
-  // left <= 0 <= right.  Branch-free so that the loads of several samples
-  // are in flight at once: a skipped sample adds +0.0f to a sum that is
-  // never -0, which leaves it unchanged.
+  // Branch-free so that the loads of several samples are in flight at once:
+  // a skipped sample adds +0.0f to a sum that is never -0, which leaves it
+  // unchanged.
   auto tested = [&](int from, int to) {
 #pragma unroll 4
     for (int x = from; x < to; x++) {
       const float X = (float)x * c - fys + im.ox;
       const float Y = (float)x * s + fyc + im.oy;
       const bool in = X >= 0.f && X < fnx && Y >= 0.f && Y < fny;
-      const float v = img[in ? (int)Y * sys + (int)X * sxs : 0];
+      const float v =
+          radon_pix<TRANS>(img, nx, ny, in ? (int)X : 0, in ? (int)Y : 0);
       const bool ok = in && !(v < 0.f);
       sum += ok ? v : 0.0f;
       cnt += ok ? 1 : 0;
@@ -166,13 +201,14 @@ __global__ __launch_bounds__(256) void k_radon(const float* stack,
   for (int x = xa; x < xb; x++) {
     const float X = (float)x * c - fys + im.ox;
     const float Y = (float)x * s + fyc + im.oy;
-    const float v = img[(int)Y * sys + (int)X * sxs];
+    const float v = radon_pix<TRANS>(img, nx, ny, (int)X, (int)Y);
     const bool ok = !(v < 0.f);
     sum += ok ? v : 0.0f;
     cnt += ok ? 1 : 0;
   }
   tested(xb, right);
-  lines[((size_t)z * num + k) * pitch + y] = cnt == 0 ? kInvalid : sum / (float)cnt;
+  *sum_out = sum;
+  *cnt_out = cnt;
 }
 
 // [nz][ny][nx] -> [nz][nx][ny] in 32 x 32 tiles staged in LDS.

@@ -484,10 +484,11 @@ void set_lds(const void* k, size_t bytes) {
 
 // Launch shape of one line-transform kernel: lines per workgroup and
 // workgroup size.  Tunables "<family>/<w>x<h>x<npair>/lines" and
-// ".../threads" override the heuristic: as many lines as keep about 'per_cu' workgroups
-// per CU over 'total' lines, within 96 KB of LDS (per_line bytes each,
-// 'fixed' once) and 'cap'; one thread per radix-8 butterfly of those lines,
-// 128 to kCcThreads.
+// ".../threads" override the heuristic: one line per workgroup (measured
+// fastest for these latency-bound transforms) unless 'total' lines would
+// give fewer than 'per_cu' workgroups per CU, within 96 KB of LDS
+// (per_line bytes each, 'fixed' once) and 'cap'; one thread per radix-8
+// butterfly of those lines, 128 to kCcThreads.
 struct Shape {
   int nl, threads;
 };
@@ -496,7 +497,8 @@ Shape cc_shape(mf_ctx* ctx, const char* family, int w, int h, int npair,
                int cap, int n) {
   const std::string key = std::string(family) + "/" + std::to_string(w) + "x" +
                           std::to_string(h) + "x" + std::to_string(npair) + "/";
-  const long long want = total / ((long long)per_cu * std::max(ctx->num_cu, 1));
+  const long long slots = (long long)per_cu * std::max(ctx->num_cu, 1);
+  const long long want = total >= slots ? 1 : slots / std::max(total, 1LL);
   const long long fit = (long long)((96 * 1024 - fixed) / per_line);
   Shape s;
   s.nl = (int)std::max(1LL, std::min<long long>({want, fit, (long long)cap}));

@@ -98,6 +98,13 @@ __global__ void k_scale(float* d, size_t plane, const float* scale) {
   }
 }
 
+// Raw pixel (x, y) of an nx-wide image: 24-bit product (both factors below
+// 2^24, the index below 2^32) and a 32-bit offset, at full rate.
+__device__ __forceinline__ float raw_pix(const float* raw, int nx, int x,
+                                         int y) {
+  return raw[__umul24((unsigned)y, (unsigned)nx) + (unsigned)x];
+}
+
 // mGRandom (CorrStack.cu:116-140): window 51, next = (19 next + 57) mod 2601.
 __device__ float corr_random(int x, int y, int nx, int ny, const float* raw) {
   if (x < 0) x = -x;
@@ -111,9 +118,9 @@ __device__ float corr_random(int x, int y, int nx, int ny, const float* raw) {
     if (ix < 0 || ix >= nx) continue;
     int iy = (int)(next / 51u) - 25 + y;
     if (iy < 0 || iy >= ny) continue;
-    return raw[(size_t)iy * nx + ix];
+    return raw_pix(raw, nx, ix, iy);
   }
-  return raw[(size_t)y * nx + x];
+  return raw_pix(raw, nx, x, y);
 }
 
 struct CorrArgs {
@@ -146,7 +153,7 @@ __device__ float corr_pixel(const CorrArgs& a, const CorrProj& pr,
       const int rx = (int)(qx + 0.5f), ry = (int)(qy + 0.5f);
       float v;
       if (rx >= 0 && rx < a.nx && ry >= 0 && ry < a.ny)
-        v = raw[(size_t)ry * a.nx + rx];
+        v = raw_pix(raw, a.nx, rx, ry);
       else if (a.fill)
         v = corr_random(rx, ry, a.nx, a.ny, raw);
       else
