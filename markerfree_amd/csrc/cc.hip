@@ -485,24 +485,22 @@ void set_lds(const void* k, size_t bytes) {
 // Launch shape of one line-transform kernel: lines per workgroup and
 // workgroup size.  Tunables "<family>/<w>x<h>x<npair>/lines" and
 // ".../threads" override the heuristic: one line per workgroup (measured
-// fastest for these latency-bound transforms) unless 'total' lines would
-// give fewer than 'per_cu' workgroups per CU, within 96 KB of LDS
-// (per_line bytes each, 'fixed' once) and 'cap'; one thread per radix-8
-// butterfly of those lines, 128 to kCcThreads.
+// fastest for these latency-bound transforms, for one pair and for a whole
+// sweep of pairs alike), at most what 96 KB of LDS holds (per_line bytes
+// each, 'fixed' once); one thread per radix-8 butterfly of those lines, 128
+// to kCcThreads.
 struct Shape {
   int nl, threads;
 };
 Shape cc_shape(mf_ctx* ctx, const char* family, int w, int h, int npair,
-               long long total, int per_cu, size_t per_line, size_t fixed,
-               int cap, int n) {
+               size_t per_line, size_t fixed, int n) {
   const std::string key = std::string(family) + "/" + std::to_string(w) + "x" +
                           std::to_string(h) + "x" + std::to_string(npair) + "/";
-  const long long slots = (long long)per_cu * std::max(ctx->num_cu, 1);
-  const long long want = total >= slots ? 1 : slots / std::max(total, 1LL);
-  const long long fit = (long long)((96 * 1024 - fixed) / per_line);
+  const int fit =
+      std::max(1, (int)(((size_t)96 * 1024 - std::min(fixed, (size_t)64 * 1024)) /
+                        per_line));
   Shape s;
-  s.nl = (int)std::max(1LL, std::min<long long>({want, fit, (long long)cap}));
-  s.nl = std::max(1, std::min(tune_get(ctx, key + "lines", s.nl), (int)fit));
+  s.nl = std::max(1, std::min(tune_get(ctx, key + "lines", 1), fit));
   const int t = (s.nl * std::max(n / 8, 1) + 63) / 64 * 64;
   s.threads = std::max(128, std::min(t, kCcThreads));
   s.threads = tune_get(ctx, key + "threads", s.threads);
@@ -543,8 +541,8 @@ void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
   const FftDevPlan& PR = ctx->fft->plan(w / 2);
   const size_t lr = fft_lds_len(PR);
   {
-    const Shape sh = cc_shape(ctx, "cc_rowsf", w, h, npair, (long long)nimg * h, 8,
-                              2 * lr * 8, lr * 8, 16, (int)lr);
+    const Shape sh = cc_shape(ctx, "cc_rowsf", w, h, npair, 2 * lr * 8,
+                              lr * 8, (int)lr);
     const size_t lds = (2 * sh.nl * lr + lr) * sizeof(float2);
     set_lds((const void*)k_cc_rows_fwd<TAB>, lds);
     hipLaunchKernelGGL(k_cc_rows_fwd<TAB>, dim3(cdiv(h, sh.nl), nimg),
@@ -557,8 +555,8 @@ void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
     const size_t lc = fft_lds_len(PC);
     const int nc = w / 2 + 1;
     // Two lines (the pair's two columns) per column of the group.
-    const Shape sh = cc_shape(ctx, "cc_cols", w, h, npair, (long long)npair * nc, 2,
-                              4 * lc * 8, lc * 8, 8, 2 * (int)lc);
+    const Shape sh = cc_shape(ctx, "cc_cols", w, h, npair, 4 * lc * 8, lc * 8,
+                              2 * (int)lc);
     const size_t lds = (4 * sh.nl * lc + lc) * sizeof(float2);
     set_lds((const void*)k_cc_cols<TAB>, lds);
     const int ngroups = cdiv(nc, sh.nl);
@@ -567,8 +565,8 @@ void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
     MF_HIP(hipGetLastError());
   }
   Best* peaks = (Best*)(part + (size_t)2 * nimg * kMomParts);
-  const Shape sh = cc_shape(ctx, "cc_rowsi", w, h, npair, (long long)npair * h, 8,
-                            (2 * lr + 1) * 8, lr * 8, 16, (int)lr);
+  const Shape sh = cc_shape(ctx, "cc_rowsi", w, h, npair, (2 * lr + 1) * 8,
+                            lr * 8, (int)lr);
   const size_t lds = ((2 * lr + 1) * sh.nl + lr) * sizeof(float2);
   set_lds((const void*)k_cc_rows_inv<TAB>, lds);
   const int nparts = cdiv(h, sh.nl);
