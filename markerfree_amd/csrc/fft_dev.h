@@ -147,19 +147,29 @@ __device__ __forceinline__ void fft_butterfly(int R, const float2* src, int ss,
       return;
     }
     default: {
-      // Generic odd-prime radix: O(R^2) from LDS; DFT_R twiddles come from
-      // the length-N table, w_R^{pq} = tw[(pq mod R) * N/R].
+      // Generic odd-prime radix: O(R^2) from LDS.  The stage twiddles are
+      // applied to the inputs once, in place: within a stage every input
+      // belongs to this butterfly alone, and the input buffer is the next
+      // stage's output (with no twiddles in the first stage, Ns = 1, the
+      // caller's data are never modified).  DFT_R twiddles come from the
+      // length-N table, w_R^{pq} = tw[(pq mod R) * N/R], with pq mod R
+      // stepped by pp.
       const int step = N / R;
+      if (tws) {
+        float2* s = const_cast<float2*>(src);
+        for (int q = 1; q < R; q++) {
+          const float2 w = tw[q * tws];
+          s[q * ss] = cmul(s[q * ss], inv ? cconj(w) : w);
+        }
+      }
       for (int pp = 0; pp < R; pp++) {
-        float2 acc = make_float2(0.f, 0.f);
-        for (int q = 0; q < R; q++) {
-          float2 v = src[q * ss];
-          if (q > 0 && tws) {
-            float2 w = tw[q * tws];
-            v = cmul(v, inv ? cconj(w) : w);
-          }
-          float2 wr = tw[((pp * q) % R) * step];
-          acc = cadd(acc, cmul(v, inv ? cconj(wr) : wr));
+        float2 acc = src[0];
+        int k = 0;
+        for (int q = 1; q < R; q++) {
+          k += pp;
+          if (k >= R) k -= R;
+          const float2 wr = tw[k * step];
+          acc = cadd(acc, cmul(src[q * ss], inv ? cconj(wr) : wr));
         }
         dst[pp * ds] = acc;
       }

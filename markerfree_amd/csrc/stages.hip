@@ -140,29 +140,40 @@ __device__ float corr_pixel(const CorrArgs& a, const CorrProj& pr,
   const float hx = (float)a.cx * 0.5f, hy = (float)a.ny * 0.5f;
   const float ox = pr.shift_x + (float)a.nx * 0.5f;
   const float oy = pr.shift_y + (float)a.ny * 0.5f;
+  // The bin's corrected pixels are summed in the reference's order; the
+  // in-image reads of up to four of them are issued before any is used, so
+  // they are in flight together (the reference stops at the first invalid
+  // pixel, after which its sum is discarded: summing on changes nothing).
+  constexpr int kB = 4;
   float sum = 0.f;
   bool bad = false;
-  for (int iy = 0; iy < a.biny && !bad; iy++) {
+  for (int iy = 0; iy < a.biny; iy++) {
     const float py = (float)(y * a.biny + iy) + 0.5f - hy;
-    for (int ix = 0; ix < a.binx; ix++) {
-      const float px = (float)(x * a.binx + ix) + 0.5f - hx;
-      float qx = px * pr.cos_a - py * pr.sin_a;
-      float qy = px * pr.sin_a + py * pr.cos_a;
-      qx += ox;
-      qy += oy;
-      const int rx = (int)(qx + 0.5f), ry = (int)(qy + 0.5f);
-      float v;
-      if (rx >= 0 && rx < a.nx && ry >= 0 && ry < a.ny)
-        v = raw_pix(raw, a.nx, rx, ry);
-      else if (a.fill)
-        v = corr_random(rx, ry, a.nx, a.ny, raw);
-      else
-        v = kInvalid;
-      if (v < -1e10f) {
-        bad = true;
-        break;
+    for (int ix0 = 0; ix0 < a.binx; ix0 += kB) {
+      int rx[kB], ry[kB];
+      bool in[kB];
+      float v[kB];
+#pragma unroll
+      for (int j = 0; j < kB; j++) {
+        const float px = (float)(x * a.binx + ix0 + j) + 0.5f - hx;
+        float qx = px * pr.cos_a - py * pr.sin_a;
+        float qy = px * pr.sin_a + py * pr.cos_a;
+        qx += ox;
+        qy += oy;
+        rx[j] = (int)(qx + 0.5f);
+        ry[j] = (int)(qy + 0.5f);
+        in[j] = ix0 + j < a.binx && rx[j] >= 0 && rx[j] < a.nx && ry[j] >= 0 &&
+                ry[j] < a.ny;
+        v[j] = raw_pix(raw, a.nx, in[j] ? rx[j] : 0, in[j] ? ry[j] : 0);
       }
-      sum += v;
+#pragma unroll
+      for (int j = 0; j < kB; j++) {
+        if (ix0 + j >= a.binx) break;
+        if (!in[j])
+          v[j] = a.fill ? corr_random(rx[j], ry[j], a.nx, a.ny, raw) : kInvalid;
+        bad = bad || v[j] < -1e10f;
+        sum += v[j];
+      }
     }
   }
   return bad ? kInvalid : sum;
