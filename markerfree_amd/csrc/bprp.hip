@@ -656,6 +656,199 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_tab(const BpParams p,
   }
 }
 
+// Two-row table-driven form.  The rays of image rows y0 and y1 = y0 + 1 at
+// one detector position follow the same path through the same voxels, and
+// every voxel looks up the same sinogram positions; only the sinogram values
+// differ.  A work unit therefore covers one ray chunk of a row pair: the
+// sample coordinates, the z table row, each projection's index and range test
+// are evaluated once, and one 8-byte LDS read returns both rows' values from
+// a row-interleaved copy of the sinogram rows.  Each row's voxel values and
+// ray sum still take exactly the single-row operations in the same order, so
+// the output equals k_bprp_tab bit for bit.  Workgroups of up to
+// kFu2Threads rays keep enough waves resident under the larger LDS footprint.
+constexpr int kFu2Threads = 1024;
+
+size_t tab2_lds_bytes(int n, int bx, int tb) {
+  return (size_t)tb * ztab_stride(n) * 4 + ((size_t)8 << n) +
+         (size_t)n * ((size_t)bx + 1) * 8;
+}
+
+// Rows y0 and y1 of the n reference projections as {y0, y1} pairs, encoded
+// like load_sino_rows.  Returns nonzero if either row holds a skipped sample.
+__device__ int load_sino_pair(const BpParams& p, int n, int y0, int y1,
+                              float2* sv) {
+  const int bx = p.bx, rowp = bx + 1;
+  const size_t plane = (size_t)bx * p.by;
+  const float* s0 = p.stack + (size_t)p.i0 * plane + (size_t)y0 * bx;
+  const float* s1 = p.stack + (size_t)p.i0 * plane + (size_t)y1 * bx;
+  int bad = 0;
+  for (int i = 0; i < n; i++) {
+    const float c = p.c[i];
+    for (int x = threadIdx.x; x < rowp; x += blockDim.x) {
+      float2 vc = make_float2(0.f, 0.f);
+      if (x < bx) {
+        const float v0 = s0[(size_t)i * plane + x];
+        const float v1 = s1[(size_t)i * plane + x];
+        const bool b0 = v0 < -1e10f, b1 = v1 < -1e10f;
+        bad |= (b0 || b1) ? 1 : 0;
+        vc.x = b0 ? -INFINITY : v0 * c;
+        vc.y = b1 ? -INFINITY : v1 * c;
+      }
+      sv[i * rowp + x] = vc;
+    }
+  }
+  return __syncthreads_or(bad);
+}
+
+template <int N>
+__global__ __launch_bounds__(kFu2Threads) void k_bprp_tab2(const BpParams p,
+                                                           const RpParams q,
+                                                           int per) {
+  extern __shared__ float4 smem4[];
+  constexpr int ZS = ztab_stride(N);
+  float* ztab = reinterpret_cast<float*>(smem4);
+  float2* wtab = reinterpret_cast<float2*>(ztab + p.tb * ZS);
+  float2* sv = wtab + (1 << N);
+  const int bx = p.bx, rowp = bx + 1;
+  const int vx = 2 * bx;
+  const float cx = (float)bx * 0.5f;
+  const float hvx = (float)vx * 0.5f, htb = (float)p.tb * 0.5f;
+  const float iex = (float)(vx - 1), iez = (float)(q.tb - 1);
+  const unsigned iend_bits = __float_as_uint((float)(bx - 1));
+  const float ox = 0.5f - hvx, oz = 0.5f - htb;
+  const bool fast_ok = p.cnt_all >= 0.001f;
+
+  for (int k = threadIdx.x; k < p.tb * ZS; k += blockDim.x) {
+    const int iz = k / ZS, i = k - iz * ZS;
+    float v = 0.f;
+#pragma unroll
+    for (int j = 0; j < N; j++)
+      if (i == j) v = fmaf((float)iz + oz, p.s[j], cx);
+    ztab[k] = v;
+  }
+  for (int m = threadIdx.x; m < (1 << N); m += blockDim.x) {
+    float w = 0.f;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+      if ((m >> (N - 1 - i)) & 1) w += p.w[i];
+    wtab[m] = make_float2(w, 1.0f / w);
+  }
+
+  const int nt = blockDim.x;
+  const int nch = cdiv(bx, nt);
+  const int units = cdiv(p.by, 2) * nch;
+  const int ua = blockIdx.x * per, ub = min(units, ua + per);
+  int pcur = -1, any_bad = 0, y0 = 0, y1 = 0;
+  for (int un = ua; un < ub; un++) {
+    const int pr = un / nch;
+    if (pr != pcur) {
+      __syncthreads();  // tables built; all waves done with the previous rows
+      y0 = 2 * pr;
+      y1 = min(y0 + 1, p.by - 1);  // an odd last row pairs with itself
+      any_bad = load_sino_pair(p, N, y0, y1, sv);
+      pcur = pr;
+    }
+    const int u = (un - pr * nch) * nt + threadIdx.x;
+    const bool live = u < bx;
+    const int uu = live ? u : bx - 1;  // idle lanes shadow a real ray
+    const float xp = (float)uu + 0.5f - (float)bx * 0.5f;
+    const float tx = fmaf(xp, q.c, (float)vx * 0.5f);
+    const float tz = fmaf(xp, q.s, (float)q.tb * 0.5f);
+    const float z0 = (-xp * q.s) / q.c - 0.5f * (float)q.L;
+    int t0, t1;
+    rp_sample_range(q, tx, tz, z0, &t0, &t1);
+    int fa, fb;
+    fast_interval(p, q, N, tx, tz, z0, &fa, &fb);
+    fa = max(fa, t0);
+    fb = min(fb, t1);
+    const int ta = wave_max(fa), tb = wave_min(fb);
+    const bool has_fast = fast_ok && !any_bad && ta < tb;
+    const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
+    double sum0 = 0.0, sum1 = 0.0;
+    int cnt0 = 0, cnt1 = 0;
+    // Samples that take the reference's tests.  With skipped samples in
+    // either row each row keeps its own validity mask (a row without them
+    // never matches the -inf test, so its mask is the shared one).
+    auto general = [&](int from, int to, auto bad) {
+      for (int t = from; t < to; t++) {
+        const float fz = (float)t + z0;
+        const float X = fmaf(-fz, q.s, tx);
+        const float Z = fmaf(fz, q.c, tz);
+        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const float vfx = (inb ? truncf(X) : 0.f) + ox;
+        float zs[ZS];
+        load_z<N>(ztab, inb ? (int)Z : 0, zs);
+        float acc0 = 0.f, acc1 = 0.f;
+        unsigned m0 = 0, m1 = 0;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          const float r = fmaf(vfx, p.c[i], zs[i]);
+          // The same unsigned range test as k_bprp_tab.
+          const bool ok = __float_as_uint(r) <= iend_bits;
+          float2 val = sv[i * rowp + (ok ? (int)r : bx)];
+          if (decltype(bad)::value) {
+            const bool ok0 = ok && (val.x != -INFINITY);
+            const bool ok1 = ok && (val.y != -INFINITY);
+            val.x = ok0 ? val.x : 0.f;
+            val.y = ok1 ? val.y : 0.f;
+            m0 = 2 * m0 + (ok0 ? 1u : 0u);
+            m1 = 2 * m1 + (ok1 ? 1u : 0u);
+          } else {
+            m0 = 2 * m0 + (ok ? 1u : 0u);
+          }
+          acc0 = fmaf(val.x, p.w[i], acc0);
+          acc1 = fmaf(val.y, p.w[i], acc1);
+        }
+        if (!decltype(bad)::value) m1 = m0;
+        const float2 w0 = wtab[m0], w1 = wtab[m1];
+        const float v0 =
+            (w0.x < 0.001f) ? kInvalid : div_by_rcp(acc0, w0.x, w0.y);
+        const float v1 =
+            (w1.x < 0.001f) ? kInvalid : div_by_rcp(acc1, w1.x, w1.y);
+        const bool ok0 = inb && (v0 >= -1e10f);
+        const bool ok1 = inb && (v1 >= -1e10f);
+        sum0 += ok0 ? (double)v0 : 0.0;
+        sum1 += ok1 ? (double)v1 : 0.0;
+        cnt0 += ok0 ? 1 : 0;
+        cnt1 += ok1 ? 1 : 0;
+      }
+    };
+    if (any_bad) {
+      general(t0, t1, std::true_type{});
+    } else {
+      general(t0, a, std::false_type{});
+      if (has_fast) {
+        for (int t = a; t < b; t++) {
+          const float fz = (float)t + z0;
+          const float vfx = truncf(fmaf(-fz, q.s, tx)) + ox;
+          float zs[ZS];
+          load_z<N>(ztab, (int)fmaf(fz, q.c, tz), zs);
+          float acc0 = 0.f, acc1 = 0.f;
+#pragma unroll
+          for (int i = 0; i < N; i++) {
+            const float2 val = sv[i * rowp + (int)fmaf(vfx, p.c[i], zs[i])];
+            acc0 = fmaf(val.x, p.w[i], acc0);
+            acc1 = fmaf(val.y, p.w[i], acc1);
+          }
+          sum0 += (double)div_by_rcp(acc0, p.cnt_all, p.rcp_all);
+          sum1 += (double)div_by_rcp(acc1, p.cnt_all, p.rcp_all);
+        }
+        cnt0 += b - a;
+        cnt1 += b - a;
+      }
+      general(b, t1, std::false_type{});
+    }
+    if (live) {
+      const float lim = 0.8f * (float)q.L;
+      q.out[(size_t)y0 * bx + u] =
+          ((float)cnt0 < lim) ? kInvalid : (float)(sum0 / (double)cnt0);
+      if (y1 != y0)
+        q.out[(size_t)y1 * bx + u] =
+            ((float)cnt1 < lim) ? kInvalid : (float)(sum1 / (double)cnt1);
+    }
+  }
+}
+
 __global__ void k_fill(float* p, size_t n, float v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -681,22 +874,27 @@ struct KernelTable<0> {
 
 template <int N>
 struct TabKernelTable {
-  static void fill(FusedKernel* tk) {
+  static void fill(FusedKernel* tk, FusedKernel* tk2) {
     tk[N] = k_bprp_tab<N>;
-    TabKernelTable<N - 1>::fill(tk);
+    tk2[N] = k_bprp_tab2<N>;
+    TabKernelTable<N - 1>::fill(tk, tk2);
   }
 };
 template <>
 struct TabKernelTable<0> {
-  static void fill(FusedKernel* tk) { tk[0] = nullptr; }
+  static void fill(FusedKernel* tk, FusedKernel* tk2) {
+    tk[0] = nullptr;
+    tk2[0] = nullptr;
+  }
 };
 
 struct Kernels {
   FusedKernel fu[kMaxFixedN + 1];
   FusedKernel tab[kTabMaxN + 1];
+  FusedKernel tab2[kTabMaxN + 1];
   Kernels() {
     KernelTable<kMaxFixedN>::fill(fu);
-    TabKernelTable<kTabMaxN>::fill(tab);
+    TabKernelTable<kTabMaxN>::fill(tab, tab2);
   }
 };
 
@@ -802,25 +1000,33 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
     }
     FusedKernel k = kernels().fu[kid];
     size_t flds = lds;
-    const size_t tlds = tab_lds_bytes(n, bx, tb);
-    if (mode == 1 && n <= kTabMaxN && tlds <= 160 * 1024) {
+    int threads = kFuThreads;
+    long long units = (long long)by * cdiv(bx, kFuThreads);
+    const bool tab = (mode == 1 || mode == 3) && n <= kTabMaxN;
+    if (tab && mode == 1 && by > 1 && tab2_lds_bytes(n, bx, tb) <= 160 * 1024) {
+      // Row pairs; ray chunks of at most kFu2Threads, evened out.
+      k = kernels().tab2[n];
+      flds = tab2_lds_bytes(n, bx, tb);
+      const int nch = cdiv(bx, kFu2Threads);
+      threads = cdiv(cdiv(bx, nch), 64) * 64;
+      units = (long long)cdiv(by, 2) * nch;
+    } else if (tab && tab_lds_bytes(n, bx, tb) <= 160 * 1024) {
       k = kernels().tab[n];
-      flds = tlds;
+      flds = tab_lds_bytes(n, bx, tb);
     }
     if (flds > 64 * 1024)
       MF_HIP(hipFuncSetAttribute((const void*)k,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)flds));
-    // One resident wave of workgroups over the (row, ray chunk) units.
+    // One resident wave of workgroups over the units.
     int occ = 0;
     MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k,
-                                                        kFuThreads, flds));
-    const long long units = (long long)by * cdiv(bx, kFuThreads);
+                                                        threads, flds));
     const long long slots = (long long)std::max(occ, 1) * std::max(ctx->num_cu, 1);
     const int per = cdiv(units, std::min(units, slots));
     TimedLaunch tl(ctx->timer, "bprp_fused", st);
-    hipLaunchKernelGGL(k, dim3(cdiv(units, per)), dim3(kFuThreads), flds, st,
-                       bp, rp, per);
+    hipLaunchKernelGGL(k, dim3(cdiv(units, per)), dim3(threads), flds, st, bp,
+                       rp, per);
     MF_HIP(hipGetLastError());
     return;
   }
