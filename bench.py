@@ -406,6 +406,11 @@ def main():
             "pa_sweeps_per_series": round(sweeps, 2),
         },
     }
+    if achieved > HBM_PEAK_GBS:
+        out["roofline"]["note"] = (
+            "frac above 1: the fused kernel delivers the backprojection "
+            "faster than writing its slab at HBM peak could; it never writes "
+            "the slab ('traffic' is what it moves) and is bound by VALU issue")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(series,
                                            int(k_n / max(args.steps, 1)),

@@ -656,59 +656,84 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_tab(const BpParams p,
   }
 }
 
-// Two-row table-driven form.  The rays of image rows y0 and y1 = y0 + 1 at
-// one detector position follow the same path through the same voxels, and
-// every voxel looks up the same sinogram positions; only the sinogram values
-// differ.  A work unit therefore covers one ray chunk of a row pair: the
-// sample coordinates, the z table row, each projection's index and range test
-// are evaluated once, and one 8-byte LDS read returns both rows' values from
-// a row-interleaved copy of the sinogram rows.  Each row's voxel values and
-// ray sum still take exactly the single-row operations in the same order, so
-// the output equals k_bprp_tab bit for bit.  Workgroups of up to
-// kFu2Threads rays keep enough waves resident under the larger LDS footprint.
-constexpr int kFu2Threads = 1024;
+// Multi-row table-driven form.  The rays of image rows y0 .. y0+R-1 at one
+// detector position follow the same path through the same voxels, and every
+// voxel looks up the same sinogram positions; only the sinogram values
+// differ.  A work unit therefore covers one ray chunk of a group of R rows:
+// the sample coordinates, the z table row, each projection's index and range
+// test are evaluated once, and the R rows' values come from a row-interleaved
+// LDS copy of the sinogram rows (RowSlots) with one address per lookup.
+// Each row's voxel values and ray sum still take exactly the single-row
+// operations in the same order, so the output equals k_bprp_tab bit for bit.
+// Workgroups of up to kRowsThreads rays keep enough waves resident under the
+// larger LDS footprint.
+constexpr int kRowsThreads = 1024;
 
-size_t tab2_lds_bytes(int n, int bx, int tb) {
+// Sinogram slots of R rows: R = 2 keeps one float2 {row 0, row 1} per
+// (projection, position).
+template <int R>
+struct RowSlots;
+
+template <>
+struct RowSlots<2> {
+  __host__ __device__ static size_t floats(int n, int rowp) {
+    return (size_t)n * rowp * 2;
+  }
+  __device__ static float* at(float* sv, int rowp, int i, int x, int r) {
+    return sv + ((size_t)i * rowp + x) * 2 + r;
+  }
+  __device__ static void read(const float* sv, int rowp, int i, int idx,
+                              float* v) {
+    const float2 a =
+        reinterpret_cast<const float2*>(sv)[(size_t)i * rowp + idx];
+    v[0] = a.x;
+    v[1] = a.y;
+  }
+};
+
+size_t rows_lds_bytes(int n, int bx, int tb) {
   return (size_t)tb * ztab_stride(n) * 4 + ((size_t)8 << n) +
-         (size_t)n * ((size_t)bx + 1) * 8;
+         RowSlots<2>::floats(n, bx + 1) * 4;
 }
 
-// Rows y0 and y1 of the n reference projections as {y0, y1} pairs, encoded
-// like load_sino_rows.  Returns nonzero if either row holds a skipped sample.
-__device__ int load_sino_pair(const BpParams& p, int n, int y0, int y1,
-                              float2* sv) {
+// Rows y[0..R) of the n reference projections into their slots, encoded
+// like load_sino_rows.  Returns nonzero if any of the rows holds a skipped
+// sample.
+template <int R>
+__device__ int load_sino_group(const BpParams& p, int n, const int* y,
+                               float* sv) {
   const int bx = p.bx, rowp = bx + 1;
   const size_t plane = (size_t)bx * p.by;
-  const float* s0 = p.stack + (size_t)p.i0 * plane + (size_t)y0 * bx;
-  const float* s1 = p.stack + (size_t)p.i0 * plane + (size_t)y1 * bx;
   int bad = 0;
   for (int i = 0; i < n; i++) {
     const float c = p.c[i];
+    const float* src = p.stack + (size_t)(p.i0 + i) * plane;
     for (int x = threadIdx.x; x < rowp; x += blockDim.x) {
-      float2 vc = make_float2(0.f, 0.f);
-      if (x < bx) {
-        const float v0 = s0[(size_t)i * plane + x];
-        const float v1 = s1[(size_t)i * plane + x];
-        const bool b0 = v0 < -1e10f, b1 = v1 < -1e10f;
-        bad |= (b0 || b1) ? 1 : 0;
-        vc.x = b0 ? -INFINITY : v0 * c;
-        vc.y = b1 ? -INFINITY : v1 * c;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        float vc = 0.f;
+        if (x < bx) {
+          const float v = src[(size_t)y[r] * bx + x];
+          const bool b = v < -1e10f;
+          bad |= b ? 1 : 0;
+          vc = b ? -INFINITY : v * c;
+        }
+        *RowSlots<R>::at(sv, rowp, i, x, r) = vc;
       }
-      sv[i * rowp + x] = vc;
     }
   }
   return __syncthreads_or(bad);
 }
 
-template <int N>
-__global__ __launch_bounds__(kFu2Threads) void k_bprp_tab2(const BpParams p,
-                                                           const RpParams q,
-                                                           int per) {
+template <int N, int R>
+__global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
+                                                            const RpParams q,
+                                                            int per) {
   extern __shared__ float4 smem4[];
   constexpr int ZS = ztab_stride(N);
   float* ztab = reinterpret_cast<float*>(smem4);
   float2* wtab = reinterpret_cast<float2*>(ztab + p.tb * ZS);
-  float2* sv = wtab + (1 << N);
+  float* sv = reinterpret_cast<float*>(wtab + (1 << N));
   const int bx = p.bx, rowp = bx + 1;
   const int vx = 2 * bx;
   const float cx = (float)bx * 0.5f;
@@ -736,19 +761,21 @@ __global__ __launch_bounds__(kFu2Threads) void k_bprp_tab2(const BpParams p,
 
   const int nt = blockDim.x;
   const int nch = cdiv(bx, nt);
-  const int units = cdiv(p.by, 2) * nch;
+  const int units = cdiv(p.by, R) * nch;
   const int ua = blockIdx.x * per, ub = min(units, ua + per);
-  int pcur = -1, any_bad = 0, y0 = 0, y1 = 0;
+  int gcur = -1, any_bad = 0;
+  int y[R];
   for (int un = ua; un < ub; un++) {
-    const int pr = un / nch;
-    if (pr != pcur) {
+    const int gr = un / nch;
+    if (gr != gcur) {
       __syncthreads();  // tables built; all waves done with the previous rows
-      y0 = 2 * pr;
-      y1 = min(y0 + 1, p.by - 1);  // an odd last row pairs with itself
-      any_bad = load_sino_pair(p, N, y0, y1, sv);
-      pcur = pr;
+      // Rows past the last repeat it (computed, not stored).
+#pragma unroll
+      for (int r = 0; r < R; r++) y[r] = min(R * gr + r, p.by - 1);
+      any_bad = load_sino_group<R>(p, N, y, sv);
+      gcur = gr;
     }
-    const int u = (un - pr * nch) * nt + threadIdx.x;
+    const int u = (un - gr * nch) * nt + threadIdx.x;
     const bool live = u < bx;
     const int uu = live ? u : bx - 1;  // idle lanes shadow a real ray
     const float xp = (float)uu + 0.5f - (float)bx * 0.5f;
@@ -764,11 +791,16 @@ __global__ __launch_bounds__(kFu2Threads) void k_bprp_tab2(const BpParams p,
     const int ta = wave_max(fa), tb = wave_min(fb);
     const bool has_fast = fast_ok && !any_bad && ta < tb;
     const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
-    double sum0 = 0.0, sum1 = 0.0;
-    int cnt0 = 0, cnt1 = 0;
-    // Samples that take the reference's tests.  With skipped samples in
-    // either row each row keeps its own validity mask (a row without them
-    // never matches the -inf test, so its mask is the shared one).
+    double sum[R];
+    int cnt[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      sum[r] = 0.0;
+      cnt[r] = 0;
+    }
+    // Samples that take the reference's tests.  With skipped samples in any
+    // row each row keeps its own validity mask (a row without them never
+    // matches the -inf test, so its mask is the shared one).
     auto general = [&](int from, int to, auto bad) {
       for (int t = from; t < to; t++) {
         const float fz = (float)t + z0;
@@ -778,39 +810,48 @@ __global__ __launch_bounds__(kFu2Threads) void k_bprp_tab2(const BpParams p,
         const float vfx = (inb ? truncf(X) : 0.f) + ox;
         float zs[ZS];
         load_z<N>(ztab, inb ? (int)Z : 0, zs);
-        float acc0 = 0.f, acc1 = 0.f;
-        unsigned m0 = 0, m1 = 0;
+        float vals[N][R];
+        unsigned okm = 0;  // bit i: projection i in range
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          const float r = fmaf(vfx, p.c[i], zs[i]);
+          const float rr = fmaf(vfx, p.c[i], zs[i]);
           // The same unsigned range test as k_bprp_tab.
-          const bool ok = __float_as_uint(r) <= iend_bits;
-          float2 val = sv[i * rowp + (ok ? (int)r : bx)];
-          if (decltype(bad)::value) {
-            const bool ok0 = ok && (val.x != -INFINITY);
-            const bool ok1 = ok && (val.y != -INFINITY);
-            val.x = ok0 ? val.x : 0.f;
-            val.y = ok1 ? val.y : 0.f;
-            m0 = 2 * m0 + (ok0 ? 1u : 0u);
-            m1 = 2 * m1 + (ok1 ? 1u : 0u);
-          } else {
-            m0 = 2 * m0 + (ok ? 1u : 0u);
-          }
-          acc0 = fmaf(val.x, p.w[i], acc0);
-          acc1 = fmaf(val.y, p.w[i], acc1);
+          const bool ok = __float_as_uint(rr) <= iend_bits;
+          okm |= ok ? (1u << i) : 0u;
+          RowSlots<R>::read(sv, rowp, i, ok ? (int)rr : bx, vals[i]);
         }
-        if (!decltype(bad)::value) m1 = m0;
-        const float2 w0 = wtab[m0], w1 = wtab[m1];
-        const float v0 =
-            (w0.x < 0.001f) ? kInvalid : div_by_rcp(acc0, w0.x, w0.y);
-        const float v1 =
-            (w1.x < 0.001f) ? kInvalid : div_by_rcp(acc1, w1.x, w1.y);
-        const bool ok0 = inb && (v0 >= -1e10f);
-        const bool ok1 = inb && (v1 >= -1e10f);
-        sum0 += ok0 ? (double)v0 : 0.0;
-        sum1 += ok1 ? (double)v1 : 0.0;
-        cnt0 += ok0 ? 1 : 0;
-        cnt1 += ok1 ? 1 : 0;
+        float acc[R];
+        unsigned m[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          acc[r] = 0.f;
+          m[r] = 0;
+        }
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          const bool ok = (okm >> i) & 1u;
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            float val = vals[i][r];
+            if (decltype(bad)::value) {
+              const bool okr = ok && (val != -INFINITY);
+              val = okr ? val : 0.f;
+              m[r] = 2 * m[r] + (okr ? 1u : 0u);
+            } else if (r == 0) {
+              m[0] = 2 * m[0] + (ok ? 1u : 0u);
+            }
+            acc[r] = fmaf(val, p.w[i], acc[r]);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const float2 wr = wtab[decltype(bad)::value ? m[r] : m[0]];
+          const float v =
+              (wr.x < 0.001f) ? kInvalid : div_by_rcp(acc[r], wr.x, wr.y);
+          const bool ok = inb && (v >= -1e10f);
+          sum[r] += ok ? (double)v : 0.0;
+          cnt[r] += ok ? 1 : 0;
+        }
       }
     };
     if (any_bad) {
@@ -823,28 +864,236 @@ __global__ __launch_bounds__(kFu2Threads) void k_bprp_tab2(const BpParams p,
           const float vfx = truncf(fmaf(-fz, q.s, tx)) + ox;
           float zs[ZS];
           load_z<N>(ztab, (int)fmaf(fz, q.c, tz), zs);
-          float acc0 = 0.f, acc1 = 0.f;
+          // All lookups of the sample are issued before the first use, so
+          // their LDS latencies overlap.
+          float vals[N][R];
 #pragma unroll
-          for (int i = 0; i < N; i++) {
-            const float2 val = sv[i * rowp + (int)fmaf(vfx, p.c[i], zs[i])];
-            acc0 = fmaf(val.x, p.w[i], acc0);
-            acc1 = fmaf(val.y, p.w[i], acc1);
+          for (int i = 0; i < N; i++)
+            RowSlots<R>::read(sv, rowp, i, (int)fmaf(vfx, p.c[i], zs[i]),
+                              vals[i]);
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            float acc = 0.f;
+#pragma unroll
+            for (int i = 0; i < N; i++) acc = fmaf(vals[i][r], p.w[i], acc);
+            sum[r] += (double)div_by_rcp(acc, p.cnt_all, p.rcp_all);
           }
-          sum0 += (double)div_by_rcp(acc0, p.cnt_all, p.rcp_all);
-          sum1 += (double)div_by_rcp(acc1, p.cnt_all, p.rcp_all);
         }
-        cnt0 += b - a;
-        cnt1 += b - a;
+#pragma unroll
+        for (int r = 0; r < R; r++) cnt[r] += b - a;
       }
       general(b, t1, std::false_type{});
     }
     if (live) {
       const float lim = 0.8f * (float)q.L;
-      q.out[(size_t)y0 * bx + u] =
-          ((float)cnt0 < lim) ? kInvalid : (float)(sum0 / (double)cnt0);
-      if (y1 != y0)
-        q.out[(size_t)y1 * bx + u] =
-            ((float)cnt1 < lim) ? kInvalid : (float)(sum1 / (double)cnt1);
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (R * gr + r < p.by)
+          q.out[(size_t)y[r] * bx + u] =
+              ((float)cnt[r] < lim) ? kInvalid
+                                    : (float)(sum[r] / (double)cnt[r]);
+    }
+  }
+}
+
+// Four-row form.  With four image rows per work unit the sinogram slots
+// alone fill LDS (one float4 {row 0..3} per projection and position, no
+// guard entry), so nothing else is tabulated: the z half of each lookup
+// coordinate is evaluated per sample with the z table's own fma, an
+// out-of-range projection reads position 0 and contributes with weight 0
+// (adding a zero product leaves the sum unchanged), and a tested sample
+// accumulates its weight sum in reference order and divides through the
+// sum's correctly rounded reciprocal.  The result equals k_bprp_tab bit for
+// bit.  At 1024 image rows, four rows per unit is also one unit per CU.
+//
+// The kernel holds no LDS beyond the slots, so there is no workgroup-wide
+// flag for skipped samples (stored as -inf); each ray detects them itself.
+// A lookup of -inf makes its sample's quotient NaN, whatever the weights:
+// in the test-free samples the NaN reaches the ray sum, and a tested sample
+// records it.  A ray that met one is evaluated again with the per-row tests
+// (which are exact in every case), so the output stays bit-identical.
+constexpr int kQuadRows = 4;
+
+size_t quad_lds_bytes(int n, int bx) { return (size_t)n * bx * 16; }
+
+// Rows y[0..4) of the n reference projections as float4 slots, encoded like
+// load_sino_rows.
+__device__ void load_sino_quad(const BpParams& p, int n, const int* y,
+                               float4* sq) {
+  const int bx = p.bx;
+  const size_t plane = (size_t)bx * p.by;
+  for (int i = 0; i < n; i++) {
+    const float c = p.c[i];
+    const float* src = p.stack + (size_t)(p.i0 + i) * plane;
+    for (int x = threadIdx.x; x < bx; x += blockDim.x) {
+      float v[kQuadRows];
+#pragma unroll
+      for (int r = 0; r < kQuadRows; r++) {
+        const float s = src[(size_t)y[r] * bx + x];
+        v[r] = s < -1e10f ? -INFINITY : s * c;
+      }
+      sq[i * bx + x] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+  }
+  __syncthreads();
+}
+
+template <int N>
+__global__ __launch_bounds__(kRowsThreads) void k_bprp_quad(const BpParams p,
+                                                            const RpParams q,
+                                                            int per) {
+  constexpr int R = kQuadRows;
+  extern __shared__ float4 sq[];  // [N][bx]
+  const int bx = p.bx;
+  const int vx = 2 * bx;
+  const float cx = (float)bx * 0.5f;
+  const float hvx = (float)vx * 0.5f, htb = (float)p.tb * 0.5f;
+  const float iex = (float)(vx - 1), iez = (float)(q.tb - 1);
+  const unsigned iend_bits = __float_as_uint((float)(bx - 1));
+  const float ox = 0.5f - hvx, oz = 0.5f - htb;
+  const bool fast_ok = p.cnt_all >= 0.001f;
+
+  const int nt = blockDim.x;
+  const int nch = cdiv(bx, nt);
+  const int units = cdiv(p.by, R) * nch;
+  const int ua = blockIdx.x * per, ub = min(units, ua + per);
+  int gcur = -1;
+  int y[R];
+  for (int un = ua; un < ub; un++) {
+    const int gr = un / nch;
+    if (gr != gcur) {
+      __syncthreads();  // all waves are done with the previous rows
+      // Rows past the last repeat it (computed, not stored).
+#pragma unroll
+      for (int r = 0; r < R; r++) y[r] = min(R * gr + r, p.by - 1);
+      load_sino_quad(p, N, y, sq);
+      gcur = gr;
+    }
+    const int u = (un - gr * nch) * nt + threadIdx.x;
+    const bool live = u < bx;
+    const int uu = live ? u : bx - 1;  // idle lanes shadow a real ray
+    const float xp = (float)uu + 0.5f - (float)bx * 0.5f;
+    const float tx = fmaf(xp, q.c, (float)vx * 0.5f);
+    const float tz = fmaf(xp, q.s, (float)q.tb * 0.5f);
+    const float z0 = (-xp * q.s) / q.c - 0.5f * (float)q.L;
+    int t0, t1;
+    rp_sample_range(q, tx, tz, z0, &t0, &t1);
+    int fa, fb;
+    fast_interval(p, q, N, tx, tz, z0, &fa, &fb);
+    fa = max(fa, t0);
+    fb = min(fb, t1);
+    const int ta = wave_max(fa), tb = wave_min(fb);
+    const bool has_fast = fast_ok && ta < tb;
+    const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
+    double sum[R];
+    int cnt[R];
+    bool seen = false;  // a tested sample's quotient was NaN
+    // Samples that take the reference's tests; with 'bad' each row keeps its
+    // own validity and weight sum, as skipped samples require.
+    auto general = [&](int from, int to, auto bad) {
+      for (int t = from; t < to; t++) {
+        const float fz = (float)t + z0;
+        const float X = fmaf(-fz, q.s, tx);
+        const float Z = fmaf(fz, q.c, tz);
+        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const float vfx = (inb ? truncf(X) : 0.f) + ox;
+        const float vfz = (inb ? truncf(Z) : 0.f) + oz;
+        float4 vals[N];
+        unsigned okm = 0;  // bit i: projection i in range
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          const float rr = fmaf(vfx, p.c[i], fmaf(vfz, p.s[i], cx));
+          // The same unsigned range test as k_bprp_tab.
+          const bool ok = __float_as_uint(rr) <= iend_bits;
+          okm |= ok ? (1u << i) : 0u;
+          vals[i] = sq[i * bx + (ok ? (int)rr : 0)];
+        }
+        float acc[R], ws[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] = ws[r] = 0.f;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          const bool ok = (okm >> i) & 1u;
+          const float v4[R] = {vals[i].x, vals[i].y, vals[i].z, vals[i].w};
+          if (decltype(bad)::value) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+              const bool okr = ok && (v4[r] != -INFINITY);
+              acc[r] = fmaf(okr ? v4[r] : 0.f, p.w[i], acc[r]);
+              ws[r] += okr ? p.w[i] : 0.f;
+            }
+          } else {
+            const float wi = ok ? p.w[i] : 0.f;
+            ws[0] += wi;
+#pragma unroll
+            for (int r = 0; r < R; r++) acc[r] = fmaf(v4[r], wi, acc[r]);
+          }
+        }
+        float rc = 0.f;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const float w = decltype(bad)::value ? ws[r] : ws[0];
+          if (decltype(bad)::value || r == 0) rc = 1.0f / w;
+          const float v = (w < 0.001f) ? kInvalid : div_by_rcp(acc[r], w, rc);
+          if (!decltype(bad)::value) seen |= v != v;
+          const bool ok = inb && (v >= -1e10f);
+          sum[r] += ok ? (double)v : 0.0;
+          cnt[r] += ok ? 1 : 0;
+        }
+      }
+    };
+    auto reset = [&]() {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        sum[r] = 0.0;
+        cnt[r] = 0;
+      }
+    };
+    reset();
+    general(t0, a, std::false_type{});
+    if (has_fast) {
+      for (int t = a; t < b; t++) {
+        const float fz = (float)t + z0;
+        const float vfx = truncf(fmaf(-fz, q.s, tx)) + ox;
+        const float vfz = truncf(fmaf(fz, q.c, tz)) + oz;
+        // All lookups of the sample are issued before the first use, so
+        // their LDS latencies overlap.
+        float4 vals[N];
+#pragma unroll
+        for (int i = 0; i < N; i++)
+          vals[i] = sq[i * bx + (int)fmaf(vfx, p.c[i], fmaf(vfz, p.s[i], cx))];
+        float acc[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] = 0.f;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          acc[0] = fmaf(vals[i].x, p.w[i], acc[0]);
+          acc[1] = fmaf(vals[i].y, p.w[i], acc[1]);
+          acc[2] = fmaf(vals[i].z, p.w[i], acc[2]);
+          acc[3] = fmaf(vals[i].w, p.w[i], acc[3]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          sum[r] += (double)div_by_rcp(acc[r], p.cnt_all, p.rcp_all);
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) cnt[r] += b - a;
+    }
+    general(b, t1, std::false_type{});
+#pragma unroll
+    for (int r = 0; r < R; r++) seen |= sum[r] != sum[r];
+    if (seen) {
+      reset();
+      general(t0, t1, std::true_type{});
+    }
+    if (live) {
+      const float lim = 0.8f * (float)q.L;
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (R * gr + r < p.by)
+          q.out[(size_t)y[r] * bx + u] =
+              ((float)cnt[r] < lim) ? kInvalid
+                                    : (float)(sum[r] / (double)cnt[r]);
     }
   }
 }
@@ -874,27 +1123,28 @@ struct KernelTable<0> {
 
 template <int N>
 struct TabKernelTable {
-  static void fill(FusedKernel* tk, FusedKernel* tk2) {
+  static void fill(FusedKernel* tk, FusedKernel* tk2, FusedKernel* tk4) {
     tk[N] = k_bprp_tab<N>;
-    tk2[N] = k_bprp_tab2<N>;
-    TabKernelTable<N - 1>::fill(tk, tk2);
+    tk2[N] = k_bprp_rows<N, 2>;
+    tk4[N] = k_bprp_quad<N>;
+    TabKernelTable<N - 1>::fill(tk, tk2, tk4);
   }
 };
 template <>
 struct TabKernelTable<0> {
-  static void fill(FusedKernel* tk, FusedKernel* tk2) {
-    tk[0] = nullptr;
-    tk2[0] = nullptr;
+  static void fill(FusedKernel* tk, FusedKernel* tk2, FusedKernel* tk4) {
+    tk[0] = tk2[0] = tk4[0] = nullptr;
   }
 };
 
 struct Kernels {
   FusedKernel fu[kMaxFixedN + 1];
   FusedKernel tab[kTabMaxN + 1];
-  FusedKernel tab2[kTabMaxN + 1];
+  FusedKernel rows2[kTabMaxN + 1];
+  FusedKernel quad[kTabMaxN + 1];
   Kernels() {
     KernelTable<kMaxFixedN>::fill(fu);
-    TabKernelTable<kTabMaxN>::fill(tab, tab2);
+    TabKernelTable<kTabMaxN>::fill(tab, rows2, quad);
   }
 };
 
@@ -1002,17 +1252,30 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
     size_t flds = lds;
     int threads = kFuThreads;
     long long units = (long long)by * cdiv(bx, kFuThreads);
-    const bool tab = (mode == 1 || mode == 3) && n <= kTabMaxN;
-    if (tab && mode == 1 && by > 1 && tab2_lds_bytes(n, bx, tb) <= 160 * 1024) {
-      // Row pairs; ray chunks of at most kFu2Threads, evened out.
-      k = kernels().tab2[n];
-      flds = tab2_lds_bytes(n, bx, tb);
-      const int nch = cdiv(bx, kFu2Threads);
-      threads = cdiv(cdiv(bx, nch), 64) * 64;
-      units = (long long)cdiv(by, 2) * nch;
-    } else if (tab && tab_lds_bytes(n, bx, tb) <= 160 * 1024) {
+    // Table-driven forms: mode 1 takes the most image rows per work unit
+    // whose sinogram slots fit in LDS (4, then 2, then 1); modes 3 and 4
+    // pin one and two rows.
+    const bool tab = (mode == 1 || mode >= 3) && n <= kTabMaxN;
+    const size_t cap = 160 * 1024;
+    int rows = 0;
+    if (tab && mode == 1 && by >= kQuadRows && quad_lds_bytes(n, bx) <= cap) {
+      rows = kQuadRows;
+      k = kernels().quad[n];
+      flds = quad_lds_bytes(n, bx);
+    } else if (tab && (mode == 1 || mode == 4) && by > 1 &&
+               rows_lds_bytes(n, bx, tb) <= cap) {
+      rows = 2;
+      k = kernels().rows2[n];
+      flds = rows_lds_bytes(n, bx, tb);
+    } else if (tab && tab_lds_bytes(n, bx, tb) <= cap) {
       k = kernels().tab[n];
       flds = tab_lds_bytes(n, bx, tb);
+    }
+    if (rows > 0) {
+      // Groups of image rows; ray chunks of at most kRowsThreads, evened out.
+      const int nch = cdiv(bx, kRowsThreads);
+      threads = cdiv(cdiv(bx, nch), 64) * 64;
+      units = (long long)cdiv(by, rows) * nch;
     }
     if (flds > 64 * 1024)
       MF_HIP(hipFuncSetAttribute((const void*)k,

@@ -193,7 +193,7 @@ def _check_target(ctx, orc, stack, tilts, skip, target, nproj, tb, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("bx,by,nz,step,tb", [
     (128, 16, 21, 3.0, 40),
     (256, 8, 41, 3.0, 100),   # small-series binned geometry, reduced rows
@@ -213,7 +213,7 @@ def test_gpu_reproject_parity(ctx, orc, mode, bx, by, nz, step, tb):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_gpu_reproject_bitexact_fp32(ctx, orc, mode):
     """Against the oracle's FP32 mode (backprojection accumulated as
     fma(v*cos, w, acc) in reference order, ray sums in FP64) the HIP
@@ -257,7 +257,7 @@ def test_gpu_fast_paths_bitexact_wide(ctx, orc, bx, by, tb):
                           d_out, mode=0, d_slab=d_slab)
             np.testing.assert_array_equal(d_slab.download(), slab)
             np.testing.assert_array_equal(d_out.download(), ref)
-            for mode in (1, 2, 3):
+            for mode in (1, 2, 3, 4):
                 ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
                               d_out, mode=mode)
                 np.testing.assert_array_equal(d_out.download(), ref)
@@ -266,16 +266,20 @@ def test_gpu_fast_paths_bitexact_wide(ctx, orc, bx, by, tb):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bx,by,tb", [(1100, 5, 90), (300, 7, 60), (96, 1, 30)])
-def test_gpu_row_pairs_bitexact(ctx, orc, bx, by, tb):
-    """The two-row form (mode 1) against the oracle's FP32 mode and the
-    one-row table form (mode 3): odd row counts (the last row pairs with
-    itself), a single row, ray chunks that split an image row, and skipped
-    samples in one row of a pair but not the other."""
+@pytest.mark.parametrize("bx,by,tb", [(1100, 5, 90), (300, 7, 60), (96, 1, 30),
+                                      (160, 2, 40), (250, 9, 50)])
+def test_gpu_row_groups_bitexact(ctx, orc, bx, by, tb):
+    """The multi-row forms (mode 1: four rows per unit where they fit in
+    LDS, else two; mode 4: two rows) against the oracle's FP32 mode and the
+    one-row table form (mode 3): row counts that leave a partial last group,
+    a single row, ray chunks that split an image row, and skipped samples in
+    one row of a group but not the others."""
     stack, tilts = _series(bx, by, 41, 3.0, seed=bx + by)
     if by > 2:
         stack[20, 2, bx // 3: bx // 3 + 5] = INVALID
     stack[22, by - 1, 7] = INVALID
+    # Position 0 is where the four-row form's out-of-range lookups read.
+    stack[24, 0, 0] = INVALID
     skip = np.zeros(41, np.uint8)
     d_stack = ctx.array(stack)
     d_out = ctx.empty((by, bx))
@@ -283,7 +287,7 @@ def test_gpu_row_pairs_bitexact(ctx, orc, bx, by, tb):
         skip[:] = 0
         skip[target] = 1
         ref = orc.reproject(stack, tb, tilts, skip, target, 10, acc64=False)
-        for mode in (1, 3):
+        for mode in (1, 3, 4):
             ctx.reproject(d_stack, bx, by, 41, tb, tilts, skip, target, 10,
                           d_out, mode=mode)
             np.testing.assert_array_equal(d_out.download(), ref)
@@ -300,7 +304,7 @@ def test_gpu_reproject_invalid_samples(ctx, orc):
     stack[4, 1, :] = INVALID
     skip = np.zeros(21, np.uint8)
     skip[15] = 1
-    for mode in (0, 1, 2, 3):
+    for mode in (0, 1, 2, 3, 4):
         _check_target(ctx, orc, stack, tilts, skip, 15, 10, 30, mode)
 
 
@@ -310,7 +314,7 @@ def test_gpu_reproject_empty_range(ctx, orc):
     skip = np.ones(11, np.uint8)
     d_stack = ctx.array(stack)
     d_out = ctx.empty((3, 64))
-    for mode in (0, 1, 2, 3):
+    for mode in (0, 1, 2, 3, 4):
         ctx.reproject(d_stack, 64, 3, 11, 20, tilts, skip, 5, 10, d_out,
                       mode=mode)
         assert np.all(d_out.download() == np.float32(INVALID))
