@@ -107,11 +107,12 @@ mf_status mf_find_proj_range(const float* tilts, int nz,
  * them at tilts[target] (mGForProj, :68-124) into d_reproj [by][bx].
  * mode 0: materialised slab (d_slab [by][tb][2bx] if non-NULL, else a
  * library workspace); mode 1: fused backprojection+reprojection that samples
- * the slab on the fly (identical output, no slab in HBM), table-driven when
- * the target's tables fit in LDS, with three (else two) image rows per work
- * unit when their sinogram rows fit as well; mode 2: the fused form without
- * per-target tables (identical output; the fallback of mode 1); modes 3 and
- * 4: the table-driven form one and two rows at a time (identical output). */
+ * the slab on the fly (identical output, no slab in HBM), with as many image
+ * rows per work unit (4, 2 or 1) as the sinogram rows allow in LDS; mode 2:
+ * the one-row fused form without per-target tables (identical output; the
+ * last fallback of mode 1); modes 3, 4 and 5 pin forms of mode 1 (one
+ * row with tables, two rows with tables, two rows without; identical
+ * output). */
 mf_status mf_reproject(mf_ctx* ctx, const float* d_stack, int bx, int by,
                        int nz, int tb, const float* tilts,
                        const unsigned char* skip, int target, int nproj,
@@ -180,7 +181,7 @@ typedef struct {
   int out_bin;       /* -g field 6 */
   int nproj;         /* -p: neighbouring projections per target */
   int verbose;       /* print the reference's stage trace to stdout */
-  int reproj_mode;   /* mf_reproject mode 0..4: 0 slab, 1 fused (default) */
+  int reproj_mode;   /* mf_reproject mode 0..5: 0 slab, 1 fused (default) */
 } mf_options;
 
 /* Defaults of InitOpts (main/opts.h:110-124): AlignZ 600, OutBin 1, NPROJ 10;

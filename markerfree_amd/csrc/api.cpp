@@ -290,7 +290,7 @@ mf_status mf_reproject(mf_ctx* ctx, const float* d_stack, int bx, int by,
   if (!ctx) return MF_EINVAL;
   return guard(ctx, [&] {
     MF_REQUIRE(d_stack && tilts && skip && d_reproj, "null argument");
-    MF_REQUIRE(mode >= 0 && mode <= 4, "mode must be 0..4");
+    MF_REQUIRE(mode >= 0 && mode <= 5, "mode must be 0..5");
     DeviceScope d(ctx);
     reproject_target(ctx, ctx->stream, d_stack, bx, by, nz, tb, tilts, skip,
                      target, nproj, mode, d_reproj, d_slab);
@@ -608,7 +608,7 @@ mf_status mf_apply_offset(mf_series* s) {
 mf_status mf_projalign_setup(mf_series* s, float zshift, int align_z,
                              int nproj, int reproj_mode) {
   return sguard(s, [&] {
-    MF_REQUIRE(reproj_mode >= 0 && reproj_mode <= 4, "reproj_mode is 0..4");
+    MF_REQUIRE(reproj_mode >= 0 && reproj_mode <= 5, "reproj_mode is 0..5");
     s->reproj_mode = reproj_mode;
     series_projalign_setup(s, zshift, align_z, nproj);
   });
@@ -633,8 +633,8 @@ mf_status mf_run(mf_series* s, const mf_options* o) {
   return sguard(s, [&] {
     MF_REQUIRE(o, "null options");
     MF_REQUIRE(o->nproj > 0, "nProj must be > 0");
-    MF_REQUIRE(o->reproj_mode >= 0 && o->reproj_mode <= 4,
-               "reproj_mode is 0..4");
+    MF_REQUIRE(o->reproj_mode >= 0 && o->reproj_mode <= 5,
+               "reproj_mode is 0..5");
     series_run(s, o);
   });
 }

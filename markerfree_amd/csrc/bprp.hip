@@ -896,9 +896,11 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
   }
 }
 
-// Four-row form.  With four image rows per work unit the sinogram slots
-// alone fill LDS (one float4 {row 0..3} per projection and position, no
-// guard entry), so nothing else is tabulated: the z half of each lookup
+// Table-free multi-row form (R = 4 or 2 image rows per work unit).  With four
+// rows per unit at 1024 columns, or two rows at 1920-2048 columns, the
+// sinogram slots alone fill LDS (one float4 or float2 {row 0..R-1} per
+// projection and position, no guard entry), so nothing else is tabulated:
+// the z half of each lookup
 // coordinate is evaluated per sample with the z table's own fma, an
 // out-of-range projection reads position 0 and contributes with weight 0
 // (adding a zero product leaves the sum unchanged), and a tested sample
@@ -912,38 +914,63 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
 // in the test-free samples the NaN reaches the ray sum, and a tested sample
 // records it.  A ray that met one is evaluated again with the per-row tests
 // (which are exact in every case), so the output stays bit-identical.
-constexpr int kQuadRows = 4;
+size_t free_lds_bytes(int r, int n, int bx) { return (size_t)n * bx * 4 * r; }
 
-size_t quad_lds_bytes(int n, int bx) { return (size_t)n * bx * 16; }
+template <int R>
+struct FreeSlot;
+template <>
+struct FreeSlot<4> {
+  typedef float4 T;
+  __device__ static T make(const float* v) {
+    return make_float4(v[0], v[1], v[2], v[3]);
+  }
+  __device__ static void split(const T& s, float* v) {
+    v[0] = s.x;
+    v[1] = s.y;
+    v[2] = s.z;
+    v[3] = s.w;
+  }
+};
+template <>
+struct FreeSlot<2> {
+  typedef float2 T;
+  __device__ static T make(const float* v) { return make_float2(v[0], v[1]); }
+  __device__ static void split(const T& s, float* v) {
+    v[0] = s.x;
+    v[1] = s.y;
+  }
+};
 
-// Rows y[0..4) of the n reference projections as float4 slots, encoded like
+// Rows y[0..R) of the n reference projections as slots, encoded like
 // load_sino_rows.
-__device__ void load_sino_quad(const BpParams& p, int n, const int* y,
-                               float4* sq) {
+template <int R>
+__device__ void load_sino_slots(const BpParams& p, int n, const int* y,
+                                typename FreeSlot<R>::T* sq) {
   const int bx = p.bx;
   const size_t plane = (size_t)bx * p.by;
   for (int i = 0; i < n; i++) {
     const float c = p.c[i];
     const float* src = p.stack + (size_t)(p.i0 + i) * plane;
     for (int x = threadIdx.x; x < bx; x += blockDim.x) {
-      float v[kQuadRows];
+      float v[R];
 #pragma unroll
-      for (int r = 0; r < kQuadRows; r++) {
+      for (int r = 0; r < R; r++) {
         const float s = src[(size_t)y[r] * bx + x];
         v[r] = s < -1e10f ? -INFINITY : s * c;
       }
-      sq[i * bx + x] = make_float4(v[0], v[1], v[2], v[3]);
+      sq[i * bx + x] = FreeSlot<R>::make(v);
     }
   }
   __syncthreads();
 }
 
-template <int N>
-__global__ __launch_bounds__(kRowsThreads) void k_bprp_quad(const BpParams p,
+template <int N, int R>
+__global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
                                                             const RpParams q,
                                                             int per) {
-  constexpr int R = kQuadRows;
-  extern __shared__ float4 sq[];  // [N][bx]
+  typedef typename FreeSlot<R>::T Slot;
+  extern __shared__ float4 smem_free[];
+  Slot* sq = reinterpret_cast<Slot*>(smem_free);  // [N][bx]
   const int bx = p.bx;
   const int vx = 2 * bx;
   const float cx = (float)bx * 0.5f;
@@ -966,7 +993,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_quad(const BpParams p,
       // Rows past the last repeat it (computed, not stored).
 #pragma unroll
       for (int r = 0; r < R; r++) y[r] = min(R * gr + r, p.by - 1);
-      load_sino_quad(p, N, y, sq);
+      load_sino_slots<R>(p, N, y, sq);
       gcur = gr;
     }
     const int u = (un - gr * nch) * nt + threadIdx.x;
@@ -998,7 +1025,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_quad(const BpParams p,
         const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
         const float vfx = (inb ? truncf(X) : 0.f) + ox;
         const float vfz = (inb ? truncf(Z) : 0.f) + oz;
-        float4 vals[N];
+        Slot vals[N];
         unsigned okm = 0;  // bit i: projection i in range
 #pragma unroll
         for (int i = 0; i < N; i++) {
@@ -1014,7 +1041,8 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_quad(const BpParams p,
 #pragma unroll
         for (int i = 0; i < N; i++) {
           const bool ok = (okm >> i) & 1u;
-          const float v4[R] = {vals[i].x, vals[i].y, vals[i].z, vals[i].w};
+          float v4[R];
+          FreeSlot<R>::split(vals[i], v4);
           if (decltype(bad)::value) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -1058,7 +1086,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_quad(const BpParams p,
         const float vfz = truncf(fmaf(fz, q.c, tz)) + oz;
         // All lookups of the sample are issued before the first use, so
         // their LDS latencies overlap.
-        float4 vals[N];
+        Slot vals[N];
 #pragma unroll
         for (int i = 0; i < N; i++)
           vals[i] = sq[i * bx + (int)fmaf(vfx, p.c[i], fmaf(vfz, p.s[i], cx))];
@@ -1067,10 +1095,10 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_quad(const BpParams p,
         for (int r = 0; r < R; r++) acc[r] = 0.f;
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          acc[0] = fmaf(vals[i].x, p.w[i], acc[0]);
-          acc[1] = fmaf(vals[i].y, p.w[i], acc[1]);
-          acc[2] = fmaf(vals[i].z, p.w[i], acc[2]);
-          acc[3] = fmaf(vals[i].w, p.w[i], acc[3]);
+          float v4[R];
+          FreeSlot<R>::split(vals[i], v4);
+#pragma unroll
+          for (int r = 0; r < R; r++) acc[r] = fmaf(v4[r], p.w[i], acc[r]);
         }
 #pragma unroll
         for (int r = 0; r < R; r++)
@@ -1123,17 +1151,20 @@ struct KernelTable<0> {
 
 template <int N>
 struct TabKernelTable {
-  static void fill(FusedKernel* tk, FusedKernel* tk2, FusedKernel* tk4) {
+  static void fill(FusedKernel* tk, FusedKernel* tk2, FusedKernel* f2,
+                   FusedKernel* f4) {
     tk[N] = k_bprp_tab<N>;
     tk2[N] = k_bprp_rows<N, 2>;
-    tk4[N] = k_bprp_quad<N>;
-    TabKernelTable<N - 1>::fill(tk, tk2, tk4);
+    f2[N] = k_bprp_free<N, 2>;
+    f4[N] = k_bprp_free<N, 4>;
+    TabKernelTable<N - 1>::fill(tk, tk2, f2, f4);
   }
 };
 template <>
 struct TabKernelTable<0> {
-  static void fill(FusedKernel* tk, FusedKernel* tk2, FusedKernel* tk4) {
-    tk[0] = tk2[0] = tk4[0] = nullptr;
+  static void fill(FusedKernel* tk, FusedKernel* tk2, FusedKernel* f2,
+                   FusedKernel* f4) {
+    tk[0] = tk2[0] = f2[0] = f4[0] = nullptr;
   }
 };
 
@@ -1141,12 +1172,22 @@ struct Kernels {
   FusedKernel fu[kMaxFixedN + 1];
   FusedKernel tab[kTabMaxN + 1];
   FusedKernel rows2[kTabMaxN + 1];
-  FusedKernel quad[kTabMaxN + 1];
+  FusedKernel free2[kTabMaxN + 1];
+  FusedKernel free4[kTabMaxN + 1];
   Kernels() {
     KernelTable<kMaxFixedN>::fill(fu);
-    TabKernelTable<kTabMaxN>::fill(tab, rows2, quad);
+    TabKernelTable<kTabMaxN>::fill(tab, rows2, free2, free4);
   }
 };
+
+// Whether 'dyn' bytes of dynamic LDS fit in a workgroup's 160 KB next to
+// kernel k's static LDS (block reductions such as __syncthreads_or use some).
+bool lds_fits(const void* k, size_t dyn) {
+  if (!k || dyn > 160 * 1024) return false;
+  hipFuncAttributes a;
+  MF_HIP(hipFuncGetAttributes(&a, k));
+  return dyn + a.sharedSizeBytes <= 160 * 1024;
+}
 
 const Kernels& kernels() {
   static Kernels k;
@@ -1252,23 +1293,31 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
     size_t flds = lds;
     int threads = kFuThreads;
     long long units = (long long)by * cdiv(bx, kFuThreads);
-    // Table-driven forms: mode 1 takes the most image rows per work unit
-    // whose sinogram slots fit in LDS (4, then 2, then 1); modes 3 and 4
-    // pin one and two rows.
+    // Multi-row forms.  Mode 1 takes the table-free form with the most image
+    // rows per unit whose slots fit in LDS (4, then 2; measured faster than
+    // the two-row form with tables wherever both fit), then the two-row form
+    // with tables, then one row; modes 3, 4 and 5 pin one row, two rows with
+    // tables and two rows without.
     const bool tab = (mode == 1 || mode >= 3) && n <= kTabMaxN;
-    const size_t cap = 160 * 1024;
+    const Kernels& ks = kernels();
     int rows = 0;
-    if (tab && mode == 1 && by >= kQuadRows && quad_lds_bytes(n, bx) <= cap) {
-      rows = kQuadRows;
-      k = kernels().quad[n];
-      flds = quad_lds_bytes(n, bx);
-    } else if (tab && (mode == 1 || mode == 4) && by > 1 &&
-               rows_lds_bytes(n, bx, tb) <= cap) {
+    if (tab && mode == 1 && by >= 4 &&
+        lds_fits((const void*)ks.free4[n], free_lds_bytes(4, n, bx))) {
+      rows = 4;
+      k = ks.free4[n];
+      flds = free_lds_bytes(4, n, bx);
+    } else if (tab && (mode == 1 || mode == 5) && by > 1 &&
+               lds_fits((const void*)ks.free2[n], free_lds_bytes(2, n, bx))) {
       rows = 2;
-      k = kernels().rows2[n];
+      k = ks.free2[n];
+      flds = free_lds_bytes(2, n, bx);
+    } else if (tab && (mode == 1 || mode == 4) && by > 1 &&
+               lds_fits((const void*)ks.rows2[n], rows_lds_bytes(n, bx, tb))) {
+      rows = 2;
+      k = ks.rows2[n];
       flds = rows_lds_bytes(n, bx, tb);
-    } else if (tab && tab_lds_bytes(n, bx, tb) <= cap) {
-      k = kernels().tab[n];
+    } else if (tab && lds_fits((const void*)ks.tab[n], tab_lds_bytes(n, bx, tb))) {
+      k = ks.tab[n];
       flds = tab_lds_bytes(n, bx, tb);
     }
     if (rows > 0) {
@@ -1277,6 +1326,7 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
       threads = cdiv(cdiv(bx, nch), 64) * 64;
       units = (long long)cdiv(by, rows) * nch;
     }
+    MF_REQUIRE(lds_fits((const void*)k, flds), "sinogram rows exceed LDS");
     if (flds > 64 * 1024)
       MF_HIP(hipFuncSetAttribute((const void*)k,
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1303,6 +1353,7 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
                        kInvalid);
   } else {
     const void* k = (const void*)k_bp_slab;
+    MF_REQUIRE(lds_fits(k, lds), "sinogram rows exceed LDS");
     if (lds > 64 * 1024)
       MF_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)lds));

@@ -193,7 +193,7 @@ def _check_target(ctx, orc, stack, tilts, skip, target, nproj, tb, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("bx,by,nz,step,tb", [
     (128, 16, 21, 3.0, 40),
     (256, 8, 41, 3.0, 100),   # small-series binned geometry, reduced rows
@@ -213,7 +213,7 @@ def test_gpu_reproject_parity(ctx, orc, mode, bx, by, nz, step, tb):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
 def test_gpu_reproject_bitexact_fp32(ctx, orc, mode):
     """Against the oracle's FP32 mode (backprojection accumulated as
     fma(v*cos, w, acc) in reference order, ray sums in FP64) the HIP
@@ -257,7 +257,7 @@ def test_gpu_fast_paths_bitexact_wide(ctx, orc, bx, by, tb):
                           d_out, mode=0, d_slab=d_slab)
             np.testing.assert_array_equal(d_slab.download(), slab)
             np.testing.assert_array_equal(d_out.download(), ref)
-            for mode in (1, 2, 3, 4):
+            for mode in (1, 2, 3, 4, 5):
                 ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
                               d_out, mode=mode)
                 np.testing.assert_array_equal(d_out.download(), ref)
@@ -270,10 +270,11 @@ def test_gpu_fast_paths_bitexact_wide(ctx, orc, bx, by, tb):
                                       (160, 2, 40), (250, 9, 50)])
 def test_gpu_row_groups_bitexact(ctx, orc, bx, by, tb):
     """The multi-row forms (mode 1: four rows per unit where they fit in
-    LDS, else two; mode 4: two rows) against the oracle's FP32 mode and the
-    one-row table form (mode 3): row counts that leave a partial last group,
-    a single row, ray chunks that split an image row, and skipped samples in
-    one row of a group but not the others."""
+    LDS, else two; modes 4 and 5: two rows with and without tables) against
+    the oracle's FP32 mode and the one-row table form (mode 3): row counts
+    that leave a partial last group, a single row, ray chunks that split an
+    image row, and skipped samples in one row of a group but not the
+    others."""
     stack, tilts = _series(bx, by, 41, 3.0, seed=bx + by)
     if by > 2:
         stack[20, 2, bx // 3: bx // 3 + 5] = INVALID
@@ -287,8 +288,33 @@ def test_gpu_row_groups_bitexact(ctx, orc, bx, by, tb):
         skip[:] = 0
         skip[target] = 1
         ref = orc.reproject(stack, tb, tilts, skip, target, 10, acc64=False)
-        for mode in (1, 3, 4):
+        for mode in (1, 3, 4, 5):
             ctx.reproject(d_stack, bx, by, 41, tb, tilts, skip, target, 10,
+                          d_out, mode=mode)
+            np.testing.assert_array_equal(d_out.download(), ref)
+    d_stack.free()
+    d_out.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bx", [1920, 2048])
+def test_gpu_wide_rows_bitexact(ctx, orc, bx):
+    """Widths where only the table-free two-row form fits (at 2048 columns
+    and ten neighbours its slots take exactly the 160 KB of LDS): mode 1
+    (which picks it), mode 5 and the one-row form against the oracle's FP32
+    mode, with a skipped sample at position 0 of one row."""
+    by, nz, tb = 3, 41, 40
+    stack, tilts = _series(bx, by, nz, 3.0, seed=bx)
+    stack[22, 1, 0] = INVALID
+    d_stack = ctx.array(stack)
+    d_out = ctx.empty((by, bx))
+    skip = np.zeros(nz, np.uint8)
+    for target in (20, 31):
+        skip[:] = 0
+        skip[target] = 1
+        ref = orc.reproject(stack, tb, tilts, skip, target, 10, acc64=False)
+        for mode in (1, 5, 3):
+            ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
                           d_out, mode=mode)
             np.testing.assert_array_equal(d_out.download(), ref)
     d_stack.free()
@@ -304,7 +330,7 @@ def test_gpu_reproject_invalid_samples(ctx, orc):
     stack[4, 1, :] = INVALID
     skip = np.zeros(21, np.uint8)
     skip[15] = 1
-    for mode in (0, 1, 2, 3, 4):
+    for mode in (0, 1, 2, 3, 4, 5):
         _check_target(ctx, orc, stack, tilts, skip, 15, 10, 30, mode)
 
 
@@ -314,7 +340,7 @@ def test_gpu_reproject_empty_range(ctx, orc):
     skip = np.ones(11, np.uint8)
     d_stack = ctx.array(stack)
     d_out = ctx.empty((3, 64))
-    for mode in (0, 1, 2, 3, 4):
+    for mode in (0, 1, 2, 3, 4, 5):
         ctx.reproject(d_stack, 64, 3, 11, 20, tilts, skip, 5, 10, d_out,
                       mode=mode)
         assert np.all(d_out.download() == np.float32(INVALID))
