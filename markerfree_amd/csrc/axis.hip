@@ -13,7 +13,6 @@
 #include "axis.h"
 #include "cc.h"
 #include "context.h"
-#include "devutil.h"
 #include "fft_dev.h"
 #include "stages.h"
 
@@ -79,6 +78,8 @@ __global__ __launch_bounds__(256) void k_common_region(int num, int L, int nx,
   reg[2 * ((size_t)k * L + y) + 1] = right;
 }
 
+constexpr int kRadonLines = 64;  // lines (lanes) per workgroup
+
 struct RadonImg {
   float ct;      // (float)cos(tilt * D2R)
   float ox, oy;  // nx/2 + shift_x, ny/2 + shift_y
@@ -87,9 +88,12 @@ struct RadonImg {
 // GRadon (Rotate.cu:83-136): mean of the nearest non-negative pixels along
 // the candidate direction over the cos(tilt)-cropped common region; the sum
 // is sequential in x exactly as the reference thread sums it.  One-dimensional
-// grid over (line block, candidate, image), image slowest, remapped so that
-// each XCD works through its own range of images and their pixels stay in
-// its L2.  The lanes of a wave trace parallel lines a pixel apart, stepping
+// grid over (line block, candidate, image), image slowest, in the hardware's
+// own round-robin over the XCDs: every XCD works on the same image at once.
+// One wave per workgroup and eight samples in flight per lane; measured 1.4x
+// faster at 1024^2 and 1.8x at 5760x4092 than 256-lane workgroups with four
+// samples in flight and a per-XCD image range.  The lanes of a wave trace
+// parallel lines a pixel apart, stepping
 // by (-sin, cos); for candidates nearer the image's x axis than its y axis
 // that step crosses rows, so those read the transposed stack (tstack, [nz]
 // [nx][ny]), where it runs along them.  Samples inside the analytic
@@ -115,14 +119,14 @@ __device__ __forceinline__ void radon_line(const float* img, int nx, int ny,
                                            int right, float* sum_out,
                                            int* cnt_out);
 
-__global__ __launch_bounds__(256) void k_radon(const float* stack,
+__global__ __launch_bounds__(kRadonLines) void k_radon(const float* stack,
                                                const float* tstack, int nx,
                                                int ny, int num, int L,
                                                const float* cs, const int* reg,
                                                const RadonImg* imgs,
                                                float* lines, int pitch) {
-  const int yblocks = cdiv(L, 256);
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int yblocks = cdiv(L, kRadonLines);
+  const int blk = blockIdx.x;
   const int z = blk / (yblocks * num);
   const int rem = blk - z * yblocks * num;
   const int k = rem / yblocks;
@@ -197,7 +201,7 @@ __device__ __forceinline__ void radon_line(const float* img, int nx, int ny,
     }
   };
   tested(left, xa);
-#pragma unroll 4
+#pragma unroll 8
   for (int x = xa; x < xb; x++) {
     const float X = (float)x * c - fys + im.ox;
     const float Y = (float)x * s + fyc + im.oy;
@@ -469,7 +473,8 @@ float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
   }
   {
     TimedLaunch tl(ctx->timer, "radon", st);
-    hipLaunchKernelGGL(k_radon, dim3(cdiv(L, 256) * num * nz), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_radon, dim3(cdiv(L, kRadonLines) * num * nz),
+                       dim3(kRadonLines), 0, st,
                        d_stack, tstack, nx, ny, num, L, d_cs, reg, d_ri, lines,
                        pitch);
   }
