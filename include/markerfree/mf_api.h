@@ -182,10 +182,14 @@ typedef struct {
   int nproj;         /* -p: neighbouring projections per target */
   int verbose;       /* print the reference's stage trace to stdout */
   int reproj_mode;   /* mf_reproject mode 0..5: 0 slab, 1 fused (default) */
+  int pa_pairs;      /* 1 (default): a projection-matching sweep measures the
+                        pair of targets z0+i, z0-i together when neither
+                        reads the other's image; 0: one target at a time.
+                        Both give identical results. */
 } mf_options;
 
 /* Defaults of InitOpts (main/opts.h:110-124): AlignZ 600, OutBin 1, NPROJ 10;
- * reproj_mode 1. */
+ * reproj_mode 1, pa_pairs 1. */
 void mf_options_default(mf_options* o);
 
 typedef struct mf_series mf_series;

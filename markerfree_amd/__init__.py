@@ -294,7 +294,7 @@ class Options(C.Structure):
                 ("zshift", C.c_float), ("thickness", C.c_int),
                 ("align_z", C.c_int), ("out_bin", C.c_int),
                 ("nproj", C.c_int), ("verbose", C.c_int),
-                ("reproj_mode", C.c_int)]
+                ("reproj_mode", C.c_int), ("pa_pairs", C.c_int)]
 
 
 _sig("mf_options_default", None, C.POINTER(Options))

@@ -455,6 +455,7 @@ void mf_options_default(mf_options* o) {
   o->nproj = 10;
   o->verbose = 0;
   o->reproj_mode = 1;
+  o->pa_pairs = 1;
 }
 
 mf_status mf_series_create(mf_ctx* ctx, int nx, int ny, int nz,

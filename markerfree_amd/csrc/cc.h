@@ -39,6 +39,22 @@ void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
                 const float* d_proj, int w, int h, float* d_work,
                 double* d_mom, float* d_ishift);
 
+// One input image of a correlation: its pixels (row pitch in floats) and the
+// padded work image [h][pad_x(w)] it is normalised and transformed in.
+struct CcImg {
+  const float* src;
+  int pitch;
+  float* dst;
+};
+
+// MeaAlignProj for npair independent pairs at once, images from the device
+// table d_tab (entry 2k: reprojection k, entry 2k+1: projection k, each with
+// its own work image); d_ishift receives npair float2.  Every pair's result
+// equals pa_measure's on the same images.  Workspace: cc_scratch_doubles(
+// npair, h) doubles.
+void pa_measure_tab(mf_ctx* ctx, hipStream_t st, const CcImg* d_tab,
+                    int npair, int w, int h, double* d_mom, float* d_ishift);
+
 size_t stretch_cc_work_floats(int w, int h, int npair);
 size_t cc_scratch_doubles(int npair, int h);
 

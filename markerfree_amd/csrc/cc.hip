@@ -126,14 +126,6 @@ __global__ __launch_bounds__(256) void k_stretch_pairs(
   out[(size_t)blockIdx.y * h * pitch + (size_t)y * pitch + x] = v;
 }
 
-// One input image of a correlation: its pixels (row pitch in floats) and the
-// padded work image [h][pad_x(w)] it is normalised and transformed in.
-struct CcImg {
-  const float* src;
-  int pitch;
-  float* dst;
-};
-
 // Images of a launch: a device table (TAB), or for two-image launches the
 // kernel argument itself, so the per-target measurement needs no upload.  The
 // two inline images are flat scalar fields selected one by one, and the
@@ -634,6 +626,16 @@ void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
   // B = 500 (:258-271); PAFindPeak window (:325-375).
   cc_run(ctx, st, imgs, 1, w, h, 1, 1.0f, 500.0f, 1, cc_mask(ctx, w, h, true),
          d_mom, d_ishift);
+}
+
+void pa_measure_tab(mf_ctx* ctx, hipStream_t st, const CcImg* d_tab,
+                    int npair, int w, int h, double* d_mom, float* d_ishift) {
+  MF_REQUIRE(npair > 0, "no pairs to measure");
+  CcImgs imgs{};
+  imgs.tab = d_tab;
+  // The same normalisation, filter and peak window as pa_measure.
+  cc_run(ctx, st, imgs, npair, w, h, 1, 1.0f, 500.0f, 1,
+         cc_mask(ctx, w, h, true), d_mom, d_ishift);
 }
 
 }  // namespace mf

@@ -15,6 +15,7 @@ struct mf_series {
   int nx = 0, ny = 0, nz = 0;
   int verbose = 0;
   int reproj_mode = 1;       // 0 materialised slab, 1 fused
+  bool pa_pairs = true;      // sweep steps may measure two targets at once
   std::vector<float> file_tilts;  // as read (file order)
   std::vector<float> tilts;       // ascending; + offset once applied
   // AlignParam (Util.h:14-21), shifts in FP64.
@@ -39,7 +40,7 @@ struct mf_series {
   float pa_z0 = 0.0f;
   std::vector<float> pa_angles;
   mf::CorrGeom pa_geom{};
-  mf::DevBuf pa_corr, pa_reproj, pa_work, pa_mom, pa_dev, pa_slab;
+  mf::DevBuf pa_corr, pa_reproj, pa_work, pa_mom, pa_dev, pa_slab, pa_tab;
   std::string last_error;
   ~mf_series();
 };

@@ -22,7 +22,7 @@ mf_series::~mf_series() {
   for (mf::DevBuf* b : {&raw, &stack, &tmp, &tr_corr, &tr_work, &tr_mom,
                         &tr_shift, &tr_tab, &cproj, &off_corr, &pa_corr,
                         &pa_reproj, &pa_work, &pa_mom, &pa_dev,
-                        &pa_slab, &axw.lines, &axw.reg, &axw.small,
+                        &pa_slab, &pa_tab, &axw.lines, &axw.reg, &axw.small,
                         &axw.tstack, &offw.buf})
     b->release();
   if (st) (void)hipStreamDestroy(st);
@@ -80,7 +80,7 @@ CorrProj* upload_corr(mf_series* s, const std::vector<CorrProj>& p) {
 
 // Device-side parameter block of a projection-matching sweep.
 struct PaDev {
-  float2* ishift;  // [1]
+  float2* ishift;  // [2] measured shift of the step's one or two targets
   float2* meas;    // [nz] measured shift (raw pixels) per target
   float* err;      // [nz]
   float2* ind;     // [nz] Z-induced shift (CalcZInducedShift)
@@ -91,25 +91,28 @@ struct PaDev {
 
 PaDev pa_layout(mf_series* s) {
   const int nz = s->nz;
-  const size_t bytes = 8 + (size_t)nz * (8 + 4 + 8 + 8 + 8 + 8) + 64;
+  const size_t bytes = 16 + (size_t)nz * (8 + 4 + 8 + 8 + 8 + 8) + 64;
   char* p = (char*)s->pa_dev.ensure(s->ctx, bytes);
   PaDev d;
   d.sx = (double*)p;
   d.sy = d.sx + nz;
   d.ishift = (float2*)(d.sy + nz);
-  d.meas = d.ishift + 1;
+  d.meas = d.ishift + 2;
   d.ind = d.meas + nz;
   d.rcs = d.ind + nz;
   d.err = (float*)(d.rcs + nz);
   return d;
 }
 
-// ClacAlignProj's parameter update (ProjAlign.cpp:276-296) for target t:
+// ClacAlignProj's parameter update (ProjAlign.cpp:276-296) for the step's
+// targets (thread j: target tj with ishift[j]; one thread per target):
 // shift = RotShift(ishift * bin, axis) + Z-induced; param += shift.  The
 // target's correction parameters are refreshed for its re-correction.
-__global__ void k_pa_update(PaDev d, int t, float binx, float biny,
+__global__ void k_pa_update(PaDev d, int t0, int t1, float binx, float biny,
                             CorrProj* cp) {
-  const float2 ish = d.ishift[0];
+  const int j = threadIdx.x;
+  const int t = j ? t1 : t0;
+  const float2 ish = d.ishift[j];
   const float a0 = ish.x * binx, a1 = ish.y * biny;
   d.meas[t] = make_float2(a0, a1);
   d.err[t] = (float)sqrt((double)(a0 * a0 + a1 * a1));
@@ -344,9 +347,10 @@ void series_projalign_setup(mf_series* s, float zshift, int align_z,
   const CorrGeom& g = s->pa_geom;
   const size_t plane = (size_t)g.bx * g.by, padb = pad_x(g.bx);
   s->pa_corr.ensure(s->ctx, plane * s->nz * sizeof(float));
-  s->pa_reproj.ensure(s->ctx, plane * sizeof(float));
-  s->pa_work.ensure(s->ctx, 2 * (size_t)g.by * padb * sizeof(float));
-  s->pa_mom.ensure(s->ctx, cc_scratch_doubles(1, g.by) * sizeof(double));
+  // Two targets per sweep step at most: two reprojections, four work images.
+  s->pa_reproj.ensure(s->ctx, 2 * plane * sizeof(float));
+  s->pa_work.ensure(s->ctx, 4 * (size_t)g.by * padb * sizeof(float));
+  s->pa_mom.ensure(s->ctx, cc_scratch_doubles(2, g.by) * sizeof(double));
   s->pa_ready = true;
   if (s->verbose)
     printf("bin:%d   cmpsize:%d,%d\nbinsize:%d, %d\n", s->pa_bin, g.bx / 2 + 1,
@@ -417,24 +421,85 @@ float series_projalign_iter(mf_series* s) {
   float* slab = nullptr;
   if (s->reproj_mode == 0)
     slab = (float*)s->pa_slab.ensure(ctx, plane * 2 * s->pa_tb * sizeof(float));
-  std::vector<int> order;
-  auto align = [&](int t) {
-    // ClacAlignProj (ProjAlign.cpp:259-319).
-    reproject_target(ctx, s->st, corr, g.bx, g.by, nz, s->pa_tb, ang.data(),
-                     skip.data(), t, s->pa_nproj, s->reproj_mode, reproj, slab);
-    pa_measure(ctx, s->st, reproj, corr + t * plane, g.bx, g.by, work, mom,
-               (float*)d.ishift);
-    hipLaunchKernelGGL(k_pa_update, dim3(1), dim3(1), 0, s->st, d, t, binx,
-                       biny, cp);
-    MF_HIP(hipGetLastError());
-    correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, t, 1,
-                    true, corr + t * plane);
-    skip[t] = 0;
-    order.push_back(t);
+  // The sweep visits z0+1, z0-1, z0+2, z0-2, ...  A step takes the pair
+  // (z0+i, z0-i) together when the second target's reference range, formed
+  // exactly as it would be after the first is aligned, does not contain the
+  // first: then neither target reads the other's corrected image, both see
+  // the same parameters as in the one-by-one order, and their measurements,
+  // updates and re-corrections can share launches.  The schedule depends
+  // only on the tilts, so it is fixed before the sweep.
+  struct Step {
+    int a, b;  // b < 0: one target
   };
-  for (int i = 1; i < nz; i++) {
-    if (z0 + i < nz) align(z0 + i);
-    if (z0 - i >= 0) align(z0 - i);
+  std::vector<Step> steps;
+  {
+    std::vector<unsigned char> sk = skip;
+    for (int i = 1; i < nz; i++) {
+      const int a = z0 + i < nz ? z0 + i : -1, b = z0 - i >= 0 ? z0 - i : -1;
+      if (a >= 0 && b >= 0) {
+        sk[a] = 0;
+        int r[2];
+        find_proj_range(ang.data(), nz, sk.data(), b, s->pa_nproj, r);
+        sk[b] = 0;
+        if (s->pa_pairs && (r[0] < 0 || a < r[0] || a > r[1])) {
+          steps.push_back({a, b});
+        } else {
+          steps.push_back({a, -1});
+          steps.push_back({b, -1});
+        }
+      } else if (a >= 0 || b >= 0) {
+        const int t = a >= 0 ? a : b;
+        sk[t] = 0;
+        steps.push_back({t, -1});
+      }
+    }
+  }
+  // Correlation image tables of the paired steps: reprojection k and
+  // projection k (k = 0, 1) with their work images.
+  const size_t wimg = (size_t)g.by * pad_x(g.bx);
+  std::vector<CcImg> tab;
+  for (const Step& st : steps)
+    if (st.b >= 0) {
+      const int ts[2] = {st.a, st.b};
+      for (int k = 0; k < 2; k++) {
+        tab.push_back(CcImg{reproj + k * plane, g.bx, work + (2 * k) * wimg});
+        tab.push_back(CcImg{corr + ts[k] * plane, g.bx, work + (2 * k + 1) * wimg});
+      }
+    }
+  CcImg* d_tab = nullptr;
+  if (!tab.empty()) {
+    d_tab = (CcImg*)s->pa_tab.ensure(ctx, tab.size() * sizeof(CcImg));
+    MF_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(CcImg),
+                          hipMemcpyHostToDevice, s->st));
+  }
+  std::vector<int> order;
+  int paired = 0;
+  for (const Step& st : steps) {
+    // ClacAlignProj (ProjAlign.cpp:259-319), for one target or a pair.
+    const int n = st.b >= 0 ? 2 : 1;
+    const int ts[2] = {st.a, st.b};
+    for (int k = 0; k < n; k++) {
+      reproject_target(ctx, s->st, corr, g.bx, g.by, nz, s->pa_tb, ang.data(),
+                       skip.data(), ts[k], s->pa_nproj, s->reproj_mode,
+                       reproj + k * plane, slab);
+      skip[ts[k]] = 0;  // the next target's range sees this one aligned
+    }
+    if (n == 2) {
+      pa_measure_tab(ctx, s->st, d_tab + 4 * paired, 2, g.bx, g.by, mom,
+                     (float*)d.ishift);
+      paired++;
+    } else {
+      pa_measure(ctx, s->st, reproj, corr + st.a * plane, g.bx, g.by, work,
+                 mom, (float*)d.ishift);
+    }
+    hipLaunchKernelGGL(k_pa_update, dim3(1), dim3(n), 0, s->st, d, st.a,
+                       st.b, binx, biny, cp);
+    MF_HIP(hipGetLastError());
+    for (int k = 0; k < n; k++) {
+      correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, ts[k],
+                      1, true, corr + ts[k] * plane);
+      order.push_back(ts[k]);
+    }
   }
   std::vector<float> err(nz), meas(2 * nz);
   MF_HIP(hipMemcpyAsync(s->sx.data(), d.sx, nz * 8, hipMemcpyDeviceToHost, s->st));
@@ -496,13 +561,14 @@ size_t series_hbm_bytes(int nx, int ny, int nz, const mf_options* o) {
   const CorrGeom gc = CorrGeom::make(nx, ny, tb);
   const size_t coarse = (size_t)4 * gc.bx * gc.by * nz +
                         stretch_cc_work_floats(gc.bx, gc.by, nz) * 4;
-  // Projection matching: the corrected, R-weighted stack and the
-  // measurement's work images; the slab if materialised.
+  // Projection matching: the corrected, R-weighted stack, two
+  // reprojections and four work images (a sweep step measures up to two
+  // targets); the slab if materialised.
   const int pb = std::max(std::max((int)(nx / 2048.0f + 0.5f),
                                    (int)(ny / 2048.0f + 0.5f)), 1);
   const CorrGeom gp = CorrGeom::make(nx, ny, pb);
   size_t pa = (size_t)4 * nz * gp.by * gp.bx +
-              (size_t)4 * 3 * gp.by * pad_x(gp.bx);
+              (size_t)4 * 6 * gp.by * pad_x(gp.bx);
   if (o->reproj_mode == 0)
     pa += (size_t)4 * 2 * gp.bx * gp.by * (o->align_z / pb / 2 * 2);
   // Common lines (lines and spectra of every image and candidate, the
@@ -518,6 +584,7 @@ void series_run(mf_series* s, const mf_options* o) {
   // Process::DoIt (Process.cpp:34-92) minus file I/O.
   s->verbose = o->verbose;
   s->reproj_mode = o->reproj_mode;
+  s->pa_pairs = o->pa_pairs != 0;
   if (!s->preprocessed) series_preprocess(s);
   if (o->align_z == 0) return;
   // SetParam (Process.cpp:208-221).

@@ -237,18 +237,24 @@ def test_gpu_pipeline_lockstep(ctx, mf, orc, series):
 
 @pytest.mark.gpu
 def test_gpu_run_modes_agree_and_recover_truth(ctx, mf, series):
-    """mf_run end to end in both reprojection modes: identical results (the
-    fused and materialised paths are bit-identical), and the recovered axis
-    and shifts are close to the generator's truth (sanity, not parity)."""
+    """mf_run end to end in both reprojection modes, and with the
+    projection-matching sweep measuring independent target pairs together
+    or one target at a time: identical results (the fused and materialised
+    paths are bit-identical, and a pair is formed only when neither target
+    reads the other's image), and the recovered axis and shifts are close to
+    the generator's truth (sanity, not parity)."""
     s = series
-    res = []
-    for mode in (0, 1):
+    res, outs = [], []
+    for mode, pairs in ((0, 1), (1, 1), (1, 0)):
         G = mf.Series(ctx, s.stack, s.tilts)
-        G.run(align_z=96, nproj=10, reproj_mode=mode)
+        G.run(align_z=96, nproj=10, reproj_mode=mode, pa_pairs=pairs)
         res.append(G.params())
+        outs.append(G.correct(1))
         G.close()
-    np.testing.assert_array_equal(res[0][0], res[1][0])
-    np.testing.assert_array_equal(res[0][1], res[1][1])
+    for r, o in zip(res[1:], outs[1:]):
+        np.testing.assert_array_equal(res[0][0], r[0])
+        np.testing.assert_array_equal(res[0][1], r[1])
+        np.testing.assert_array_equal(outs[0], o)
     sx, sy, rot, off = res[0]
     assert abs(float(rot[0]) - s.axis) < 2.0
     z0 = int(np.argmin(np.abs(s.tilts)))
