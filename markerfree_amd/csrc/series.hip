@@ -495,11 +495,10 @@ float series_projalign_iter(mf_series* s) {
     hipLaunchKernelGGL(k_pa_update, dim3(1), dim3(n), 0, s->st, d, st.a,
                        st.b, binx, biny, cp);
     MF_HIP(hipGetLastError());
-    for (int k = 0; k < n; k++) {
-      correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, ts[k],
-                      1, true, corr + ts[k] * plane);
-      order.push_back(ts[k]);
-    }
+    // Re-correct the step's targets (one launch for a pair).
+    correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, st.a, n,
+                    true, corr + st.a * plane, n == 2 ? st.b - st.a : 1);
+    for (int k = 0; k < n; k++) order.push_back(ts[k]);
   }
   std::vector<float> err(nz), meas(2 * nz);
   MF_HIP(hipMemcpyAsync(s->sx.data(), d.sx, nz * 8, hipMemcpyDeviceToHost, s->st));

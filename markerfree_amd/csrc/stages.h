@@ -55,11 +55,12 @@ void correct_bin(mf_ctx* ctx, hipStream_t st, const float* d_raw,
 void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
                   int nrows, float* d_out);
 // correct_bin with random fill choice, then rweight_rows, in one pass: the
-// R-weighted binned projections z0 .. z0+count-1 go to d_out as
-// [count][by][bx] (unpadded; d_out holds projection z0 first).
+// R-weighted binned projections z0 + k*zstep (k < count) go to d_out as
+// planes of by x bx (unpadded), projection z0 + k*zstep at plane k*zstep
+// (d_out holds projection z0 first).
 void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
                      const CorrGeom& g, const CorrProj* d_proj, int z0,
-                     int count, bool random_fill, float* d_out);
+                     int count, bool random_fill, float* d_out, int zstep = 1);
 
 // ------------------------------------------------------- correlation
 // Per-image FP64 moments sum(v) and sum(v^2) over valid pixels of w x h

@@ -129,6 +129,7 @@ struct CorrArgs {
   float* out;
   int nx, ny, cx, bx, by, binx, biny, z0, fill, out_pitch;
   size_t out_stride;
+  int zstep;  // k_correct_rweight: image z0 + blockIdx.y * zstep
 };
 
 // Binned output pixel (x, y) of projection z: each of its bin x bin corrected
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(kRowThreads) void k_correct_rweight(
   float2* tw = X + nl * xs;
   const int r0 = blockIdx.x * nl;
   const int nv = min(nl, ca.by - r0);
-  const int z = ca.z0 + blockIdx.y;
+  const int z = ca.z0 + (int)blockIdx.y * ca.zstep;
   const CorrProj pr = ca.proj[z];
   const float* raw = ca.raw + (size_t)z * ca.nx * ca.ny;
   fft_load_tw(P, tw);
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(kRowThreads) void k_correct_rweight(
   }
   __syncthreads();
   float2* zz = fft_lines(a, b, P, nl, true, tw);
-  float* out = ca.out + (size_t)blockIdx.y * ca.out_stride;
+  float* out = ca.out + (ptrdiff_t)blockIdx.y * ca.zstep * (ptrdiff_t)ca.out_stride;
   for (LineIter it(N); it.l < nl; it.next()) {
     int l = it.l, k = it.k;
     if (l >= nv) continue;
@@ -400,6 +401,7 @@ void correct_bin(mf_ctx* ctx, hipStream_t st, const float* d_raw,
   a.fill = random_fill ? 1 : 0;
   a.out_pitch = out_pitch;
   a.out_stride = out_stride;
+  a.zstep = 1;
   TimedLaunch tl(ctx->timer, "correct", st);
   hipLaunchKernelGGL(k_correct_bin, dim3(cdiv((long long)g.bx * g.by, 256), count),
                      dim3(256), 0, st, a);
@@ -428,8 +430,9 @@ void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
 
 void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
                      const CorrGeom& g, const CorrProj* d_proj, int z0,
-                     int count, bool random_fill, float* d_out) {
+                     int count, bool random_fill, float* d_out, int zstep) {
   if (count <= 0) return;
+  MF_REQUIRE(zstep != 0 || count == 1, "image step must be nonzero");
   MF_REQUIRE(g.bx % 2 == 0 && g.bx >= 2, "R-weighting needs an even row length");
   CorrArgs a;
   a.raw = d_raw;
@@ -446,6 +449,7 @@ void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
   a.fill = random_fill ? 1 : 0;
   a.out_pitch = g.bx;
   a.out_stride = (size_t)g.bx * g.by;
+  a.zstep = zstep;
   const FftDevPlan& P = ctx->fft->plan(g.bx / 2);
   const int ls = fft_lds_len(P);
   const size_t per = (size_t)(2 * ls + P.n + 1) * sizeof(float2);
