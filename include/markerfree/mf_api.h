@@ -196,10 +196,20 @@ typedef struct mf_series mf_series;
 
 /* Copies a raw stack [nz][ny][nx] (file order, as read from the MRC) and its
  * tilt angles (file order, the .rawtlt) into HBM.  Replaces
- * Process::ReadStack/ReadAngles (main/Process.cpp:129-206) minus file I/O. */
+ * Process::ReadStack/ReadAngles (main/Process.cpp:129-206) minus file I/O.
+ * stack_zyx NULL: the raw stack is left to mf_series_upload. */
 mf_status mf_series_create(mf_ctx* ctx, int nx, int ny, int nz,
                            const float* stack_zyx, const float* tilts,
                            mf_series** out);
+/* Streaming ingest (no counterpart in the reference, which reads the whole
+ * stack before any GPU work): copies raw images [z0, z0+count) from host
+ * memory into a series created with stack_zyx NULL.  The copy is queued on
+ * the series' stream (asynchronous when host is page-locked, from
+ * mf_host_alloc), so later stages are ordered after it; the call returns
+ * once the previous upload has finished, so a caller reading the next
+ * images into the previous upload's buffer overlaps the reading with this
+ * copy.  count 0 only waits for the last upload. */
+mf_status mf_series_upload(mf_series* s, int z0, int count, const float* host);
 /* Destroy every series before the context it was created on. */
 void mf_series_destroy(mf_series* s);
 const char* mf_series_last_error(const mf_series* s);

@@ -34,6 +34,9 @@ int mfio_mrc_read_info(const char* path, mfio_mrc_info* info);
  * 1024 + next bytes, converted to float (mode 0 unsigned, 1 signed int16,
  * 2 float32; mode 6 unsigned int16 in addition).  out: nx*ny*nz floats. */
 int mfio_mrc_read(const char* path, float* out);
+/* Images [z0, z0+count) only, as float [count][ny][nx], for streaming
+ * ingest (reads no other image data). */
+int mfio_mrc_read_slices(const char* path, int z0, int count, float* out);
 
 /* The aligned-stack writer of Process::CorrectStack (Process.cpp:333-351):
  * InitializeHeader + SetSize (mrcstack.cpp:421-454), float32 data, then

@@ -286,6 +286,7 @@ _sig("mf_correct_proj", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_float,
 _sig("mf_stretch_cc", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.c_float,
      C.c_float, C.c_float, C.c_float, _fp)
 _sig("mf_pa_measure", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _fp)
+_sig("mf_series_upload", C.c_int, _vp, C.c_int, C.c_int, _fp)
 
 
 class Options(C.Structure):
@@ -380,13 +381,20 @@ Context.pa_measure = _ctx_pa_measure
 class Series:
     """mf_series: one tilt series resident in HBM (Process counterpart)."""
 
-    def __init__(self, ctx: Context, stack, tilts_file_order):
-        stack = np.ascontiguousarray(stack, np.float32)
-        self.nz, self.ny, self.nx = stack.shape
+    def __init__(self, ctx: Context, stack, tilts_file_order, shape=None):
+        """stack [nz][ny][nx] in file order; or stack None and shape
+        (nz, ny, nx), the images to follow through upload()."""
+        if stack is None:
+            self.nz, self.ny, self.nx = shape
+            ptr = None
+        else:
+            stack = np.ascontiguousarray(stack, np.float32)
+            self.nz, self.ny, self.nx = stack.shape
+            ptr = f32p(stack)
         t = np.ascontiguousarray(tilts_file_order, np.float32)
         h = _vp()
         ctx._check(_lib.mf_series_create(ctx.h, self.nx, self.ny, self.nz,
-                                         f32p(stack), f32p(t), C.byref(h)))
+                                         ptr, f32p(t), C.byref(h)))
         self.ctx = ctx
         self.h = h.value
         ctx._series.add(self)
@@ -408,6 +416,19 @@ class Series:
 
     def verbose(self, on=True):
         self._check(_lib.mf_series_set_verbose(self.h, int(on)))
+
+    def upload(self, z0, images):
+        """mf_series_upload: raw images [count][ny][nx] from z0 on (returns
+        once the previous upload is done; 'images' must stay alive and
+        unchanged until the next upload() call returns).  images None waits
+        for the last upload."""
+        if images is None:
+            self._check(_lib.mf_series_upload(self.h, 0, 0, None))
+            return
+        assert images.dtype == np.float32 and images.flags.c_contiguous
+        assert images.shape[1:] == (self.ny, self.nx)
+        self._check(_lib.mf_series_upload(self.h, z0, images.shape[0],
+                                          f32p(images)))
 
     def preprocess(self):
         self._check(_lib.mf_preprocess(self.h))

@@ -465,7 +465,7 @@ mf_status mf_series_create(mf_ctx* ctx, int nx, int ny, int nz,
   *out = nullptr;
   mf_series* s = nullptr;
   mf_status st = guard(ctx, [&] {
-    MF_REQUIRE(stack && tilts && nx >= 16 && ny >= 16 && nz >= 2,
+    MF_REQUIRE(tilts && nx >= 16 && ny >= 16 && nz >= 2,
                "series needs at least 2 images of 16x16");
     DeviceScope d(ctx);
     s = new mf_series;
@@ -477,8 +477,10 @@ mf_status mf_series_create(mf_ctx* ctx, int nx, int ny, int nz,
     MF_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     const size_t bytes = (size_t)nx * ny * nz * sizeof(float);
     float* raw = (float*)s->raw.ensure(ctx, bytes);
-    MF_HIP(hipMemcpyAsync(raw, stack, bytes, hipMemcpyHostToDevice, s->st));
-    MF_HIP(hipStreamSynchronize(s->st));
+    if (stack) {
+      MF_HIP(hipMemcpyAsync(raw, stack, bytes, hipMemcpyHostToDevice, s->st));
+      MF_HIP(hipStreamSynchronize(s->st));
+    }
     s->sx.assign(nz, 0.0);
     s->sy.assign(nz, 0.0);
     s->rot.assign(nz, 0.0f);
@@ -519,6 +521,31 @@ mf_status sguard(mf_series* s, F&& f) {
 }  // namespace
 
 extern "C" {
+
+mf_status mf_series_upload(mf_series* s, int z0, int count, const float* host) {
+  if (!s) return MF_EINVAL;
+  return sguard(s, [&] {
+    MF_REQUIRE(count >= 0 && z0 >= 0 && z0 + count <= s->nz,
+               "image range out of bounds");
+    MF_REQUIRE(count == 0 || host, "null host images");
+    MF_REQUIRE(!s->preprocessed, "the raw stack is gone after preprocessing");
+    DeviceScope d(s->ctx);
+    for (hipEvent_t& e : s->up_ev)
+      if (!e) MF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // Upload k records event k mod 2, then waits for upload k-1.
+    const long long k = s->up_n;
+    if (count > 0) {
+      const size_t plane = (size_t)s->nx * s->ny;
+      float* raw = (float*)s->raw.get() + (size_t)z0 * plane;
+      MF_HIP(hipMemcpyAsync(raw, host, (size_t)count * plane * sizeof(float),
+                            hipMemcpyHostToDevice, s->st));
+      MF_HIP(hipEventRecord(s->up_ev[k & 1], s->st));
+      s->up_n = k + 1;
+    }
+    if (k > 0) MF_HIP(hipEventSynchronize(s->up_ev[(k - 1) & 1]));
+  });
+}
+
 
 mf_status mf_series_set_verbose(mf_series* s, int verbose) {
   return sguard(s, [&] { s->verbose = verbose; });

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <sys/types.h>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -95,17 +96,21 @@ int mfio_mrc_read_info(const char* path, mfio_mrc_info* info) {
   return 0;
 }
 
-int mfio_mrc_read(const char* path, float* out) {
+int mfio_mrc_read_slices(const char* path, int z0, int count, float* out) {
   if (!path || !out) return fail("null argument");
   File fh(std::fopen(path, "rb"));
   if (!fh.f) return fail(std::string("cannot open ") + path);
   MrcHeader h;
   if (read_header(fh.f, path, &h)) return -1;
-  if (std::fseek(fh.f, (long)sizeof(MrcHeader) + h.next, SEEK_SET) != 0)
-    return fail(std::string("cannot seek to MRC data: ") + path);
+  if (z0 < 0 || count < 0 || z0 + count > h.nz)
+    return fail(std::string("MRC image range out of bounds: ") + path);
   const size_t plane = (size_t)h.nx * h.ny, psize = mode_bytes(h.mode);
+  const long long at = (long long)sizeof(MrcHeader) + h.next +
+                       (long long)z0 * (long long)(plane * psize);
+  if (fseeko(fh.f, (off_t)at, SEEK_SET) != 0)
+    return fail(std::string("cannot seek to MRC data: ") + path);
   std::vector<unsigned char> buf(plane * psize);
-  for (int z = 0; z < h.nz; z++) {
+  for (int z = 0; z < count; z++) {
     if (std::fread(buf.data(), psize, plane, fh.f) != plane)
       return fail(std::string("short MRC data: ") + path);
     float* o = out + (size_t)z * plane;
@@ -129,6 +134,12 @@ int mfio_mrc_read(const char* path, float* out) {
     }
   }
   return 0;
+}
+
+int mfio_mrc_read(const char* path, float* out) {
+  mfio_mrc_info info;
+  if (mfio_mrc_read_info(path, &info)) return -1;
+  return mfio_mrc_read_slices(path, 0, info.nz, out);
 }
 
 int mfio_mrc_write(const char* path, const float* data, int nx, int ny,

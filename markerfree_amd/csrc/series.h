@@ -16,6 +16,8 @@ struct mf_series {
   int verbose = 0;
   int reproj_mode = 1;       // 0 materialised slab, 1 fused
   bool pa_pairs = true;      // sweep steps may measure two targets at once
+  hipEvent_t up_ev[2] = {nullptr, nullptr};  // streaming uploads (alternate)
+  long long up_n = 0;                        // uploads recorded
   std::vector<float> file_tilts;  // as read (file order)
   std::vector<float> tilts;       // ascending; + offset once applied
   // AlignParam (Util.h:14-21), shifts in FP64.

@@ -25,6 +25,8 @@ mf_series::~mf_series() {
                         &pa_slab, &pa_tab, &axw.lines, &axw.reg, &axw.small,
                         &axw.tstack, &offw.buf})
     b->release();
+  for (hipEvent_t e : up_ev)
+    if (e) (void)hipEventDestroy(e);
   if (st) (void)hipStreamDestroy(st);
 }
 

@@ -37,6 +37,8 @@ def lib():
         L.mfio_last_error.restype = C.c_char_p
         L.mfio_mrc_read_info.argtypes = [C.c_char_p, C.POINTER(MrcInfo)]
         L.mfio_mrc_read.argtypes = [C.c_char_p, f32p]
+        L.mfio_mrc_read_slices.argtypes = [C.c_char_p, C.c_int, C.c_int,
+                                           f32p]
         L.mfio_mrc_write.argtypes = [C.c_char_p, f32p, C.c_int, C.c_int,
                                      C.c_int, f32p]
         L.mfio_read_angles.argtypes = [C.c_char_p, f32p, C.c_int]
@@ -75,6 +77,15 @@ def read_mrc(path) -> np.ndarray:
     info = mrc_info(path)
     out = np.empty((info.nz, info.ny, info.nx), np.float32)
     _check(lib().mfio_mrc_read(str(path).encode(), _f32(out)))
+    return out
+
+
+def read_mrc_slices(path, z0, count) -> np.ndarray:
+    """Images [z0, z0+count) as float32 [count][ny][nx]."""
+    info = mrc_info(path)
+    out = np.empty((max(count, 0), info.ny, info.nx), np.float32)
+    _check(lib().mfio_mrc_read_slices(str(path).encode(), z0, count,
+                                      _f32(out)))
     return out
 
 

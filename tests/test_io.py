@@ -131,6 +131,28 @@ def test_mrc_reader_modes_and_extended_header(fio, tmp_path, dtype):
     np.testing.assert_array_equal(fio.read_mrc(path), a.astype(np.float32))
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.float32, np.uint16])
+def test_mrc_slice_reads_match_whole_read(fio, tmp_path, dtype):
+    """Streaming ingest reads image ranges: every range equals the same
+    images of the whole read, past an extended header; ranges outside the
+    stack are refused."""
+    rng = np.random.default_rng(2)
+    if dtype == np.float32:
+        a = rng.standard_normal((7, 6, 5)).astype(dtype)
+    else:
+        info = np.iinfo(dtype)
+        a = rng.integers(info.min, info.max, (7, 6, 5), endpoint=True).astype(dtype)
+    path = tmp_path / "s.mrc"
+    fio.write_mrc(path, a, next_bytes=bytes(range(37)))
+    whole = fio.read_mrc(path)
+    for z0, n in ((0, 7), (0, 1), (3, 2), (6, 1), (2, 0)):
+        np.testing.assert_array_equal(fio.read_mrc_slices(path, z0, n),
+                                      whole[z0:z0 + n])
+    for z0, n in ((5, 3), (-1, 1), (0, 8)):
+        with pytest.raises(OSError, match="out of bounds"):
+            fio.read_mrc_slices(path, z0, n)
+
+
 def test_mrc_reader_rejects_unknown_mode_and_short_files(fio, tmp_path):
     path = tmp_path / "c.mrc"
     fio.write_mrc(path, np.zeros((2, 2, 2), np.float32))

@@ -121,3 +121,31 @@ def test_gpu_pinned_host_buffers_same_results(ctx, mf):
     with pytest.raises(ValueError):
         G.correct(1, out=np.empty((1, 2, 3), np.float32))
     G.close()
+
+
+@pytest.mark.gpu
+def test_gpu_streamed_upload_equals_whole_stack(ctx, mf):
+    """Streaming ingest: a series created without images and filled chunk by
+    chunk (page-locked and ordinary host buffers, uneven chunks) gives the
+    same results as one created from the whole stack."""
+    s = _mix()[1]
+    ref = _alone(mf, ctx, s)
+    nz, ny, nx = s.stack.shape
+    for pinned, chunk in ((True, 8), (False, 5)):
+        G = mf.Series(ctx, None, s.tilts, shape=(nz, ny, nx))
+        bufs = [mf.pinned_empty((chunk, ny, nx)) if pinned
+                else np.empty((chunk, ny, nx), np.float32) for _ in range(2)]
+        for k, z0 in enumerate(range(0, nz, chunk)):
+            n = min(chunk, nz - z0)
+            b = bufs[k % 2][:n]
+            b[...] = s.stack[z0:z0 + n]  # this buffer's last upload is done
+            G.upload(z0, b)
+        G.upload(0, None)
+        with pytest.raises(mf.MFError, match="out of bounds"):
+            G.upload(nz - 1, bufs[0][:2])
+        G.run(**OPTS)
+        _same((G.params(), G.correct(1)), ref)
+        with pytest.raises(mf.MFError, match="preprocessing"):
+            G.upload(0, bufs[0][:1])
+        G.close()
+
