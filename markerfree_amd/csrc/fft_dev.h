@@ -208,6 +208,121 @@ __device__ __forceinline__ float2* fft_stockham(float2* x, float2* y, int N,
   return x;
 }
 
+// ------------------------------------------------ power-of-two transforms
+// Lines of N = 2^k points (256..4096) take a Stockham schedule fixed at
+// compile time: radix-8 stages, then one radix-2 or radix-4 stage for the
+// remainder.  Every index is a shift or mask, every stride an immediate
+// offset, and the complex products are contracted to FMAs; no runtime
+// radix dispatch remains.  Same twiddle table and Stockham ordering as the
+// generic path (results agree to FP32 rounding, not bit for bit).
+
+__host__ __device__ constexpr int pow2_log(int n) {
+  return n <= 1 ? 0 : 1 + pow2_log(n / 2);
+}
+// Radix of stage s for N: eights, then 2^(log N mod 3).
+__host__ __device__ constexpr int pow2_radix(int N, int s) {
+  return s < pow2_log(N) / 3 ? 8 : (1 << (pow2_log(N) % 3));
+}
+__host__ __device__ constexpr int pow2_stages(int N) {
+  return pow2_log(N) / 3 + (pow2_log(N) % 3 ? 1 : 0);
+}
+
+template <bool INV>
+__device__ __forceinline__ float2 tw_mul(float2 v, float2 w) {
+#pragma clang fp contract(fast)
+  return INV ? make_float2(v.x * w.x + v.y * w.y, v.y * w.x - v.x * w.y)
+             : make_float2(v.x * w.x - v.y * w.y, v.y * w.x + v.x * w.y);
+}
+
+// One radix-R butterfly (R = 2, 4, 8): inputs src[q*SS] times twiddles
+// tw[q*tws], DFT_R, outputs dst[p*DS].
+template <int R, int SS, int DS, bool INV>
+__device__ __forceinline__ void pow2_butterfly(const float2* src, float2* dst,
+                                               int tws,
+                                               const float2* __restrict__ tw) {
+#pragma clang fp contract(fast)
+  float2 x[R];
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    x[q] = src[q * SS];
+    if (q > 0 && tws) x[q] = tw_mul<INV>(x[q], tw[q * tws]);
+  }
+  if (R == 2) {
+    dst[0] = cadd(x[0], x[1]);
+    dst[DS] = csub(x[0], x[1]);
+  } else if (R == 4) {
+    const float2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+    const float2 c = cadd(x[1], x[3]), d = cmul_mi(csub(x[1], x[3]), INV);
+    dst[0] = cadd(a, c);
+    dst[DS] = cadd(b, d);
+    dst[2 * DS] = csub(a, c);
+    dst[3 * DS] = csub(b, d);
+  } else {
+    const float kS = 0.70710678118654752f;
+    float2 a = cadd(x[0], x[4]), b = csub(x[0], x[4]);
+    float2 c = cadd(x[2], x[6]), d = cmul_mi(csub(x[2], x[6]), INV);
+    const float2 e0 = cadd(a, c), e1 = cadd(b, d), e2 = csub(a, c),
+                 e3 = csub(b, d);
+    a = cadd(x[1], x[5]);
+    b = csub(x[1], x[5]);
+    c = cadd(x[3], x[7]);
+    d = cmul_mi(csub(x[3], x[7]), INV);
+    float2 o0 = cadd(a, c), o1 = cadd(b, d), o2 = csub(a, c), o3 = csub(b, d);
+    // w8^1 o1 and w8^3 o3: (x, y) * (s, -s) forward, (s, s) inverse.
+    o1 = INV ? make_float2(kS * (o1.x - o1.y), kS * (o1.x + o1.y))
+             : make_float2(kS * (o1.x + o1.y), kS * (o1.y - o1.x));
+    o2 = cmul_mi(o2, INV);
+    o3 = INV ? make_float2(-kS * (o3.x + o3.y), kS * (o3.x - o3.y))
+             : make_float2(kS * (o3.y - o3.x), -kS * (o3.x + o3.y));
+    dst[0] = cadd(e0, o0);
+    dst[4 * DS] = csub(e0, o0);
+    dst[DS] = cadd(e1, o1);
+    dst[5 * DS] = csub(e1, o1);
+    dst[2 * DS] = cadd(e2, o2);
+    dst[6 * DS] = csub(e2, o2);
+    dst[3 * DS] = cadd(e3, o3);
+    dst[7 * DS] = csub(e3, o3);
+  }
+}
+
+// Stage S of the N-point schedule over nl lines (stride N) from x to y.
+template <int N, int S, bool INV>
+__device__ __forceinline__ void pow2_stage(const float2* x, float2* y, int nl,
+                                           const float2* __restrict__ tw) {
+  constexpr int R = pow2_radix(N, S);
+  constexpr int NS = S == 0 ? 1 : (1 << (3 * S));  // product of earlier radices
+  constexpr int NR = N / R;
+  constexpr int TSTRIDE = N / (NS * R);
+  for (LineIter it(NR); it.l < nl; it.next()) {
+    const int j = it.k;
+    const int q = j / NS, k = j & (NS - 1);
+    pow2_butterfly<R, NR, NS, INV>(x + it.l * N + j,
+                                   y + it.l * N + q * NS * R + k,
+                                   k * TSTRIDE, tw);
+  }
+  __syncthreads();
+}
+
+template <int N, int S, bool INV>
+__device__ __forceinline__ float2* pow2_stages_from(float2* x, float2* y,
+                                                    int nl,
+                                                    const float2* tw) {
+  if constexpr (S == pow2_stages(N)) {
+    return x;
+  } else {
+    pow2_stage<N, S, INV>(x, y, nl, tw);
+    return pow2_stages_from<N, S + 1, INV>(y, x, nl, tw);
+  }
+}
+
+// Complex transform of nl lines of N points (stride N) in x, y scratch;
+// returns the buffer holding the result.  Caller syncs after loading.
+template <int N, bool INV>
+__device__ __forceinline__ float2* fft_pow2(float2* x, float2* y, int nl,
+                                           const float2* tw) {
+  return pow2_stages_from<N, 0, INV>(x, y, nl, tw);
+}
+
 // Line stride in LDS for a plan.
 __host__ __device__ __forceinline__ int fft_lds_len(const FftDevPlan& P) {
   return P.blue ? P.m : P.n;
@@ -228,8 +343,27 @@ __device__ __forceinline__ void fft_load_tw(const FftDevPlan& P, float2* tw) {
 __device__ __forceinline__ float2* fft_lines(float2* buf, float2* tmp,
                                              const FftDevPlan& P, int nl,
                                              bool inv, const float2* tw) {
-  if (!P.blue)
-    return fft_stockham(buf, tmp, P.n, P.n, P.nrad, P.rad, tw, nl, inv);
+  if (!P.blue) {
+    switch (P.n) {
+      case 256:
+        return inv ? fft_pow2<256, true>(buf, tmp, nl, tw)
+                   : fft_pow2<256, false>(buf, tmp, nl, tw);
+      case 512:
+        return inv ? fft_pow2<512, true>(buf, tmp, nl, tw)
+                   : fft_pow2<512, false>(buf, tmp, nl, tw);
+      case 1024:
+        return inv ? fft_pow2<1024, true>(buf, tmp, nl, tw)
+                   : fft_pow2<1024, false>(buf, tmp, nl, tw);
+      case 2048:
+        return inv ? fft_pow2<2048, true>(buf, tmp, nl, tw)
+                   : fft_pow2<2048, false>(buf, tmp, nl, tw);
+      case 4096:
+        return inv ? fft_pow2<4096, true>(buf, tmp, nl, tw)
+                   : fft_pow2<4096, false>(buf, tmp, nl, tw);
+      default:
+        return fft_stockham(buf, tmp, P.n, P.n, P.nrad, P.rad, tw, nl, inv);
+    }
+  }
   const int n = P.n, m = P.m;
   for (LineIter it(m); it.l < nl; it.next()) {
     const int l = it.l, k = it.k;
