@@ -102,6 +102,22 @@ def tune(size=(1024, 1024), nz=61, reps=20, log=print):
                                   "cc_rowsi" + pre], "cc",
                             lambda: s.coarse_iter(), max(2, reps // 10), log))
         s.close()
+        # Projection-matching sweeps: the shapes of a step that measures and
+        # re-corrects two targets at once, and of the whole-stack correction
+        # that starts a sweep (the one-target shapes are tuned above).  One
+        # sweep per timing; the other shapes of the sweep keep their values.
+        s = Series(ctx, stack, tilts)
+        s.preprocess()
+        s.projalign_setup(0.0, 300, 10)
+        sweeps = max(2, reps // 10)
+        out.update(_descend(ctx, ["cc_rowsf" + f"/{w}x{h}x2/",
+                                  "cc_cols" + f"/{w}x{h}x2/",
+                                  "cc_rowsi" + f"/{w}x{h}x2/"], "cc",
+                            lambda: s.projalign_iter(), sweeps, log))
+        out.update(_descend(ctx, [f"corr_rw/{w}x{h}x2/", f"corr_rw/{w}x{h}x{nz}/"],
+                            "correct_rweight", lambda: s.projalign_iter(),
+                            sweeps, log))
+        s.close()
     return out
 
 
