@@ -13,6 +13,7 @@
 #include "axis.h"
 #include "cc.h"
 #include "context.h"
+#include "devutil.h"
 #include "fft_dev.h"
 #include "stages.h"
 
@@ -88,11 +89,16 @@ struct RadonImg {
 // GRadon (Rotate.cu:83-136): mean of the nearest non-negative pixels along
 // the candidate direction over the cos(tilt)-cropped common region; the sum
 // is sequential in x exactly as the reference thread sums it.  One-dimensional
-// grid over (line block, candidate, image), image slowest, in the hardware's
-// own round-robin over the XCDs: every XCD works on the same image at once.
-// One wave per workgroup and eight samples in flight per lane; measured 1.4x
-// faster at 1024^2 and 1.8x at 5760x4092 than 256-lane workgroups with four
-// samples in flight and a per-XCD image range.  The lanes of a wave trace
+// grid over (candidate, line block, image), image slowest and candidate
+// fastest, so that blocks launched together trace nearly the same pixels.
+// In the hardware's round-robin over the XCDs every XCD works on the same
+// image at once; with 'remap' consecutive blocks (neighbouring candidates)
+// share one XCD's L2 instead, which pays for images much larger than the L2
+// and candidates a small fraction of a degree apart (5760x4092, 0.2 degree:
+// 99 -> 76 ms) and costs elsewhere (1024^2, or 1.8 degree steps).  One wave
+// per workgroup and eight samples in flight per lane; measured 1.4x faster
+// at 1024^2 and 1.8x at 5760x4092 than 256-lane workgroups with four
+// samples in flight.  The lanes of a wave trace
 // parallel lines a pixel apart, stepping
 // by (-sin, cos); for candidates nearer the image's x axis than its y axis
 // that step crosses rows, so those read the transposed stack (tstack, [nz]
@@ -124,13 +130,15 @@ __global__ __launch_bounds__(kRadonLines) void k_radon(const float* stack,
                                                int ny, int num, int L,
                                                const float* cs, const int* reg,
                                                const RadonImg* imgs,
-                                               float* lines, int pitch) {
+                                               float* lines, int pitch,
+                                               int remap) {
   const int yblocks = cdiv(L, kRadonLines);
-  const int blk = blockIdx.x;
+  const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const int z = blk / (yblocks * num);
   const int rem = blk - z * yblocks * num;
-  const int k = rem / yblocks;
-  const int y = (rem - k * yblocks) * blockDim.x + threadIdx.x;
+  const int yb = rem / num;
+  const int k = rem - yb * num;
+  const int y = yb * blockDim.x + threadIdx.x;
   if (y >= L) return;
   const RadonImg im = imgs[z];
   const float c = cs[2 * k], s = cs[2 * k + 1];
@@ -473,10 +481,14 @@ float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
   }
   {
     TimedLaunch tl(ctx->timer, "radon", st);
+    // Neighbouring candidates on one XCD for images well beyond its 4 MB L2
+    // and candidate steps under a degree.
+    const float step = num > 1 ? std::fabs(a.rot[1] - a.rot[0]) : 180.0f;
+    const int remap = (size_t)nx * ny * 4 > ((size_t)32 << 20) && step < 1.0f;
     hipLaunchKernelGGL(k_radon, dim3(cdiv(L, kRadonLines) * num * nz),
                        dim3(kRadonLines), 0, st,
                        d_stack, tstack, nx, ny, num, L, d_cs, reg, d_ri, lines,
-                       pitch);
+                       pitch, remap);
   }
   MF_HIP(hipGetLastError());
   return lines;
