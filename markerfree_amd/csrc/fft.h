@@ -22,6 +22,7 @@ struct FftDevPlan {
   int m;       // Bluestein convolution length (smooth), else n
   int nrad;    // radices of the direct transform (of n, or of m)
   int rad[kMaxRad];
+  const int* drad;      // the same radices in device memory (read per stage)
   const float2* tw;     // exp(-2 pi i k / L), L = direct length
   const float2* chirp;  // Bluestein: exp(-i pi k^2 / n), k < n
   const float2* bf;     // Bluestein: FFT_m(b)/m for the forward direction

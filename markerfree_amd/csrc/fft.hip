@@ -110,6 +110,10 @@ const FftDevPlan& FftCache::plan(int n) {
   for (int k = 0; k <= n; k++)
     half[k] = f2(std::polar(1.0, -M_PI * (double)k / n));
   size_t o_tw = push(tw), o_half = push(half), o_chirp = 0, o_bf = 0, o_bi = 0;
+  // Radices, two ints per table entry.
+  std::vector<float2> rv((P.nrad + 1) / 2 + 1);
+  std::memcpy(rv.data(), P.rad, P.nrad * sizeof(int));
+  const size_t o_rad = push(rv);
   if (P.blue) {
     const int m = P.m;
     std::vector<cd> chirp(n);
@@ -141,6 +145,7 @@ const FftDevPlan& FftCache::plan(int n) {
   float2* base = (float2*)e->mem;
   P.tw = base + o_tw;
   P.half = base + o_half;
+  P.drad = reinterpret_cast<const int*>(base + o_rad);
   if (P.blue) {
     P.chirp = base + o_chirp;
     P.bf = base + o_bf;

@@ -165,6 +165,7 @@ __device__ __forceinline__ void fft_butterfly(int R, const float2* src, int ss,
       for (int pp = 0; pp < R; pp++) {
         float2 acc = src[0];
         int k = 0;
+#pragma unroll 4
         for (int q = 1; q < R; q++) {
           k += pp;
           if (k >= R) k -= R;
@@ -178,26 +179,48 @@ __device__ __forceinline__ void fft_butterfly(int R, const float2* src, int ss,
   }
 }
 
+// One Stockham stage of radix R over nl lines: butterfly j = (q, k) reads
+// x[j + p*N/R] and writes y[q*Ns*R + k + p*Ns].  R > 0 is fixed at compile
+// time (its butterfly unrolled, no per-butterfly radix dispatch); R = 0 takes
+// the radix from 'rr' at run time.
+template <int R>
+__device__ __forceinline__ void stockham_stage(const float2* x, float2* y,
+                                               int N, int ls, int Ns, int rr,
+                                               const float2* __restrict__ tw,
+                                               int nl, bool inv) {
+  const int r = R ? R : rr;
+  const int NR = N / r;
+  const int tstride = N / (Ns * r);
+  const float rNs = 1.0f / (float)Ns;
+  for (LineIter it(NR); it.l < nl; it.next()) {
+    const int j = it.k;
+    const int q = qdiv(j, rNs), k = j - q * Ns;
+    fft_butterfly(r, x + it.l * ls + j, NR, y + it.l * ls + q * Ns * r + k,
+                  Ns, k * tstride, tw, N, inv);
+  }
+}
+
 // Direct Stockham transform of nl lines of length N (line stride ls) held in
 // x; y is scratch of the same shape.  Returns the buffer holding the result.
 // Caller must __syncthreads() before (data loaded) -- stages sync themselves.
 __device__ __forceinline__ float2* fft_stockham(float2* x, float2* y, int N,
                                                 int ls, int nrad,
-                                                const int* rad,
+                                                const int* __restrict__ rad,
                                                 const float2* __restrict__ tw,
                                                 int nl, bool inv) {
   int Ns = 1;
   for (int s = 0; s < nrad; s++) {
     const int R = rad[s];
-    const int NR = N / R;
-    const int tstride = N / (Ns * R);
-    const float rNs = 1.0f / (float)Ns;
-    for (LineIter it(NR); it.l < nl; it.next()) {
-      const int j = it.k;
-      const int q = qdiv(j, rNs), k = j - q * Ns;
-      const float2* src = x + it.l * ls + j;
-      float2* dst = y + it.l * ls + q * Ns * R + k;
-      fft_butterfly(R, src, NR, dst, Ns, k * tstride, tw, N, inv);
+    switch (R) {
+      case 2: stockham_stage<2>(x, y, N, ls, Ns, R, tw, nl, inv); break;
+      case 3: stockham_stage<3>(x, y, N, ls, Ns, R, tw, nl, inv); break;
+      case 4: stockham_stage<4>(x, y, N, ls, Ns, R, tw, nl, inv); break;
+      case 5: stockham_stage<5>(x, y, N, ls, Ns, R, tw, nl, inv); break;
+      case 7: stockham_stage<7>(x, y, N, ls, Ns, R, tw, nl, inv); break;
+      case 8: stockham_stage<8>(x, y, N, ls, Ns, R, tw, nl, inv); break;
+      case 11: stockham_stage<11>(x, y, N, ls, Ns, R, tw, nl, inv); break;
+      case 13: stockham_stage<13>(x, y, N, ls, Ns, R, tw, nl, inv); break;
+      default: stockham_stage<0>(x, y, N, ls, Ns, R, tw, nl, inv); break;
     }
     __syncthreads();
     float2* t = x;
@@ -361,7 +384,7 @@ __device__ __forceinline__ float2* fft_lines(float2* buf, float2* tmp,
         return inv ? fft_pow2<4096, true>(buf, tmp, nl, tw)
                    : fft_pow2<4096, false>(buf, tmp, nl, tw);
       default:
-        return fft_stockham(buf, tmp, P.n, P.n, P.nrad, P.rad, tw, nl, inv);
+        return fft_stockham(buf, tmp, P.n, P.n, P.nrad, P.drad, tw, nl, inv);
     }
   }
   const int n = P.n, m = P.m;
@@ -375,13 +398,13 @@ __device__ __forceinline__ float2* fft_lines(float2* buf, float2* tmp,
     buf[l * m + k] = v;
   }
   __syncthreads();
-  float2* r = fft_stockham(buf, tmp, m, m, P.nrad, P.rad, tw, nl, false);
+  float2* r = fft_stockham(buf, tmp, m, m, P.nrad, P.drad, tw, nl, false);
   float2* o = (r == buf) ? tmp : buf;
   const float2* f = inv ? P.bi : P.bf;
   for (LineIter it(m); it.l < nl; it.next())
     r[it.l * m + it.k] = cmul(r[it.l * m + it.k], f[it.k]);
   __syncthreads();
-  r = fft_stockham(r, o, m, m, P.nrad, P.rad, tw, nl, true);
+  r = fft_stockham(r, o, m, m, P.nrad, P.drad, tw, nl, true);
   for (LineIter it(m); it.l < nl; it.next()) {
     const int l = it.l, k = it.k;
     if (k < n) {

@@ -136,8 +136,9 @@ struct CorrArgs {
 // pixels is the nearest raw pixel at R(axis) p + shift + centre (mGCorrect),
 // random filled or invalid outside; the bin sum is invalid if any is
 // (mGBinImage).
-__device__ float corr_pixel(const CorrArgs& a, const CorrProj& pr,
-                            const float* raw, int x, int y) {
+__device__ __forceinline__ float corr_pixel(const CorrArgs& a,
+                                            const CorrProj& pr,
+                                            const float* raw, int x, int y) {
   const float hx = (float)a.cx * 0.5f, hy = (float)a.ny * 0.5f;
   const float ox = pr.shift_x + (float)a.nx * 0.5f;
   const float oy = pr.shift_y + (float)a.ny * 0.5f;
