@@ -20,6 +20,7 @@
 // the per-projection cos/sin/weight tables sit in scalar registers; larger
 // counts use the runtime-n instantiation (N = 0).
 #include <algorithm>
+#include <map>
 #include <type_traits>
 #include <cmath>
 #include <vector>
@@ -1174,19 +1175,35 @@ struct Kernels {
   FusedKernel rows2[kTabMaxN + 1];
   FusedKernel free2[kTabMaxN + 1];
   FusedKernel free4[kTabMaxN + 1];
+  // Static LDS of every kernel above and of k_bp_slab, read once (block
+  // reductions such as __syncthreads_or use some).
+  std::map<const void*, size_t> static_lds;
   Kernels() {
     KernelTable<kMaxFixedN>::fill(fu);
     TabKernelTable<kTabMaxN>::fill(tab, rows2, free2, free4);
+    auto note = [&](const void* k) {
+      if (!k) return;
+      hipFuncAttributes a;
+      MF_HIP(hipFuncGetAttributes(&a, k));
+      static_lds[k] = a.sharedSizeBytes;
+    };
+    for (FusedKernel k : fu) note((const void*)k);
+    for (FusedKernel* t : {tab, rows2, free2, free4})
+      for (int n = 0; n <= kTabMaxN; n++) note((const void*)t[n]);
+    note((const void*)k_bp_slab);
   }
 };
 
+const Kernels& kernels();
+
 // Whether 'dyn' bytes of dynamic LDS fit in a workgroup's 160 KB next to
-// kernel k's static LDS (block reductions such as __syncthreads_or use some).
+// kernel k's static LDS.
 bool lds_fits(const void* k, size_t dyn) {
   if (!k || dyn > 160 * 1024) return false;
-  hipFuncAttributes a;
-  MF_HIP(hipFuncGetAttributes(&a, k));
-  return dyn + a.sharedSizeBytes <= 160 * 1024;
+  const auto& m = kernels().static_lds;
+  const auto it = m.find(k);
+  MF_REQUIRE(it != m.end(), "kernel without an LDS record");
+  return dyn + it->second <= 160 * 1024;
 }
 
 const Kernels& kernels() {
