@@ -142,41 +142,52 @@ __device__ __forceinline__ float corr_pixel(const CorrArgs& a,
   const float hx = (float)a.cx * 0.5f, hy = (float)a.ny * 0.5f;
   const float ox = pr.shift_x + (float)a.nx * 0.5f;
   const float oy = pr.shift_y + (float)a.ny * 0.5f;
-  // The bin's corrected pixels are summed in the reference's order; the
-  // in-image reads of up to four of them are issued before any is used, so
-  // they are in flight together (the reference stops at the first invalid
-  // pixel, after which its sum is discarded: summing on changes nothing).
-  constexpr int kB = 4;
+  // The bin's binx*biny corrected pixels are summed in the reference's
+  // (row-major) order.  They are visited in batches of kB whose in-image
+  // reads are all issued before any is used, so they are in flight together;
+  // slots past the bin are skipped by a uniform branch and read nothing.
+  // (The reference stops at the first invalid pixel, after which its sum is
+  // discarded: summing on changes nothing.)
+  constexpr int kB = 8;
+  const int nb = a.binx * a.biny;
   float sum = 0.f;
   bool bad = false;
-  for (int iy = 0; iy < a.biny; iy++) {
-    const float py = (float)(y * a.biny + iy) + 0.5f - hy;
-    for (int ix0 = 0; ix0 < a.binx; ix0 += kB) {
-      int rx[kB], ry[kB];
-      bool in[kB];
-      float v[kB];
+  int ix = 0, iy = 0;  // bin position of the batch's first pixel
+  for (int b0 = 0; b0 < nb; b0 += kB) {
+    int rx[kB], ry[kB];
+    bool in[kB];
+    float v[kB];
+    int jx = ix, jy = iy;
 #pragma unroll
-      for (int j = 0; j < kB; j++) {
-        const float px = (float)(x * a.binx + ix0 + j) + 0.5f - hx;
+    for (int j = 0; j < kB; j++) {
+      if (b0 + j < nb) {
+        const float px = (float)(x * a.binx + jx) + 0.5f - hx;
+        const float py = (float)(y * a.biny + jy) + 0.5f - hy;
         float qx = px * pr.cos_a - py * pr.sin_a;
         float qy = px * pr.sin_a + py * pr.cos_a;
         qx += ox;
         qy += oy;
         rx[j] = (int)(qx + 0.5f);
         ry[j] = (int)(qy + 0.5f);
-        in[j] = ix0 + j < a.binx && rx[j] >= 0 && rx[j] < a.nx && ry[j] >= 0 &&
-                ry[j] < a.ny;
-        v[j] = raw_pix(raw, a.nx, in[j] ? rx[j] : 0, in[j] ? ry[j] : 0);
+        in[j] = rx[j] >= 0 && rx[j] < a.nx && ry[j] >= 0 && ry[j] < a.ny;
+        v[j] = in[j] ? raw_pix(raw, a.nx, rx[j], ry[j]) : 0.f;
+        if (++jx == a.binx) {
+          jx = 0;
+          jy++;
+        }
       }
+    }
 #pragma unroll
-      for (int j = 0; j < kB; j++) {
-        if (ix0 + j >= a.binx) break;
+    for (int j = 0; j < kB; j++) {
+      if (b0 + j < nb) {
         if (!in[j])
           v[j] = a.fill ? corr_random(rx[j], ry[j], a.nx, a.ny, raw) : kInvalid;
         bad = bad || v[j] < -1e10f;
         sum += v[j];
       }
     }
+    ix = jx;
+    iy = jy;
   }
   return bad ? kInvalid : sum;
 }
