@@ -72,7 +72,11 @@ mf_status mf_d2d(mf_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
  * "fft_cols", "correct", "rweight", "correct_rweight", "radon", "cc" (the
  * launches of one correlation).  mf_timing_select restricts timing to the
  * launches of one name (NULL or "": all), so that timing one kernel adds no
- * events around the others.  No counterpart in the reference. */
+ * events around the others; mf_timing_every(ctx, k) times only every k-th
+ * launch of each name (each timed launch costs two stream markers, a few
+ * microseconds of idle device), and mf_kernel_count returns all launches of
+ * a name since the last reset, timed or not.  No counterpart in the
+ * reference. */
 mf_status mf_timing_enable(mf_ctx* ctx, int enable);
 /* Launch-shape tunables (workgroup size, lines per workgroup) keyed by kernel
  * family and shape, e.g. "cc_rows/1024x1024/threads"; a key that is not set
@@ -88,6 +92,8 @@ mf_status mf_tune_get(mf_ctx* ctx, const char* key, int* value);
  * object. */
 mf_status mf_tune_load(mf_ctx* ctx, const char* path, int* count);
 mf_status mf_timing_select(mf_ctx* ctx, const char* name);
+mf_status mf_timing_every(mf_ctx* ctx, int k);
+mf_status mf_kernel_count(mf_ctx* ctx, const char* name, long long* launches);
 mf_status mf_timing_reset(mf_ctx* ctx);
 mf_status mf_kernel_time(mf_ctx* ctx, const char* name, double* total_ms,
                          long long* launches);

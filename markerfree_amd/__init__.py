@@ -61,6 +61,8 @@ _sig("mf_d2d", C.c_int, _vp, _vp, _vp, C.c_size_t)
 _sig("mf_timing_enable", C.c_int, _vp, C.c_int)
 _sig("mf_timing_reset", C.c_int, _vp)
 _sig("mf_timing_select", C.c_int, _vp, C.c_char_p)
+_sig("mf_timing_every", C.c_int, _vp, C.c_int)
+_sig("mf_kernel_count", C.c_int, _vp, C.c_char_p, C.POINTER(C.c_longlong))
 _sig("mf_host_alloc", C.c_int, C.c_size_t, C.POINTER(_vp))
 _sig("mf_host_free", C.c_int, _vp)
 _sig("mf_tune_set", C.c_int, _vp, C.c_char_p, C.c_int)
@@ -241,11 +243,19 @@ class Context:
         return pinned_empty(shape, dtype)
 
     # ---- timing
-    def timing(self, on: bool = True, only: str | None = None):
-        """Kernel timing on or off; 'only' restricts it to one launch name."""
+    def timing(self, on: bool = True, only: str | None = None, every: int = 1):
+        """Kernel timing on or off; 'only' restricts it to one launch name,
+        'every' times only every k-th launch of a name."""
         self._check(_lib.mf_timing_select(
             self.h, only.encode() if only else None))
+        self._check(_lib.mf_timing_every(self.h, every))
         self._check(_lib.mf_timing_enable(self.h, int(on)))
+
+    def kernel_count(self, name: str) -> int:
+        """Launches of 'name' since timing_reset(), timed or not."""
+        n = C.c_longlong()
+        self._check(_lib.mf_kernel_count(self.h, name.encode(), C.byref(n)))
+        return n.value
 
     def timing_reset(self):
         self._check(_lib.mf_timing_reset(self.h))

@@ -256,6 +256,19 @@ mf_status mf_timing_select(mf_ctx* ctx, const char* name) {
   return guard(ctx, [&] { ctx->timer.select(name ? name : ""); });
 }
 
+mf_status mf_timing_every(mf_ctx* ctx, int k) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(k >= 1, "timing stride must be at least 1");
+    ctx->timer.every(k);
+  });
+}
+
+mf_status mf_kernel_count(mf_ctx* ctx, const char* name, long long* launches) {
+  if (!ctx || !name || !launches) return MF_EINVAL;
+  return guard(ctx, [&] { *launches = ctx->timer.count(name); });
+}
+
 mf_status mf_timing_reset(mf_ctx* ctx) {
   if (!ctx) return MF_EINVAL;
   return guard(ctx, [&] {

@@ -536,10 +536,7 @@ AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   const int nl = std::max(1, std::min(16, (int)((64 * 1024) / per)));
   const size_t lds = nl * per + (size_t)ls * sizeof(float2);
   MF_REQUIRE(lds + 512 <= 160 * 1024, "common line exceeds LDS");
-  if (lds > 60 * 1024)
-    MF_HIP(hipFuncSetAttribute((const void*)k_line_fft,
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
+  ensure_dyn_lds(ctx, (const void*)k_line_fft, lds);
   const int nlines = nz * num;
   hipLaunchKernelGGL(k_line_fft, dim3(cdiv(nlines, nl)), dim3(kLineThreads),
                      lds, st, lines, nlines, L, pitch, d_taper, P, nl);

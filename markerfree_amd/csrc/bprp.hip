@@ -1344,14 +1344,9 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
       units = (long long)cdiv(by, rows) * nch;
     }
     MF_REQUIRE(lds_fits((const void*)k, flds), "sinogram rows exceed LDS");
-    if (flds > 64 * 1024)
-      MF_HIP(hipFuncSetAttribute((const void*)k,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)flds));
+    ensure_dyn_lds(ctx, (const void*)k, flds);
     // One resident wave of workgroups over the units.
-    int occ = 0;
-    MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k,
-                                                        threads, flds));
+    const int occ = occupancy(ctx, (const void*)k, threads, flds);
     const long long slots = (long long)std::max(occ, 1) * std::max(ctx->num_cu, 1);
     const int per = cdiv(units, std::min(units, slots));
     TimedLaunch tl(ctx->timer, "bprp_fused", st);
@@ -1371,14 +1366,10 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   } else {
     const void* k = (const void*)k_bp_slab;
     MF_REQUIRE(lds_fits(k, lds), "sinogram rows exceed LDS");
-    if (lds > 64 * 1024)
-      MF_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds));
+    ensure_dyn_lds(ctx, k, lds);
     // One resident wave of workgroups, each owning an equal share of the
     // by*tb slab rows.
-    int occ = 0;
-    MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, kBpThreads,
-                                                        lds));
+    const int occ = occupancy(ctx, k, kBpThreads, lds);
     const long long rows = (long long)by * tb;
     const long long slots = (long long)std::max(occ, 1) * std::max(ctx->num_cu, 1);
     const int per = (int)cdiv(rows, std::min(rows, slots));

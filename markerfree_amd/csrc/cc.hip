@@ -467,11 +467,9 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_fit(const CcImgs imgs,
   out[2 * blockIdx.x + 1] = fpy - (float)(h / 2);
 }
 
-void set_lds(const void* k, size_t bytes) {
+void set_lds(mf_ctx* ctx, const void* k, size_t bytes) {
   MF_REQUIRE(bytes <= 160 * 1024, "correlation lines exceed LDS");
-  if (bytes > 64 * 1024)
-    MF_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes));
+  ensure_dyn_lds(ctx, k, bytes);
 }
 
 // Launch shape of one line-transform kernel: lines per workgroup and
@@ -536,7 +534,7 @@ void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
     const Shape sh = cc_shape(ctx, "cc_rowsf", w, h, npair, 2 * lr * 8,
                               lr * 8, (int)lr);
     const size_t lds = (2 * sh.nl * lr + lr) * sizeof(float2);
-    set_lds((const void*)k_cc_rows_fwd<TAB>, lds);
+    set_lds(ctx, (const void*)k_cc_rows_fwd<TAB>, lds);
     hipLaunchKernelGGL(k_cc_rows_fwd<TAB>, dim3(cdiv(h, sh.nl), nimg),
                        dim3(sh.threads), lds, st, imgs, w, h,
                        (const double*)part, mask, unit_std, PR, sh.nl);
@@ -550,7 +548,7 @@ void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
     const Shape sh = cc_shape(ctx, "cc_cols", w, h, npair, 4 * lc * 8, lc * 8,
                               2 * (int)lc);
     const size_t lds = (4 * sh.nl * lc + lc) * sizeof(float2);
-    set_lds((const void*)k_cc_cols<TAB>, lds);
+    set_lds(ctx, (const void*)k_cc_cols<TAB>, lds);
     const int ngroups = cdiv(nc, sh.nl);
     hipLaunchKernelGGL(k_cc_cols<TAB>, dim3(ngroups * npair), dim3(sh.threads),
                        lds, st, imgs, w, h, ngroups, PC, sh.nl, scale, bfactor);
@@ -560,7 +558,7 @@ void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
   const Shape sh = cc_shape(ctx, "cc_rowsi", w, h, npair, (2 * lr + 1) * 8,
                             lr * 8, (int)lr);
   const size_t lds = ((2 * lr + 1) * sh.nl + lr) * sizeof(float2);
-  set_lds((const void*)k_cc_rows_inv<TAB>, lds);
+  set_lds(ctx, (const void*)k_cc_rows_inv<TAB>, lds);
   const int nparts = cdiv(h, sh.nl);
   hipLaunchKernelGGL(k_cc_rows_inv<TAB>, dim3(nparts, npair), dim3(sh.threads),
                      lds, st, imgs, w, h, PR, sh.nl, mode, peaks);

@@ -26,6 +26,7 @@ hipEvent_t* KernelTimer::begin(const char* name, hipStream_t s) {
   Rec* r = nullptr;
   {
     std::lock_guard<std::mutex> g(mu_);
+    if (seen_[name]++ % every_ != 0) return nullptr;
     if (!free_.empty()) {
       r = free_.back();
       free_.pop_back();
@@ -65,6 +66,13 @@ void KernelTimer::reset() {
   std::lock_guard<std::mutex> g(mu_);
   drain();
   done_.clear();
+  seen_.clear();
+}
+
+long long KernelTimer::count(const std::string& name) {
+  std::lock_guard<std::mutex> g(mu_);
+  const auto it = seen_.find(name);
+  return it == seen_.end() ? 0 : it->second;
 }
 
 void KernelTimer::query(const std::string& name, double* total_ms,
@@ -120,6 +128,27 @@ int tune_get(mf_ctx* ctx, const std::string& key, int dflt) {
   std::lock_guard<std::mutex> g(ctx->tune_mu);
   auto it = ctx->tune.find(key);
   return it == ctx->tune.end() ? dflt : it->second;
+}
+
+void ensure_dyn_lds(mf_ctx* ctx, const void* k, size_t bytes) {
+  if (bytes <= 64 * 1024) return;
+  std::lock_guard<std::mutex> g(ctx->launch_mu);
+  size_t& have = ctx->dyn_lds[k];
+  if (have >= bytes) return;
+  MF_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes));
+  have = bytes;
+}
+
+int occupancy(mf_ctx* ctx, const void* k, int threads, size_t lds) {
+  std::lock_guard<std::mutex> g(ctx->launch_mu);
+  const auto key = std::make_tuple(k, threads, lds);
+  auto it = ctx->occ.find(key);
+  if (it != ctx->occ.end()) return it->second;
+  int occ = 0;
+  MF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, threads, lds));
+  ctx->occ[key] = occ;
+  return occ;
 }
 
 DevBuf::~DevBuf() { release(); }

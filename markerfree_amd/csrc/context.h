@@ -2,6 +2,7 @@
 #pragma once
 #include <functional>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -22,12 +23,17 @@ class KernelTimer {
   void enable(bool on) { enabled_ = on; }
   // Times only launches named 'name' (empty: all).
   void select(const std::string& name) { only_ = name; }
+  // Times every k-th launch of each selected name (k >= 1); the others are
+  // counted only, so that a sampled timing adds fewer stream markers.
+  void every(int k) { every_ = k < 1 ? 1 : k; }
   bool enabled() const { return enabled_; }
   // Returns an event pair to record around one launch, or null when off.
   hipEvent_t* begin(const char* name, hipStream_t s);
   void end(hipEvent_t* ev, hipStream_t s);
   void reset();
   void query(const std::string& name, double* total_ms, long long* n);
+  // Selected launches of 'name' since the last reset, timed or not.
+  long long count(const std::string& name);
 
  private:
   struct Rec {
@@ -35,7 +41,9 @@ class KernelTimer {
     hipEvent_t ev[2];
   };
   bool enabled_ = false;
+  int every_ = 1;
   std::string only_;
+  std::map<std::string, long long> seen_;
   std::vector<Rec*> pending_;
   std::vector<Rec*> free_;  // event pairs reused across launches
   std::map<std::string, std::pair<double, long long>> done_;
@@ -84,6 +92,15 @@ const void* ctx_table(mf_ctx* ctx, const std::string& key,
 // through mf_tune_set, else 'dflt' (the built-in heuristic).
 int tune_get(mf_ctx* ctx, const std::string& key, int dflt);
 
+// Raises kernel k's dynamic-LDS limit on the context's device to at least
+// 'bytes' (hipFuncSetAttribute once per kernel and size; kernels launched
+// with more than 64 KB of dynamic LDS need it).
+void ensure_dyn_lds(mf_ctx* ctx, const void* k, size_t bytes);
+
+// Resident workgroups per CU of kernel k for a launch shape, queried once
+// per (kernel, threads, dynamic LDS) and remembered.
+int occupancy(mf_ctx* ctx, const void* k, int threads, size_t lds);
+
 }  // namespace mf
 
 struct mf_ctx {
@@ -99,6 +116,10 @@ struct mf_ctx {
   std::unique_ptr<mf::FftCache> fft;
   std::map<std::string, std::unique_ptr<mf::DevBuf>> tables;
   std::map<std::string, int> tune;      // launch-shape tunables
-  std::mutex mu, table_mu, tune_mu;
+  // Per-kernel launch facts (ensure_dyn_lds, occupancy): host calls that
+  // are not repeated per launch.
+  std::map<const void*, size_t> dyn_lds;
+  std::map<std::tuple<const void*, int, size_t>, int> occ;
+  std::mutex mu, table_mu, tune_mu, launch_mu;
   ~mf_ctx();
 };

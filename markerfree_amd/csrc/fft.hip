@@ -273,10 +273,8 @@ __global__ __launch_bounds__(kFftThreads) void k_c2c_cols(
   }
 }
 
-void set_lds(const void* k, size_t bytes) {
-  if (bytes > 64 * 1024)
-    MF_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes));
+void set_lds(mf_ctx* ctx, const void* k, size_t bytes) {
+  ensure_dyn_lds(ctx, k, bytes);
 }
 
 void launch_rows(mf_ctx* ctx, hipStream_t st, float* buf, int n, int nrows,
@@ -291,11 +289,11 @@ void launch_rows(mf_ctx* ctx, hipStream_t st, float* buf, int n, int nrows,
   dim3 grid(cdiv(nrows, nl));
   TimedLaunch tl(ctx->timer, "fft_rows", st);
   if (r2c) {
-    set_lds((const void*)k_r2c_rows, lds);
+    set_lds(ctx, (const void*)k_r2c_rows, lds);
     hipLaunchKernelGGL(k_r2c_rows, grid, dim3(kFftThreads), lds, st, buf, nrows,
                        rows_per_img, img_stride, pitch, P, nl, scale);
   } else {
-    set_lds((const void*)k_c2r_rows, lds);
+    set_lds(ctx, (const void*)k_c2r_rows, lds);
     hipLaunchKernelGGL(k_c2r_rows, grid, dim3(kFftThreads), lds, st, buf, nrows,
                        rows_per_img, img_stride, pitch, P, nl, scale);
   }
@@ -315,7 +313,7 @@ void launch_cols(mf_ctx* ctx, hipStream_t st, float* buf, int nx, int ny,
       ((size_t)2 * nl * fft_lds_len(P) + fft_lds_len(P)) * sizeof(float2);
   MF_REQUIRE(lds <= 160 * 1024, "FFT column exceeds LDS");
   dim3 grid(cdiv(ncols, nl) * nimg);
-  set_lds((const void*)k_c2c_cols, lds);
+  set_lds(ctx, (const void*)k_c2c_cols, lds);
   TimedLaunch tl(ctx->timer, "fft_cols", st);
   hipLaunchKernelGGL(k_c2c_cols, grid, dim3(kFftThreads), lds, st,
                      (float2*)buf, ncols, ncols, img_stride / 2, P, nl,

@@ -429,10 +429,7 @@ void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
   const int nl = std::max(1, std::min(16, (int)((64 * 1024) / per)));
   const size_t lds = nl * per + (size_t)ls * sizeof(float2);
   MF_REQUIRE(lds <= 160 * 1024, "R-weighting row exceeds LDS");
-  if (lds > 64 * 1024)
-    MF_HIP(hipFuncSetAttribute((const void*)k_rweight,
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
+  ensure_dyn_lds(ctx, (const void*)k_rweight, lds);
   const float* filt = rweight_filter(ctx, bx);
   TimedLaunch tl(ctx->timer, "rweight", st);
   hipLaunchKernelGGL(k_rweight, dim3(cdiv(nrows, nl)), dim3(kRowThreads), lds,
@@ -480,10 +477,7 @@ void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
              "workgroup size must be a multiple of 64 up to 256");
   const size_t lds = nl * per + (size_t)ls * sizeof(float2);
   MF_REQUIRE(lds <= 160 * 1024, "R-weighting row exceeds LDS");
-  if (lds > 64 * 1024)
-    MF_HIP(hipFuncSetAttribute((const void*)k_correct_rweight,
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds));
+  ensure_dyn_lds(ctx, (const void*)k_correct_rweight, lds);
   const float* filt = rweight_filter(ctx, g.bx);
   TimedLaunch tl(ctx->timer, "correct_rweight", st);
   hipLaunchKernelGGL(k_correct_rweight, dim3(cdiv(g.by, nl), count),
