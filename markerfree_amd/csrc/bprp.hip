@@ -90,6 +90,25 @@ __device__ __forceinline__ float div_by_rcp(float a, float d, float r) {
   return fmaf(fmaf(-q, d, a), r, q);
 }
 
+// A uniform value copied into a vector register, so that a select between it
+// and zero takes no scalar operand besides its condition mask.
+__device__ __forceinline__ float in_vgpr(float x) {
+  float y;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "s"(x));
+  return y;
+}
+
+// RN(1/d) for d in [2^-10, 2^4] (weight sums): the division sequence of
+// 1.0f / d without its range scaling and special-case fix-up, which are the
+// identity on that range -- a hardware reciprocal estimate refined by one
+// Newton step, then two residual corrections of the quotient.
+__device__ __forceinline__ float rcp_rn(float d) {
+  const float r0 = __builtin_amdgcn_rcpf(d);
+  const float r = fmaf(fmaf(-d, r0, 1.0f), r0, r0);
+  const float q = fmaf(fmaf(-d, r, 1.0f), r, r);
+  return fmaf(fmaf(-d, q, 1.0f), r, q);
+}
+
 // V voxels per lane with per-projection range and validity tests (the
 // reference's general per-voxel loop, CalcReproj.cu:37-63).
 template <int N, int V>
@@ -916,6 +935,8 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
 // records it.  A ray that met one is evaluated again with the per-row tests
 // (which are exact in every case), so the output stays bit-identical.
 size_t free_lds_bytes(int r, int n, int bx) { return (size_t)n * bx * 4 * r; }
+template <int R>
+constexpr int kSlotShift = R == 4 ? 4 : 3;  // log2 of a slot's bytes
 
 template <int R>
 struct FreeSlot;
@@ -941,6 +962,17 @@ struct FreeSlot<2> {
     v[1] = s.y;
   }
 };
+
+// The slot at LDS byte address 'at'.
+template <int R>
+__device__ __forceinline__ typename FreeSlot<R>::T lds_slot(unsigned at) {
+  typedef float Nat __attribute__((ext_vector_type(R)));
+  const Nat n = *(const __attribute__((address_space(3))) Nat*)(size_t)at;
+  float v[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) v[r] = n[r];
+  return FreeSlot<R>::make(v);
+}
 
 // Rows y[0..R) of the n reference projections as slots, encoded like
 // load_sino_rows.
@@ -977,7 +1009,8 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
   const float cx = (float)bx * 0.5f;
   const float hvx = (float)vx * 0.5f, htb = (float)p.tb * 0.5f;
   const float iex = (float)(vx - 1), iez = (float)(q.tb - 1);
-  const unsigned iend_bits = __float_as_uint((float)(bx - 1));
+  const float iend = (float)(bx - 1);
+  const unsigned iend_bits = __float_as_uint(iend);
   const float ox = 0.5f - hvx, oz = 0.5f - htb;
   const bool fast_ok = p.cnt_all >= 0.001f;
 
@@ -1010,15 +1043,30 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
     fast_interval(p, q, N, tx, tz, z0, &fa, &fb);
     fa = max(fa, t0);
     fb = min(fb, t1);
-    const int ta = wave_max(fa), tb = wave_min(fb);
+    // The wave's common fast interval, in scalar registers (the fast loop's
+    // counter is then scalar too).
+    const int ta = __builtin_amdgcn_readfirstlane(wave_max(fa));
+    const int tb = __builtin_amdgcn_readfirstlane(wave_min(fb));
     const bool has_fast = fast_ok && ta < tb;
     const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
     double sum[R];
     int cnt[R];
-    bool seen = false;  // a tested sample's quotient was NaN
-    // Samples that take the reference's tests; with 'bad' each row keeps its
-    // own validity and weight sum, as skipped samples require.
-    auto general = [&](int from, int to, auto bad) {
+    int nuse = 0;  // samples counted by every row (no skipped samples)
+    // Samples that take the reference's tests, on the premise that no sample
+    // is skipped: then the rows share the weight sum, the validity and the
+    // count.  A skipped sample (-inf) makes a counted sample's quotient NaN,
+    // which reaches the row's sum and sends the ray to 'exact'.
+    auto tested = [&](int from, int to) {
+      // Weights and slot-row addresses in vector registers: the scalar file
+      // cannot hold them next to the range masks of this loop.
+      float wv[N];
+      unsigned rowv[N];
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        wv[i] = in_vgpr(p.w[i]);
+        rowv[i] = __float_as_uint(in_vgpr(__uint_as_float(
+            (unsigned)(size_t)(sq + i * bx))));
+      }
       for (int t = from; t < to; t++) {
         const float fz = (float)t + z0;
         const float X = fmaf(-fz, q.s, tx);
@@ -1034,6 +1082,53 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
           // The same unsigned range test as k_bprp_tab.
           const bool ok = __float_as_uint(rr) <= iend_bits;
           okm |= ok ? (1u << i) : 0u;
+          // Out of range: any slot of the row (the clamped one), read with
+          // weight 0.
+          const int ix = (int)__builtin_amdgcn_fmed3f(rr, 0.f, iend);
+          const unsigned at = rowv[i] + ((unsigned)ix << kSlotShift<R>);
+          vals[i] = lds_slot<R>(at);
+        }
+        float acc[R], ws = 0.f;
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r] = 0.f;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          const float wi = ((okm >> i) & 1u) ? wv[i] : 0.f;
+          ws += wi;
+          float v4[R];
+          FreeSlot<R>::split(vals[i], v4);
+#pragma unroll
+          for (int r = 0; r < R; r++) acc[r] = fmaf(v4[r], wi, acc[r]);
+        }
+        const float rc = rcp_rn(ws);
+        const bool use = inb && ws >= 0.001f;
+        // An unused sample adds +0 (its quotient masked to zero bits; the
+        // sums never hold -0), so no branch guards the quotients.
+        const unsigned um = use ? ~0u : 0u;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          sum[r] += (double)__uint_as_float(
+              __float_as_uint(div_by_rcp(acc[r], ws, rc)) & um);
+        nuse += use ? 1 : 0;
+      }
+    };
+    // The reference's tests with per-row validity and weight sums, exact
+    // whatever the skipped samples.
+    auto exact = [&](int from, int to) {
+      for (int t = from; t < to; t++) {
+        const float fz = (float)t + z0;
+        const float X = fmaf(-fz, q.s, tx);
+        const float Z = fmaf(fz, q.c, tz);
+        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const float vfx = (inb ? truncf(X) : 0.f) + ox;
+        const float vfz = (inb ? truncf(Z) : 0.f) + oz;
+        Slot vals[N];
+        unsigned okm = 0;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+          const float rr = fmaf(vfx, p.c[i], fmaf(vfz, p.s[i], cx));
+          const bool ok = __float_as_uint(rr) <= iend_bits;
+          okm |= ok ? (1u << i) : 0u;
           vals[i] = sq[i * bx + (ok ? (int)rr : 0)];
         }
         float acc[R], ws[R];
@@ -1044,44 +1139,27 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
           const bool ok = (okm >> i) & 1u;
           float v4[R];
           FreeSlot<R>::split(vals[i], v4);
-          if (decltype(bad)::value) {
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-              const bool okr = ok && (v4[r] != -INFINITY);
-              acc[r] = fmaf(okr ? v4[r] : 0.f, p.w[i], acc[r]);
-              ws[r] += okr ? p.w[i] : 0.f;
-            }
-          } else {
-            const float wi = ok ? p.w[i] : 0.f;
-            ws[0] += wi;
-#pragma unroll
-            for (int r = 0; r < R; r++) acc[r] = fmaf(v4[r], wi, acc[r]);
+          for (int r = 0; r < R; r++) {
+            const bool okr = ok && (v4[r] != -INFINITY);
+            acc[r] = fmaf(okr ? v4[r] : 0.f, p.w[i], acc[r]);
+            ws[r] += okr ? p.w[i] : 0.f;
           }
         }
-        float rc = 0.f;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          const float w = decltype(bad)::value ? ws[r] : ws[0];
-          if (decltype(bad)::value || r == 0) rc = 1.0f / w;
-          const float v = (w < 0.001f) ? kInvalid : div_by_rcp(acc[r], w, rc);
-          if (!decltype(bad)::value) seen |= v != v;
+          const float v = (ws[r] < 0.001f) ? kInvalid : acc[r] / ws[r];
           const bool ok = inb && (v >= -1e10f);
           sum[r] += ok ? (double)v : 0.0;
           cnt[r] += ok ? 1 : 0;
         }
       }
     };
-    auto reset = [&]() {
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        sum[r] = 0.0;
-        cnt[r] = 0;
-      }
-    };
-    reset();
-    general(t0, a, std::false_type{});
+    for (int r = 0; r < R; r++) sum[r] = 0.0;
+    tested(t0, a);
     if (has_fast) {
-      for (int t = a; t < b; t++) {
+      for (int t = ta; t < tb; t++) {
         const float fz = (float)t + z0;
         const float vfx = truncf(fmaf(-fz, q.s, tx)) + ox;
         const float vfz = truncf(fmaf(fz, q.c, tz)) + oz;
@@ -1105,15 +1183,22 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         for (int r = 0; r < R; r++)
           sum[r] += (double)div_by_rcp(acc[r], p.cnt_all, p.rcp_all);
       }
-#pragma unroll
-      for (int r = 0; r < R; r++) cnt[r] += b - a;
+      nuse += tb - ta;
     }
-    general(b, t1, std::false_type{});
+    tested(b, t1);
+    bool seen = false;
 #pragma unroll
     for (int r = 0; r < R; r++) seen |= sum[r] != sum[r];
     if (seen) {
-      reset();
-      general(t0, t1, std::true_type{});
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        sum[r] = 0.0;
+        cnt[r] = 0;
+      }
+      exact(t0, t1);
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) cnt[r] = nuse;
     }
     if (live) {
       const float lim = 0.8f * (float)q.L;
