@@ -920,20 +920,20 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
 // rows per unit at 1024 columns, or two rows at 1920-2048 columns, the
 // sinogram slots alone fill LDS (one float4 or float2 {row 0..R-1} per
 // projection and position, no guard entry), so nothing else is tabulated:
-// the z half of each lookup
-// coordinate is evaluated per sample with the z table's own fma, an
-// out-of-range projection reads position 0 and contributes with weight 0
-// (adding a zero product leaves the sum unchanged), and a tested sample
-// accumulates its weight sum in reference order and divides through the
-// sum's correctly rounded reciprocal.  The result equals k_bprp_tab bit for
-// bit.  At 1024 image rows, four rows per unit is also one unit per CU.
+// the z half of each lookup coordinate is evaluated per sample with the z
+// table's own fma, and an out-of-range projection reads the clamped slot and
+// contributes with weight 0 (adding a zero product leaves the sum
+// unchanged).  A tested sample shares its weight sum (in reference order),
+// the sum's correctly rounded reciprocal and its count across the rows.  The
+// result equals k_bprp_tab bit for bit.  At 1024 image rows, four rows per
+// unit is also one unit per CU.
 //
 // The kernel holds no LDS beyond the slots, so there is no workgroup-wide
 // flag for skipped samples (stored as -inf); each ray detects them itself.
-// A lookup of -inf makes its sample's quotient NaN, whatever the weights:
-// in the test-free samples the NaN reaches the ray sum, and a tested sample
-// records it.  A ray that met one is evaluated again with the per-row tests
-// (which are exact in every case), so the output stays bit-identical.
+// A lookup of -inf makes its sample's quotient NaN, whatever the weights,
+// and every counted sample's quotient reaches its row's ray sum.  A ray with
+// a NaN sum is evaluated again with per-row validity and weight sums (exact
+// in every case), so the output stays bit-identical.
 size_t free_lds_bytes(int r, int n, int bx) { return (size_t)n * bx * 4 * r; }
 template <int R>
 constexpr int kSlotShift = R == 4 ? 4 : 3;  // log2 of a slot's bytes
