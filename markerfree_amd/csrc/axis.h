@@ -27,6 +27,10 @@ struct AxisWork {
   // it was made from (reset this when that stack's pixels change).
   DevBuf tstack;
   const void* t_of = nullptr;
+  // Per image, kNegParts flags: whether that part of the image holds a
+  // negative value (a sample GRadon skips); made once per stack like tstack.
+  DevBuf negf;
+  const void* neg_of = nullptr;
 };
 
 struct OffsetWork {

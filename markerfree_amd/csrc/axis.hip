@@ -118,12 +118,46 @@ __device__ __forceinline__ float radon_pix(const float* img, int nx, int ny,
       reinterpret_cast<const char*>(img) + (e << 2));
 }
 
+// The same pixel with its element index clamped to the image, for lanes
+// whose coordinates may lie outside it (their value is discarded).
 template <bool TRANS>
+__device__ __forceinline__ float radon_pix_clamped(const float* img, int nx,
+                                                   int ny, int ix, int iy,
+                                                   unsigned last) {
+  const unsigned e = TRANS ? __umul24((unsigned)ix, (unsigned)ny) + (unsigned)iy
+                           : __umul24((unsigned)iy, (unsigned)nx) + (unsigned)ix;
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(img) +
+                                         (min(e, last) << 2));
+}
+
+// Steps of misalignment between the lanes above which the interior loop is
+// sheared.
+constexpr int kRadonShearMin = 2;
+
+template <bool TRANS, bool NEG>
 __device__ __forceinline__ void radon_line(const float* img, int nx, int ny,
                                            float c, float s, float fy,
                                            const RadonImg& im, int left,
                                            int right, float* sum_out,
                                            int* cnt_out);
+
+constexpr int kNegParts = 32;
+
+// Flag of part blockIdx.x of image blockIdx.y: whether it holds a value
+// below 0 (GRadon's skipped samples).  Parts are row bands.
+__global__ __launch_bounds__(256) void k_image_negative(const float* stack,
+                                                        int nx, int ny,
+                                                        int* negf) {
+  const size_t plane = (size_t)nx * ny;
+  const int rows = cdiv(ny, kNegParts);
+  const int ya = min(ny, (int)blockIdx.x * rows), yb = min(ny, ya + rows);
+  const float* p = stack + blockIdx.y * plane + (size_t)ya * nx;
+  const size_t n = (size_t)(yb - ya) * nx;
+  int any = 0;
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) any |= p[i] < 0.f;
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) negf[blockIdx.y * kNegParts + blockIdx.x] = any;
+}
 
 __global__ __launch_bounds__(kRadonLines) void k_radon(const float* stack,
                                                const float* tstack, int nx,
@@ -131,15 +165,17 @@ __global__ __launch_bounds__(kRadonLines) void k_radon(const float* stack,
                                                const float* cs, const int* reg,
                                                const RadonImg* imgs,
                                                float* lines, int pitch,
-                                               int remap) {
+                                               const int* negf, int remap) {
   const int yblocks = cdiv(L, kRadonLines);
   const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const int z = blk / (yblocks * num);
   const int rem = blk - z * yblocks * num;
   const int yb = rem / num;
   const int k = rem - yb * num;
-  const int y = yb * blockDim.x + threadIdx.x;
-  if (y >= L) return;
+  // Lanes past the last line shadow it (the interior loop takes wave-wide
+  // reductions) and store nothing.
+  const int ylive = yb * blockDim.x + threadIdx.x;
+  const int y = min(ylive, L - 1);
   const RadonImg im = imgs[z];
   const float c = cs[2 * k], s = cs[2 * k + 1];
   const bool trans = tstack && fabsf(c) > fabsf(s);
@@ -148,18 +184,31 @@ __global__ __launch_bounds__(kRadonLines) void k_radon(const float* stack,
   const int* r = reg + 2 * ((size_t)k * L + y);
   const int left = (int)((float)r[0] * im.ct + 0.5f);
   const int right = (int)((float)r[1] * im.ct + 0.5f);
+  int neg = 0;  // uniform over the workgroup (one image)
+#pragma unroll
+  for (int j = 0; j < kNegParts; j++) neg |= negf[z * kNegParts + j];
   float sum;
   int cnt;
-  if (trans)
-    radon_line<true>(img, nx, ny, c, s, fy, im, left, right, &sum, &cnt);
-  else
-    radon_line<false>(img, nx, ny, c, s, fy, im, left, right, &sum, &cnt);
-  lines[((size_t)z * num + k) * pitch + y] = cnt == 0 ? kInvalid : sum / (float)cnt;
+  if (trans) {
+    if (neg)
+      radon_line<true, true>(img, nx, ny, c, s, fy, im, left, right, &sum, &cnt);
+    else
+      radon_line<true, false>(img, nx, ny, c, s, fy, im, left, right, &sum, &cnt);
+  } else {
+    if (neg)
+      radon_line<false, true>(img, nx, ny, c, s, fy, im, left, right, &sum, &cnt);
+    else
+      radon_line<false, false>(img, nx, ny, c, s, fy, im, left, right, &sum, &cnt);
+  }
+  if (ylive < L)
+    lines[((size_t)z * num + k) * pitch + y] = cnt == 0 ? kInvalid : sum / (float)cnt;
 }
 
 // The sequential sum of one line over x in [left, right) (left <= 0 <=
-// right, as the reference's two loops [left, 0) and [0, right) run).
-template <bool TRANS>
+// right, as the reference's two loops [left, 0) and [0, right) run).  NEG:
+// the image may hold negative values, which the sum skips; without them
+// the in-image samples need no test at all.
+template <bool TRANS, bool NEG>
 __device__ __forceinline__ void radon_line(const float* img, int nx, int ny,
                                            float c, float s, float fy,
                                            const RadonImg& im, int left,
@@ -209,14 +258,74 @@ __device__ __forceinline__ void radon_line(const float* img, int nx, int ny,
     }
   };
   tested(left, xa);
+  // The wave's lanes trace parallel lines, but their in-image intervals
+  // start at different points along them, so at equal steps they can sit
+  // many rows (columns in the transposed copy) apart, one cache line per
+  // lane.  If so, each lane enters the loop o steps late, o chosen so that
+  // all lanes step through one row together; a lane outside its own
+  // interval reads a clamped address and adds nothing.
+  const float slope = TRANS ? c : s;  // row (column) per step, |.| >= 0.7
+  const bool has = xb > xa;
+  const float fxa = (float)xa;
+  const float ka = TRANS ? fxa * c - fys + im.ox : fxa * s + fyc + im.oy;
+  float kref = has ? ka : (slope > 0.f ? 1e30f : -1e30f);
+  for (int o = 32; o > 0; o >>= 1) {
+    const float other = __shfl_xor(kref, o, 64);
+    kref = slope > 0.f ? fminf(kref, other) : fmaxf(kref, other);
+  }
+  const int off = has ? max(0, (int)floorf((ka - kref) / slope + 0.5f)) : 0;
+  const int n = has ? xb - xa : 0;
+  int omax = off, span = off + n, nmax = n;
+  for (int o = 32; o > 0; o >>= 1) {
+    omax = max(omax, __shfl_xor(omax, o, 64));
+    span = max(span, __shfl_xor(span, o, 64));
+    nmax = max(nmax, __shfl_xor(nmax, o, 64));
+  }
+  // Sheared only while the idle steps stay under a fifth of the loop.
+  if (omax > kRadonShearMin && 4 * span <= 5 * nmax) {
+    const unsigned last = (unsigned)nx * (unsigned)ny - 1u;
+    // x as a float counter: consecutive integers below 2^24 are exact.
+    float fx = (float)(xa - off);
 #pragma unroll 8
-  for (int x = xa; x < xb; x++) {
-    const float X = (float)x * c - fys + im.ox;
-    const float Y = (float)x * s + fyc + im.oy;
-    const float v = radon_pix<TRANS>(img, nx, ny, (int)X, (int)Y);
-    const bool ok = !(v < 0.f);
-    sum += ok ? v : 0.0f;
-    cnt += ok ? 1 : 0;
+    for (int i = 0; i < span; i++, fx += 1.0f) {
+      const bool act = (unsigned)(i - off) < (unsigned)n;
+      const float X = fx * c - fys + im.ox;
+      const float Y = fx * s + fyc + im.oy;
+      const float v = radon_pix_clamped<TRANS>(img, nx, ny, (int)X, (int)Y, last);
+      if (NEG) {
+        const bool ok = act && !(v < 0.f);
+        sum += ok ? v : 0.0f;
+        cnt += ok ? 1 : 0;
+      } else {
+        sum += act ? v : 0.0f;
+      }
+    }
+    if (!NEG) cnt += n;
+    tested(xb, right);
+    *sum_out = sum;
+    *cnt_out = cnt;
+    return;
+  }
+  // x as a float counter: consecutive integers below 2^24 are exact.
+  float fx = (float)xa;
+  if (NEG) {
+#pragma unroll 8
+    for (int x = xa; x < xb; x++, fx += 1.0f) {
+      const float X = fx * c - fys + im.ox;
+      const float Y = fx * s + fyc + im.oy;
+      const float v = radon_pix<TRANS>(img, nx, ny, (int)X, (int)Y);
+      const bool ok = !(v < 0.f);
+      sum += ok ? v : 0.0f;
+      cnt += ok ? 1 : 0;
+    }
+  } else {
+#pragma unroll 8
+    for (int x = xa; x < xb; x++, fx += 1.0f) {
+      const float X = fx * c - fys + im.ox;
+      const float Y = fx * s + fyc + im.oy;
+      sum += radon_pix<TRANS>(img, nx, ny, (int)X, (int)Y);
+    }
+    cnt += xb - xa;
   }
   tested(xb, right);
   *sum_out = sum;
@@ -479,6 +588,13 @@ float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
     }
     tstack = t;
   }
+  int* negf = (int*)ws.negf.ensure(ctx, (size_t)nz * kNegParts * sizeof(int));
+  if (ws.neg_of != d_stack) {
+    hipLaunchKernelGGL(k_image_negative, dim3(kNegParts, nz), dim3(256), 0, st,
+                       d_stack, nx, ny, negf);
+    MF_HIP(hipGetLastError());
+    ws.neg_of = d_stack;
+  }
   {
     TimedLaunch tl(ctx->timer, "radon", st);
     // Neighbouring candidates on one XCD for images well beyond its 4 MB L2
@@ -488,7 +604,7 @@ float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
     hipLaunchKernelGGL(k_radon, dim3(cdiv(L, kRadonLines) * num * nz),
                        dim3(kRadonLines), 0, st,
                        d_stack, tstack, nx, ny, num, L, d_cs, reg, d_ri, lines,
-                       pitch, remap);
+                       pitch, negf, remap);
   }
   MF_HIP(hipGetLastError());
   return lines;

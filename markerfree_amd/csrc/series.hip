@@ -182,6 +182,7 @@ void series_preprocess(mf_series* s) {
   s->rot.assign(nz, 0.0f);
   s->offset = 0.0f;
   s->axw.t_of = nullptr;  // the stack's pixels are new
+  s->axw.neg_of = nullptr;
   s->preprocessed = true;
 }
 

@@ -34,10 +34,14 @@ def test_oracle_lines_skip_negative_pixels(orc):
     assert valid.any() and np.all(lines[valid] == np.float32(1.0))
 
 
-def _stack(nz, ny, nx, seed):
+def _stack(nz, ny, nx, seed, clean=()):
+    """Positive pixels with 5% skipped (negative) ones, except in the images
+    listed in 'clean', which hold none (the GPU sums those without tests)."""
     rng = np.random.default_rng(seed)
     s = (rng.random((nz, ny, nx)) * 3 + 0.1).astype(np.float32)
-    s[rng.random(s.shape) < 0.05] = -1.0  # skipped pixels
+    skip = rng.random(s.shape) < 0.05
+    skip[list(clean)] = False
+    s[skip] = -1.0
     return s
 
 
@@ -49,7 +53,10 @@ def _stack(nz, ny, nx, seed):
 ])
 def test_gpu_common_lines_bitexact(ctx, orc, axis0, arange):
     nz, ny, nx = 5, 150, 200
-    stack = _stack(nz, ny, nx, seed=int(axis0))
+    # Images 1 and 3 hold no skipped pixels; image 4 only one, in its last
+    # row (the last band of the per-image flags).
+    stack = _stack(nz, ny, nx, seed=int(axis0), clean=(1, 3, 4))
+    stack[4, ny - 1, nx // 2] = -1.0
     tilts = np.array([-12, -6, 0, 6, 12], np.float32)
     sx = np.array([3.5, -2.0, 1.25, 0.0, -7.0], np.float32)
     sy = np.array([-1.0, 4.0, -2.5, 6.0, 0.5], np.float32)
