@@ -37,6 +37,12 @@ struct OffsetWork {
   DevBuf buf;
 };
 
+// The stack transposed ([nz][nx][ny]) in ws.tstack, made unless ws already
+// holds it for d_stack.  Asynchronous on 'st'.
+const float* stack_transposed(mf_ctx* ctx, hipStream_t st,
+                              const float* d_stack, int nx, int ny, int nz,
+                              AxisWork& ws);
+
 // GRadon (Rotate.cu:397-428, 83-136): the mean line values of every image
 // for every candidate of 'a', in ws.lines as [nz][num] lines of a.L values at
 // a pitch of a.L + 2 floats (returned).  Asynchronous on 'st'.

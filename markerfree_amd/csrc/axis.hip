@@ -533,6 +533,19 @@ __global__ __launch_bounds__(256) void k_offset_moments(const float* corr,
 
 }  // namespace
 
+const float* stack_transposed(mf_ctx* ctx, hipStream_t st,
+                              const float* d_stack, int nx, int ny, int nz,
+                              AxisWork& ws) {
+  float* t = (float*)ws.tstack.ensure(ctx, (size_t)nx * ny * nz * sizeof(float));
+  if (ws.t_of != d_stack) {
+    hipLaunchKernelGGL(k_transpose, dim3(cdiv(nx, 32), cdiv(ny, 32), nz),
+                       dim3(256), 0, st, d_stack, t, nx, ny);
+    MF_HIP(hipGetLastError());
+    ws.t_of = d_stack;
+  }
+  return t;
+}
+
 float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
                     int ny, int nz, const float* tilts, const double* sx,
                     const double* sy, const AxisSetup& a, AxisWork& ws) {
@@ -577,17 +590,8 @@ float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
   bool colike = false;
   for (int k = 0; k < num; k++)
     colike = colike || std::fabs(a.cs[2 * k]) > std::fabs(a.cs[2 * k + 1]);
-  const float* tstack = nullptr;
-  if (colike) {
-    float* t = (float*)ws.tstack.ensure(ctx, (size_t)nx * ny * nz * sizeof(float));
-    if (ws.t_of != d_stack) {
-      hipLaunchKernelGGL(k_transpose, dim3(cdiv(nx, 32), cdiv(ny, 32), nz),
-                         dim3(256), 0, st, d_stack, t, nx, ny);
-      MF_HIP(hipGetLastError());
-      ws.t_of = d_stack;
-    }
-    tstack = t;
-  }
+  const float* tstack =
+      colike ? stack_transposed(ctx, st, d_stack, nx, ny, nz, ws) : nullptr;
   int* negf = (int*)ws.negf.ensure(ctx, (size_t)nz * kNegParts * sizeof(int));
   if (ws.neg_of != d_stack) {
     hipLaunchKernelGGL(k_image_negative, dim3(kNegParts, nz), dim3(256), 0, st,

@@ -73,6 +73,14 @@ std::vector<CorrProj> corr_params(const mf_series* s, bool shift_only) {
   return p;
 }
 
+// The preprocessed stack transposed (made once per stack, shared with the
+// common-line search): the corrections of projections rotated nearer a
+// quarter turn than none gather their rows from it.
+const float* stack_t(mf_series* s) {
+  return stack_transposed(s->ctx, s->st, (const float*)s->stack.get(), s->nx,
+                          s->ny, s->nz, s->axw);
+}
+
 CorrProj* upload_corr(mf_series* s, const std::vector<CorrProj>& p) {
   CorrProj* d = (CorrProj*)s->cproj.ensure(s->ctx, p.size() * sizeof(CorrProj));
   MF_HIP(hipMemcpyAsync(d, p.data(), p.size() * sizeof(CorrProj),
@@ -207,7 +215,7 @@ float series_coarse_iter(mf_series* s) {
   float* corr = (float*)s->tr_corr.ensure(s->ctx, plane * nz * sizeof(float));
   CorrProj* cp = upload_corr(s, corr_params(s, true));
   correct_bin(s->ctx, s->st, (const float*)s->stack.get(), g, cp, 0, nz, true,
-              corr, g.bx, plane);
+              corr, g.bx, plane, stack_t(s));
   std::vector<PairDesc> pairs;
   std::vector<int> pz;
   for (int z = 0; z < nz; z++) {
@@ -309,7 +317,7 @@ float series_find_offset(mf_series* s) {
   float* corr = (float*)s->off_corr.ensure(s->ctx, plane * s->nz * sizeof(float));
   CorrProj* cp = upload_corr(s, corr_params(s, false));
   correct_bin(s->ctx, s->st, (const float*)s->stack.get(), g, cp, 0, s->nz,
-              false, corr, g.bx, plane);
+              false, corr, g.bx, plane, stack_t(s));
   std::vector<float> ccs;
   const float best = offset_search(s->ctx, s->st, corr, g.bx, g.by, s->nz,
                                    s->tilts.data(), s->offw, &ccs);
@@ -371,7 +379,7 @@ float series_projalign_iter(mf_series* s) {
   // Correct + bin + R-weight the whole stack with the current parameters.
   CorrProj* cp = upload_corr(s, corr_params(s, false));
   correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, 0, nz, true,
-                  corr);
+                  corr, 1, stack_t(s));
   // FitRotCenterZ (ProjAlign.cpp:459-486) with dA starting at 0 (quirk Q13),
   // then RemoveOffsetZ(-1) (:488-501).
   const std::vector<float>& ang = s->pa_angles;
@@ -500,7 +508,8 @@ float series_projalign_iter(mf_series* s) {
     MF_HIP(hipGetLastError());
     // Re-correct the step's targets (one launch for a pair).
     correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, st.a, n,
-                    true, corr + st.a * plane, n == 2 ? st.b - st.a : 1);
+                    true, corr + st.a * plane, n == 2 ? st.b - st.a : 1,
+                    stack_t(s));
     for (int k = 0; k < n; k++) order.push_back(ts[k]);
   }
   std::vector<float> err(nz), meas(2 * nz);
@@ -548,7 +557,7 @@ void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by) {
   float* d = (float*)s->tmp.ensure(s->ctx, plane * s->nz * sizeof(float));
   CorrProj* cp = upload_corr(s, corr_params(s, false));
   correct_bin(s->ctx, s->st, (const float*)s->stack.get(), g, cp, 0, s->nz,
-              true, d, g.bx, plane);
+              true, d, g.bx, plane, stack_t(s));
   MF_HIP(hipMemcpyAsync(out, d, plane * s->nz * sizeof(float),
                         hipMemcpyDeviceToHost, s->st));
   MF_HIP(hipStreamSynchronize(s->st));

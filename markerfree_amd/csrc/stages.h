@@ -46,10 +46,12 @@ void scale_slices(mf_ctx* ctx, hipStream_t st, float* d, size_t plane, int nz,
 // projections starting at index z0 of the raw stack; params from d_proj
 // (device CorrProj[nz]).  Writes binned images with row pitch out_pitch and
 // image stride out_stride (floats).
+// d_raw_t (optional): d_raw transposed ([nz][nx][ny]); projections rotated
+// nearer a quarter turn than none gather from it (same values).
 void correct_bin(mf_ctx* ctx, hipStream_t st, const float* d_raw,
                  const CorrGeom& g, const CorrProj* d_proj, int z0, int count,
                  bool random_fill, float* d_out, int out_pitch,
-                 size_t out_stride);
+                 size_t out_stride, const float* d_raw_t = nullptr);
 // RWeight (CorrStack.cu:532-595) + Unpad: each padded binned row (pitch
 // pad_x(bx)) is R2C'd, ramp-filtered, C2R'd and written unpadded to d_out.
 void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
@@ -60,7 +62,8 @@ void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
 // (d_out holds projection z0 first).
 void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
                      const CorrGeom& g, const CorrProj* d_proj, int z0,
-                     int count, bool random_fill, float* d_out, int zstep = 1);
+                     int count, bool random_fill, float* d_out, int zstep = 1,
+                     const float* d_raw_t = nullptr);
 
 // ------------------------------------------------------- correlation
 // Per-image FP64 moments sum(v) and sum(v^2) over valid pixels of w x h

@@ -124,7 +124,8 @@ __device__ float corr_random(int x, int y, int nx, int ny, const float* raw) {
 }
 
 struct CorrArgs {
-  const float* raw;  // [nz][ny][nx]
+  const float* raw;   // [nz][ny][nx]
+  const float* rawT;  // the same stack transposed, [nz][nx][ny], or null
   const CorrProj* proj;
   float* out;
   int nx, ny, cx, bx, by, binx, biny, z0, fill, out_pitch;
@@ -132,13 +133,24 @@ struct CorrArgs {
   int zstep;  // k_correct_rweight: image z0 + blockIdx.y * zstep
 };
 
+// Whether projection pr reads the transposed copy: its rotation is nearer a
+// quarter turn than none, so an output row runs down the raw image's
+// columns, which are the transposed copy's rows (the same pixels: the
+// values are unchanged).
+__device__ __forceinline__ bool corr_transposed(const CorrArgs& a,
+                                                const CorrProj& pr) {
+  return a.rawT && fabsf(pr.sin_a) > fabsf(pr.cos_a);
+}
+
 // Binned output pixel (x, y) of projection z: each of its bin x bin corrected
 // pixels is the nearest raw pixel at R(axis) p + shift + centre (mGCorrect),
 // random filled or invalid outside; the bin sum is invalid if any is
 // (mGBinImage).
+template <bool TR>
 __device__ __forceinline__ float corr_pixel(const CorrArgs& a,
                                             const CorrProj& pr,
-                                            const float* raw, int x, int y) {
+                                            const float* raw, const float* rt,
+                                            int x, int y) {
   const float hx = (float)a.cx * 0.5f, hy = (float)a.ny * 0.5f;
   const float ox = pr.shift_x + (float)a.nx * 0.5f;
   const float oy = pr.shift_y + (float)a.ny * 0.5f;
@@ -170,7 +182,9 @@ __device__ __forceinline__ float corr_pixel(const CorrArgs& a,
         rx[j] = (int)(qx + 0.5f);
         ry[j] = (int)(qy + 0.5f);
         in[j] = rx[j] >= 0 && rx[j] < a.nx && ry[j] >= 0 && ry[j] < a.ny;
-        v[j] = in[j] ? raw_pix(raw, a.nx, rx[j], ry[j]) : 0.f;
+        v[j] = in[j] ? (TR ? raw_pix(rt, a.ny, ry[j], rx[j])
+                           : raw_pix(raw, a.nx, rx[j], ry[j]))
+                     : 0.f;
         if (++jx == a.binx) {
           jx = 0;
           jy++;
@@ -199,9 +213,13 @@ __global__ __launch_bounds__(256) void k_correct_bin(const CorrArgs a) {
   const int y = idx / a.bx, x = idx - y * a.bx;
   const int z = a.z0 + blockIdx.y;
   const CorrProj pr = a.proj[z];
-  const float* raw = a.raw + (size_t)z * a.nx * a.ny;
-  a.out[(size_t)blockIdx.y * a.out_stride + (size_t)y * a.out_pitch + x] =
-      corr_pixel(a, pr, raw, x, y);
+  const size_t plane = (size_t)a.nx * a.ny;
+  const float* raw = a.raw + z * plane;
+  float* o = a.out + (size_t)blockIdx.y * a.out_stride + (size_t)y * a.out_pitch + x;
+  if (corr_transposed(a, pr))
+    *o = corr_pixel<true>(a, pr, raw, a.rawT + z * plane, x, y);
+  else
+    *o = corr_pixel<false>(a, pr, raw, nullptr, x, y);
 }
 
 constexpr int kRowThreads = 256;
@@ -268,15 +286,28 @@ __global__ __launch_bounds__(kRowThreads) void k_correct_rweight(
   const int nv = min(nl, ca.by - r0);
   const int z = ca.z0 + (int)blockIdx.y * ca.zstep;
   const CorrProj pr = ca.proj[z];
-  const float* raw = ca.raw + (size_t)z * ca.nx * ca.ny;
+  const size_t plane = (size_t)ca.nx * ca.ny;
+  const float* raw = ca.raw + z * plane;
   fft_load_tw(P, tw);
-  for (LineIter it(N); it.l < nl; it.next()) {
-    int l = it.l, k = it.k;
-    float2 v = make_float2(0.f, 0.f);
-    if (l < nv)
-      v = make_float2(corr_pixel(ca, pr, raw, 2 * k, r0 + l),
-                      corr_pixel(ca, pr, raw, 2 * k + 1, r0 + l));
-    a[l * ls + k] = v;
+  if (corr_transposed(ca, pr)) {
+    const float* rt = ca.rawT + z * plane;
+    for (LineIter it(N); it.l < nl; it.next()) {
+      int l = it.l, k = it.k;
+      float2 v = make_float2(0.f, 0.f);
+      if (l < nv)
+        v = make_float2(corr_pixel<true>(ca, pr, raw, rt, 2 * k, r0 + l),
+                        corr_pixel<true>(ca, pr, raw, rt, 2 * k + 1, r0 + l));
+      a[l * ls + k] = v;
+    }
+  } else {
+    for (LineIter it(N); it.l < nl; it.next()) {
+      int l = it.l, k = it.k;
+      float2 v = make_float2(0.f, 0.f);
+      if (l < nv)
+        v = make_float2(corr_pixel<false>(ca, pr, raw, nullptr, 2 * k, r0 + l),
+                        corr_pixel<false>(ca, pr, raw, nullptr, 2 * k + 1, r0 + l));
+      a[l * ls + k] = v;
+    }
   }
   __syncthreads();
   float2* Z = fft_lines(a, b, P, nl, false, tw);
@@ -396,10 +427,11 @@ void scale_slices(mf_ctx* ctx, hipStream_t st, float* d, size_t plane, int nz,
 void correct_bin(mf_ctx* ctx, hipStream_t st, const float* d_raw,
                  const CorrGeom& g, const CorrProj* d_proj, int z0, int count,
                  bool random_fill, float* d_out, int out_pitch,
-                 size_t out_stride) {
+                 size_t out_stride, const float* d_raw_t) {
   if (count <= 0) return;
   CorrArgs a;
   a.raw = d_raw;
+  a.rawT = d_raw_t;
   a.proj = d_proj;
   a.out = d_out;
   a.nx = g.nx;
@@ -439,12 +471,14 @@ void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
 
 void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
                      const CorrGeom& g, const CorrProj* d_proj, int z0,
-                     int count, bool random_fill, float* d_out, int zstep) {
+                     int count, bool random_fill, float* d_out, int zstep,
+                     const float* d_raw_t) {
   if (count <= 0) return;
   MF_REQUIRE(zstep != 0 || count == 1, "image step must be nonzero");
   MF_REQUIRE(g.bx % 2 == 0 && g.bx >= 2, "R-weighting needs an even row length");
   CorrArgs a;
   a.raw = d_raw;
+  a.rawT = d_raw_t;
   a.proj = d_proj;
   a.out = d_out;
   a.nx = g.nx;

@@ -140,6 +140,29 @@ def test_gpu_correct_matches_oracle(ctx, orc, args):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("axis", [86.5, 93.0, 4.0])
+def test_gpu_series_correction_bitexact(ctx, mf, orc, axis):
+    """A series' final correction (bin 1, random fill): axes nearer a quarter
+    turn gather from the transposed copy of the stack, the others from the
+    stack itself; both equal the oracle's CorrectProj bit for bit."""
+    nz = 5
+    stack = np.stack([blobs(150, 182, seed=20 + z) for z in range(nz)])
+    tilts = np.array([-8, -4, 0, 4, 8], np.float32)
+    G = mf.Series(ctx, stack, tilts)
+    G.preprocess()
+    pre = G.stack()
+    sx = np.array([3.25, -2.5, 0.0, 7.75, -11.0])
+    sy = np.array([-1.5, 4.0, 2.25, -6.0, 0.5])
+    G.set_params(sx, sy, np.full(nz, axis, np.float32))
+    got = G.correct(1)
+    for z in range(nz):
+        ref = orc.correct_proj(pre[z], float(sx[z]), float(sy[z]), axis, False,
+                               True, 1, False)
+        np.testing.assert_array_equal(got[z], ref)
+    G.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tilts,axis", [((0.0, 0.0), 0.0), ((3.0, 6.0), 85.0),
                                         ((-30.0, -33.0), 93.0),
                                         ((50.0, 57.0), 80.0)])
