@@ -320,6 +320,43 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_cols(
   }
 }
 
+// k_cc_cols for one spectral column per workgroup, 256 threads and
+// power-of-two columns of N points, with the transforms in place: LDS holds
+// the pair's two columns and the twiddles (3N float2, 24 KB at N = 1024,
+// against 5N), so twice as many workgroups are resident per CU -- all of a
+// two-pair sweep step at 1024^2 (1026) in one round.  Same arithmetic.
+template <bool TAB, int N>
+__global__ __launch_bounds__(kCcThreads) void k_cc_cols_inpl(
+    const CcImgs imgs, int w, int h, int ngroups, FftDevPlan P, float scale,
+    float bfactor) {
+  extern __shared__ float2 sm2[];
+  float2* x = sm2;          // [2][N]: the pair's columns
+  float2* tw = x + 2 * N;
+  const int nc = w / 2 + 1;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int pair = blk / ngroups;
+  const int c0 = blk - pair * ngroups;
+  const CcImg ia = cc_img<TAB>(imgs, 2 * pair), ib = cc_img<TAB>(imgs, 2 * pair + 1);
+  const float2* A = reinterpret_cast<const float2*>(ia.dst);
+  float2* B = reinterpret_cast<float2*>(ib.dst);
+  fft_load_tw(P, tw);
+  for (int r = threadIdx.x; r < 2 * N; r += kCcThreads) {
+    const int img = r >= N, row = r - img * N;
+    x[r] = (img ? B : A)[(size_t)row * nc + c0];
+  }
+  __syncthreads();
+  fft_pow2_inplace<N, false, 2, kCcThreads, N>(x, tw);
+  for (int row = threadIdx.x; row < N; row += kCcThreads) {
+    const float2 r = cscale(x[row], scale);
+    const float2 g = cscale(x[N + row], scale);
+    x[N + row] = cc_spectral(r, g, c0, row, nc, h, bfactor);
+  }
+  __syncthreads();
+  fft_pow2_inplace<N, true, 1, kCcThreads, N>(x + N, tw);
+  for (int row = threadIdx.x; row < N; row += kCcThreads)
+    B[(size_t)row * nc + c0] = x[N + row];
+}
+
 struct Best {
   float v;
   int i;
@@ -547,12 +584,29 @@ void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
     // Two lines (the pair's two columns) per column of the group.
     const Shape sh = cc_shape(ctx, "cc_cols", w, h, npair, 4 * lc * 8, lc * 8,
                               2 * (int)lc);
-    const size_t lds = (4 * sh.nl * lc + lc) * sizeof(float2);
-    set_lds(ctx, (const void*)k_cc_cols<TAB>, lds);
-    const int ngroups = cdiv(nc, sh.nl);
-    hipLaunchKernelGGL(k_cc_cols<TAB>, dim3(ngroups * npair), dim3(sh.threads),
-                       lds, st, imgs, w, h, ngroups, PC, sh.nl, scale, bfactor);
-    MF_HIP(hipGetLastError());
+    // One column per workgroup of 256 threads over power-of-two columns:
+    // the in-place form.
+    decltype(&k_cc_cols_inpl<TAB, 1024>) inpl = nullptr;
+    if (!PC.blue && sh.nl == 1 && sh.threads == kCcThreads) {
+      if (h == 512) inpl = k_cc_cols_inpl<TAB, 512>;
+      if (h == 1024) inpl = k_cc_cols_inpl<TAB, 1024>;
+      if (h == 2048) inpl = k_cc_cols_inpl<TAB, 2048>;
+    }
+    if (inpl) {
+      const size_t lds = (size_t)3 * h * sizeof(float2);
+      set_lds(ctx, (const void*)inpl, lds);
+      hipLaunchKernelGGL(inpl, dim3(nc * npair), dim3(kCcThreads), lds, st,
+                         imgs, w, h, nc, PC, scale, bfactor);
+      MF_HIP(hipGetLastError());
+    } else {
+      const size_t lds = (4 * sh.nl * lc + lc) * sizeof(float2);
+      set_lds(ctx, (const void*)k_cc_cols<TAB>, lds);
+      const int ngroups = cdiv(nc, sh.nl);
+      hipLaunchKernelGGL(k_cc_cols<TAB>, dim3(ngroups * npair),
+                         dim3(sh.threads), lds, st, imgs, w, h, ngroups, PC,
+                         sh.nl, scale, bfactor);
+      MF_HIP(hipGetLastError());
+    }
   }
   Best* peaks = (Best*)(part + (size_t)2 * nimg * kMomParts);
   const Shape sh = cc_shape(ctx, "cc_rowsi", w, h, npair, (2 * lr + 1) * 8,

@@ -308,6 +308,19 @@ __device__ __forceinline__ void pow2_butterfly(const float2* src, float2* dst,
   }
 }
 
+// The two halves of pow2_butterfly, for in-place stages: load R inputs
+// src[q*SS] times twiddles tw[q*tws] into x, then DFT_R of x to dst[p*DS].
+template <int R, int SS, bool INV>
+__device__ __forceinline__ void pow2_load(const float2* src, int tws,
+                                          const float2* __restrict__ tw,
+                                          float2* x) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int q = 0; q < R; q++) {
+    x[q] = src[q * SS];
+    if (q > 0 && tws) x[q] = tw_mul<INV>(x[q], tw[q * tws]);
+  }
+}
 // Stage S of the N-point schedule over nl lines (stride N) from x to y.
 template <int N, int S, bool INV>
 __device__ __forceinline__ void pow2_stage(const float2* x, float2* y, int nl,
@@ -336,6 +349,55 @@ __device__ __forceinline__ float2* pow2_stages_from(float2* x, float2* y,
     pow2_stage<N, S, INV>(x, y, nl, tw);
     return pow2_stages_from<N, S + 1, INV>(y, x, nl, tw);
   }
+}
+
+// Stage S in place over LINES lines (stride LS) with THREADS threads: every
+// thread loads the inputs of its butterflies (at most MAXB) into registers,
+// the workgroup syncs, then each writes its outputs over the same buffer.
+// Same arithmetic as pow2_stage, half the LDS.
+template <int N, int S, bool INV, int LINES, int THREADS, int LS>
+__device__ __forceinline__ void pow2_stage_inplace(float2* x,
+                                                   const float2* __restrict__ tw) {
+  constexpr int R = pow2_radix(N, S);
+  constexpr int NS = S == 0 ? 1 : (1 << (3 * S));
+  constexpr int NR = N / R;
+  constexpr int TSTRIDE = N / (NS * R);
+  constexpr int MAXB = (LINES * NR + THREADS - 1) / THREADS;
+  float2 v[MAXB][R];
+  int dst[MAXB];
+#pragma unroll
+  for (int b = 0; b < MAXB; b++) {
+    const int t = (int)threadIdx.x + b * THREADS;  // butterfly index
+    dst[b] = -1;
+    if (t < LINES * NR) {
+      const int l = t / NR, j = t - l * NR;
+      const int q = j / NS, k = j & (NS - 1);
+      pow2_load<R, NR, INV>(x + l * LS + j, k * TSTRIDE, tw, v[b]);
+      dst[b] = l * LS + q * NS * R + k;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MAXB; b++)
+    if (dst[b] >= 0) pow2_butterfly<R, 1, NS, INV>(v[b], x + dst[b], 0, tw);
+  __syncthreads();
+}
+
+template <int N, int S, bool INV, int LINES, int THREADS, int LS>
+__device__ __forceinline__ void pow2_inplace_from(float2* x,
+                                                  const float2* tw) {
+  if constexpr (S < pow2_stages(N)) {
+    pow2_stage_inplace<N, S, INV, LINES, THREADS, LS>(x, tw);
+    pow2_inplace_from<N, S + 1, INV, LINES, THREADS, LS>(x, tw);
+  }
+}
+
+// In-place transform of LINES lines of N points (stride LS) in x with a
+// THREADS-thread workgroup; bit-identical to fft_pow2.  Caller syncs after
+// loading; the result is in x on return (synchronised).
+template <int N, bool INV, int LINES, int THREADS, int LS>
+__device__ __forceinline__ void fft_pow2_inplace(float2* x, const float2* tw) {
+  pow2_inplace_from<N, 0, INV, LINES, THREADS, LS>(x, tw);
 }
 
 // Complex transform of nl lines of N points (stride N) in x, y scratch;

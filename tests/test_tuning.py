@@ -55,9 +55,12 @@ def _apply(c, prefix, shape):
 
 
 @pytest.mark.gpu
-def test_gpu_pa_measure_independent_of_launch_shape(fresh):
+@pytest.mark.parametrize("h,w", [(120, 144), (1024, 1024), (2048, 256)])
+def test_gpu_pa_measure_independent_of_launch_shape(fresh, h, w):
+    """Launch shapes change nothing; at power-of-two heights the default
+    column shape takes the in-place column transforms, the others the
+    ping-pong ones."""
     rng = np.random.default_rng(3)
-    h, w = 120, 144
     ref = blobs(h, w, seed=5)
     img = (np.roll(ref, (-3, 4), axis=(0, 1)) +
            0.1 * rng.standard_normal(ref.shape)).astype(np.float32)
