@@ -99,31 +99,33 @@ __device__ float cc_random(float fx, float fy, int pitch, int h, int w,
 __global__ __launch_bounds__(256) void k_stretch_pairs(
     const float* stack, int w, int h, const PairDesc* pairs, float* out,
     int fill) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= w * h) return;
-  const int y = idx / w, x = idx - y * w;
   const PairDesc pd = pairs[blockIdx.y];
   const int pitch = pad_x(w);
   const float* img = stack + (size_t)pd.img * w * h;
   const float hw = 0.5f * (float)w, hh = 0.5f * (float)h;
-  const float X = (float)x - hw + 0.5f;
-  const float Y = (float)((double)y - 0.5 * (double)h + 0.5);
-  const float buf = X * pd.m[0] + Y * pd.m[1];
-  const float Y2 = X * pd.m[1] + Y * pd.m[2] + hh - 0.5f;
-  const float X2 = buf + hw - 0.5f;
-  float v;
-  if (X2 >= 0.f && X2 < (float)(w - 1) && Y2 >= 0.f && Y2 < (float)(h - 1)) {
-    const int ix = (int)X2, iy = (int)Y2;
-    const float fx = X2 - (float)ix, fy = Y2 - (float)iy;
-    const float f2 = 1.0f - fx, f3 = 1.0f - fy;
-    const float* p = img + (size_t)iy * w + ix;
-    v = p[0] * f2 * f3 + p[1] * fx * f3 + p[w] * f2 * fy + p[w + 1] * fx * fy;
-  } else if (fill) {
-    v = cc_random(X2, Y2, pitch, h, w, img);
-  } else {
-    v = kInvalid;
+  // Grid-strided, a few pixels per thread.
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < w * h;
+       idx += gridDim.x * blockDim.x) {
+    const int y = idx / w, x = idx - y * w;
+    const float X = (float)x - hw + 0.5f;
+    const float Y = (float)((double)y - 0.5 * (double)h + 0.5);
+    const float buf = X * pd.m[0] + Y * pd.m[1];
+    const float Y2 = X * pd.m[1] + Y * pd.m[2] + hh - 0.5f;
+    const float X2 = buf + hw - 0.5f;
+    float v;
+    if (X2 >= 0.f && X2 < (float)(w - 1) && Y2 >= 0.f && Y2 < (float)(h - 1)) {
+      const int ix = (int)X2, iy = (int)Y2;
+      const float fx = X2 - (float)ix, fy = Y2 - (float)iy;
+      const float f2 = 1.0f - fx, f3 = 1.0f - fy;
+      const float* p = img + (size_t)iy * w + ix;
+      v = p[0] * f2 * f3 + p[1] * fx * f3 + p[w] * f2 * fy + p[w + 1] * fx * fy;
+    } else if (fill) {
+      v = cc_random(X2, Y2, pitch, h, w, img);
+    } else {
+      v = kInvalid;
+    }
+    out[(size_t)blockIdx.y * h * pitch + (size_t)y * pitch + x] = v;
   }
-  out[(size_t)blockIdx.y * h * pitch + (size_t)y * pitch + x] = v;
 }
 
 // Images of a launch: a device table (TAB), or for two-image launches the
@@ -644,7 +646,7 @@ void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   MF_HIP(hipMemcpyAsync(dt, pairs.data(), b0, hipMemcpyHostToDevice, st));
   MF_HIP(hipMemcpyAsync(dt + b0, im.data(), b1, hipMemcpyHostToDevice, st));
   const PairDesc* d_pairs = (const PairDesc*)dt;
-  const dim3 g2(cdiv((long long)w * h, 256), npair);
+  const dim3 g2(cdiv((long long)w * h, 256 * 4), npair);
   hipLaunchKernelGGL(k_stretch_pairs, g2, dim3(256), 0, st, d_stack, w, h,
                      d_pairs, S, 1);
   MF_HIP(hipGetLastError());

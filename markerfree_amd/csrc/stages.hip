@@ -34,13 +34,19 @@ namespace {
 
 constexpr int kRedThreads = 1024;
 
+// Min and max of part blockIdx.x (of kMinMaxParts) of slice blockIdx.y;
+// out [nz][kMinMaxParts][2] (the host folds the parts: min and max do not
+// depend on the order).
+constexpr int kMinMaxParts = 16;
 __global__ __launch_bounds__(kRedThreads) void k_minmax(const float* d,
                                                         size_t plane,
                                                         float* out) {
   __shared__ float sh[16];
-  const float* p = d + (size_t)blockIdx.x * plane;
+  const size_t part = (plane + kMinMaxParts - 1) / kMinMaxParts;
+  const size_t i0 = blockIdx.x * part, i1 = min(plane, i0 + part);
+  const float* p = d + (size_t)blockIdx.y * plane;
   float mn = 1e20f, mx = -1e20f;
-  for (size_t i = threadIdx.x; i < plane; i += blockDim.x) {
+  for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     float v = p[i];
     if (v < -1e10f) continue;
     mn = fminf(mn, v);
@@ -49,8 +55,9 @@ __global__ __launch_bounds__(kRedThreads) void k_minmax(const float* d,
   mn = block_reduce(mn, sh, [](float a, float b) { return fminf(a, b); }, 1e20f);
   mx = block_reduce(mx, sh, [](float a, float b) { return fmaxf(a, b); }, -1e20f);
   if (threadIdx.x == 0) {
-    out[2 * blockIdx.x] = mn;
-    out[2 * blockIdx.x + 1] = mx;
+    const size_t o = ((size_t)blockIdx.y * kMinMaxParts + blockIdx.x) * 2;
+    out[o] = mn;
+    out[o + 1] = mx;
   }
 }
 
@@ -206,23 +213,28 @@ __device__ __forceinline__ float corr_pixel(const CorrArgs& a,
   return bad ? kInvalid : sum;
 }
 
-// One binned output pixel per thread.
+// Binned output pixels, grid-strided over the image (a few per thread: one
+// per thread made the launch dispatch-bound).
 __global__ __launch_bounds__(256) void k_correct_bin(const CorrArgs a) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.bx * a.by) return;
-  const int y = idx / a.bx, x = idx - y * a.bx;
   const int z = a.z0 + blockIdx.y;
   const CorrProj pr = a.proj[z];
   const size_t plane = (size_t)a.nx * a.ny;
   const float* raw = a.raw + z * plane;
-  float* o = a.out + (size_t)blockIdx.y * a.out_stride + (size_t)y * a.out_pitch + x;
-  if (corr_transposed(a, pr))
-    *o = corr_pixel<true>(a, pr, raw, a.rawT + z * plane, x, y);
-  else
-    *o = corr_pixel<false>(a, pr, raw, nullptr, x, y);
+  const bool tr = corr_transposed(a, pr);
+  const int n = a.bx * a.by;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += gridDim.x * blockDim.x) {
+    const int y = idx / a.bx, x = idx - y * a.bx;
+    float* o = a.out + (size_t)blockIdx.y * a.out_stride + (size_t)y * a.out_pitch + x;
+    if (tr)
+      *o = corr_pixel<true>(a, pr, raw, a.rawT + z * plane, x, y);
+    else
+      *o = corr_pixel<false>(a, pr, raw, nullptr, x, y);
+  }
 }
 
 constexpr int kRowThreads = 256;
+constexpr int kBinPixelsPerThread = 4;
 
 // R-weighting of nl padded rows per workgroup: R2C (length bx), scale by
 // 1/N and the ramp filter, C2R, unpadded write (RWeight::DoIt + Unpad).
@@ -375,13 +387,24 @@ const float* rweight_filter(mf_ctx* ctx, int n) {
 
 void slice_minmax(mf_ctx* ctx, hipStream_t st, const float* d, size_t plane,
                   int nz, DevBuf& tmp, float* h_out) {
-  float* buf = (float*)tmp.ensure(ctx, sizeof(float) * 2 * nz);
-  hipLaunchKernelGGL(k_minmax, dim3(nz), dim3(kRedThreads), 0, st, d, plane,
-                     buf);
+  const size_t nparts = (size_t)nz * kMinMaxParts;
+  float* buf = (float*)tmp.ensure(ctx, sizeof(float) * 2 * nparts);
+  hipLaunchKernelGGL(k_minmax, dim3(kMinMaxParts, nz), dim3(kRedThreads), 0, st,
+                     d, plane, buf);
   MF_HIP(hipGetLastError());
-  MF_HIP(hipMemcpyAsync(h_out, buf, sizeof(float) * 2 * nz,
+  std::vector<float> parts(2 * nparts);
+  MF_HIP(hipMemcpyAsync(parts.data(), buf, sizeof(float) * 2 * nparts,
                         hipMemcpyDeviceToHost, st));
   MF_HIP(hipStreamSynchronize(st));
+  for (int z = 0; z < nz; z++) {
+    float mn = 1e20f, mx = -1e20f;
+    for (int k = 0; k < kMinMaxParts; k++) {
+      mn = std::fmin(mn, parts[2 * ((size_t)z * kMinMaxParts + k)]);
+      mx = std::fmax(mx, parts[2 * ((size_t)z * kMinMaxParts + k) + 1]);
+    }
+    h_out[2 * z] = mn;
+    h_out[2 * z + 1] = mx;
+  }
 }
 
 void reorder_subtract(mf_ctx* ctx, hipStream_t st, const float* d_raw,
@@ -447,7 +470,9 @@ void correct_bin(mf_ctx* ctx, hipStream_t st, const float* d_raw,
   a.out_stride = out_stride;
   a.zstep = 1;
   TimedLaunch tl(ctx->timer, "correct", st);
-  hipLaunchKernelGGL(k_correct_bin, dim3(cdiv((long long)g.bx * g.by, 256), count),
+  hipLaunchKernelGGL(k_correct_bin,
+                     dim3(cdiv((long long)g.bx * g.by, 256 * kBinPixelsPerThread),
+                          count),
                      dim3(256), 0, st, a);
   MF_HIP(hipGetLastError());
 }
