@@ -67,6 +67,7 @@ constexpr int kCcThreads = 256;
 // Partial (sum, sum of squares) blocks per image; their combination is a
 // fixed-order tree, so every consumer computes the same moments.
 constexpr int kMomParts = 256;
+constexpr int kMomBatch = 8;
 
 // Pixel of an unpadded w-wide image read through the padded pitch the
 // reference uses (pad columns hold zero here; the reference leaves them
@@ -158,11 +159,23 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_moments(const CcImgs imgs,
   const int rows = cdiv(h, kMomParts);
   const int ya = min(h, (int)blockIdx.x * rows), yb = min(h, ya + rows);
   double s1 = 0.0, s2 = 0.0;
-  for (LineIter it(w); it.l < yb - ya; it.next()) {
-    const float v = im.src[(size_t)(ya + it.l) * im.pitch + it.k];
-    const bool ok = v >= -1e10f;
-    s1 += ok ? (double)v : 0.0;
-    s2 += ok ? (double)(v * v) : 0.0;
+  // Batches of kMomBatch pixels whose loads are issued before any is used
+  // (a load per loop step left one memory latency per pixel); the sums take
+  // the pixels in the same order.
+  const int nr = yb - ya;
+  for (LineIter it(w); it.l < nr;) {
+    float v[kMomBatch];
+#pragma unroll
+    for (int j = 0; j < kMomBatch; j++) {
+      v[j] = it.l < nr ? im.src[(size_t)(ya + it.l) * im.pitch + it.k] : -1e30f;
+      it.next();
+    }
+#pragma unroll
+    for (int j = 0; j < kMomBatch; j++) {
+      const bool ok = v[j] >= -1e10f;
+      s1 += ok ? (double)v[j] : 0.0;
+      s2 += ok ? (double)(v[j] * v[j]) : 0.0;
+    }
   }
   s1 = block_reduce(s1, sh, [](double a, double b) { return a + b; }, 0.0);
   s2 = block_reduce(s2, sh, [](double a, double b) { return a + b; }, 0.0);
@@ -237,11 +250,21 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_rows_fwd(
     }
     return o;
   };
-  for (LineIter it(N); it.l < nl; it.next()) {
-    const int l = it.l, k = it.k;
-    float2 v = make_float2(0.f, 0.f);
-    if (l < nv) v = make_float2(norm(r0 + l, 2 * k), norm(r0 + l, 2 * k + 1));
-    a[l * ls + k] = v;
+  // Four of a thread's entries per batch, their loads issued together.
+  for (LineIter it(N); it.l < nl;) {
+    float2 v[4];
+    int at[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int l = it.l, k = it.k;
+      at[j] = l < nl ? l * ls + k : -1;
+      v[j] = make_float2(0.f, 0.f);
+      if (l < nv) v[j] = make_float2(norm(r0 + l, 2 * k), norm(r0 + l, 2 * k + 1));
+      it.next();
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (at[j] >= 0) a[at[j]] = v[j];
   }
   __syncthreads();
   float2* Z = fft_lines(a, b, P, nl, false, tw);
@@ -425,11 +448,21 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_rows_inv(
   const int bs = ls + 1, cp = w / 2 + 1;
   float2* map = reinterpret_cast<float2*>(im.dst);
   fft_load_tw(P, tw);
-  for (LineIter it(N + 1); it.l < nl; it.next()) {
-    const int l = it.l, k = it.k;
-    float2 v = make_float2(0.f, 0.f);
-    if (l < nv) v = map[(size_t)(r0 + l) * cp + k];
-    b[l * bs + k] = v;
+  // Four of a thread's entries per batch, their loads issued together.
+  for (LineIter it(N + 1); it.l < nl;) {
+    float2 v[4];
+    int at[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int l = it.l, k = it.k;
+      at[j] = l < nl ? l * bs + k : -1;
+      v[j] = make_float2(0.f, 0.f);
+      if (l < nv) v[j] = map[(size_t)(r0 + l) * cp + k];
+      it.next();
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (at[j] >= 0) b[at[j]] = v[j];
   }
   __syncthreads();
   for (LineIter it(N); it.l < nl; it.next()) {
