@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "context.h"
+#include "stages.h"
 
 namespace mf {
 
@@ -33,11 +34,29 @@ void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
                       float bfactor, float* d_work, double* d_mom,
                       DevBuf& tab, float* d_shift);
 
+// The per-target parameter update of a projection-matching step
+// (ClacAlignProj, ProjAlign.cpp:276-296; RotShift, :448-457), applied by the correlation's fit for map
+// j to target t[j]: meas = ishift * bin, err = |meas|, shift = RotShift(meas)
+// + Z-induced shift, (sx, sy) += shift, and the target's correction
+// parameters cp[t] refreshed.  sx == nullptr: no update.
+struct PaUpdate {
+  float* meas;        // [nz][2]
+  float* err;         // [nz]
+  const float* ind;   // [nz][2]
+  const float* rcs;   // [nz][2] cos, sin of the axis
+  double* sx;
+  double* sy;
+  CorrProj* cp;
+  int t[2];
+  float binx, biny;
+};
+
 // MeaAlignProj: d_reproj and d_proj are w x h; writes the float2 ishift.
 // Workspace: 2*h*pad_x(w) floats + cc_scratch_doubles(1, h) doubles.
 void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
                 const float* d_proj, int w, int h, float* d_work,
-                double* d_mom, float* d_ishift);
+                double* d_mom, float* d_ishift,
+                const PaUpdate* up = nullptr);
 
 // One input image of a correlation: its pixels (row pitch in floats) and the
 // padded work image [h][pad_x(w)] it is normalised and transformed in.
@@ -53,7 +72,8 @@ struct CcImg {
 // equals pa_measure's on the same images.  Workspace: cc_scratch_doubles(
 // npair, h) doubles.
 void pa_measure_tab(mf_ctx* ctx, hipStream_t st, const CcImg* d_tab,
-                    int npair, int w, int h, double* d_mom, float* d_ishift);
+                    int npair, int w, int h, double* d_mom, float* d_ishift,
+                    const PaUpdate* up = nullptr);
 
 size_t stretch_cc_work_floats(int w, int h, int npair);
 size_t cc_scratch_doubles(int npair, int h);

@@ -499,7 +499,8 @@ template <bool TAB>
 __global__ __launch_bounds__(kCcThreads) void k_cc_fit(const CcImgs imgs,
                                                        int w, int h, int mode,
                                                        const Best* part,
-                                                       int nparts, float* out) {
+                                                       int nparts, float* out,
+                                                       const PaUpdate up) {
   __shared__ Best sh[16];
   const int pitch = pad_x(w);
   const float* cc = cc_img<TAB>(imgs, 2 * blockIdx.x + 1).dst;
@@ -535,8 +536,25 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_fit(const CcImgs imgs,
   if (fabs(dx) > 1) dx = 0;
   if (fabs(dy) > 1) dy = 0;
   const float fpx = (float)(px + dx), fpy = (float)(py + dy);
-  out[2 * blockIdx.x] = fpx - (float)(w / 2);
-  out[2 * blockIdx.x + 1] = fpy - (float)(h / 2);
+  const float ishx = fpx - (float)(w / 2), ishy = fpy - (float)(h / 2);
+  out[2 * blockIdx.x] = ishx;
+  out[2 * blockIdx.x + 1] = ishy;
+  if (up.sx) {
+    const int t = up.t[blockIdx.x];
+    const float a0 = ishx * up.binx, a1 = ishy * up.biny;
+    up.meas[2 * t] = a0;
+    up.meas[2 * t + 1] = a1;
+    up.err[t] = (float)sqrt((double)(a0 * a0 + a1 * a1));
+    const float c = up.rcs[2 * t], sn = up.rcs[2 * t + 1];
+    float r0 = a0 * c - a1 * sn;
+    float r1 = a0 * sn + a1 * c;
+    r0 += up.ind[2 * t];
+    r1 += up.ind[2 * t + 1];
+    up.sx[t] += (double)r0;
+    up.sy[t] += (double)r1;
+    up.cp[t].shift_x = (float)up.sx[t];
+    up.cp[t].shift_y = (float)up.sy[t];
+  }
 }
 
 void set_lds(mf_ctx* ctx, const void* k, size_t bytes) {
@@ -574,27 +592,32 @@ Shape cc_shape(mf_ctx* ctx, const char* family, int w, int h, int npair,
 template <bool TAB>
 void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
                int w, int h, int unit_std, float scale, float bfactor,
-               int mode, const float* mask, double* part, float* d_shift);
+               int mode, const float* mask, double* part, float* d_shift,
+               const PaUpdate& up);
 
 // The correlation of npair pairs (images 2p, 2p+1 of 'imgs'): moments,
 // normalisation, forward transforms, spectral product, inverse transform,
 // peak search and fit.  'part' holds cc_scratch_doubles(npair, h) doubles.
 void cc_run(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair, int w,
             int h, int unit_std, float scale, float bfactor, int mode,
-            const float* mask, double* part, float* d_shift) {
+            const float* mask, double* part, float* d_shift,
+            const PaUpdate* up = nullptr) {
   MF_REQUIRE(w % 2 == 0 && w >= 2 && h >= 1, "bad correlation shape");
+  MF_REQUIRE(!up || npair <= 2, "a parameter update covers at most two maps");
+  const PaUpdate none{};
   if (imgs.tab)
     cc_launch<true>(ctx, st, imgs, npair, w, h, unit_std, scale, bfactor,
-                    mode, mask, part, d_shift);
+                    mode, mask, part, d_shift, up ? *up : none);
   else
     cc_launch<false>(ctx, st, imgs, npair, w, h, unit_std, scale, bfactor,
-                     mode, mask, part, d_shift);
+                     mode, mask, part, d_shift, up ? *up : none);
 }
 
 template <bool TAB>
 void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
                int w, int h, int unit_std, float scale, float bfactor,
-               int mode, const float* mask, double* part, float* d_shift) {
+               int mode, const float* mask, double* part, float* d_shift,
+               const PaUpdate& up) {
   const int nimg = 2 * npair;
   TimedLaunch tl(ctx->timer, "cc", st);
   hipLaunchKernelGGL(k_cc_moments<TAB>, dim3(kMomParts, nimg), dim3(kCcThreads), 0,
@@ -652,7 +675,7 @@ void cc_launch(mf_ctx* ctx, hipStream_t st, const CcImgs& imgs, int npair,
   hipLaunchKernelGGL(k_cc_rows_inv<TAB>, dim3(nparts, npair), dim3(sh.threads),
                      lds, st, imgs, w, h, PR, sh.nl, mode, peaks);
   hipLaunchKernelGGL(k_cc_fit<TAB>, dim3(npair), dim3(kCcThreads), 0, st, imgs, w, h,
-                     mode, (const Best*)peaks, nparts, d_shift);
+                     mode, (const Best*)peaks, nparts, d_shift, up);
   MF_HIP(hipGetLastError());
 }
 
@@ -700,7 +723,7 @@ size_t cc_scratch_doubles(int npair, int h) {
 
 void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
                 const float* d_proj, int w, int h, float* d_work,
-                double* d_mom, float* d_ishift) {
+                double* d_mom, float* d_ishift, const PaUpdate* up) {
   const size_t img = (size_t)h * pad_x(w);
   CcImgs imgs{};
   imgs.src0 = d_reproj;
@@ -712,17 +735,18 @@ void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
   // PANorm: std := 1 (ProjAlign.cpp:272-293); unnormalised transforms and
   // B = 500 (:258-271); PAFindPeak window (:325-375).
   cc_run(ctx, st, imgs, 1, w, h, 1, 1.0f, 500.0f, 1, cc_mask(ctx, w, h, true),
-         d_mom, d_ishift);
+         d_mom, d_ishift, up);
 }
 
 void pa_measure_tab(mf_ctx* ctx, hipStream_t st, const CcImg* d_tab,
-                    int npair, int w, int h, double* d_mom, float* d_ishift) {
+                    int npair, int w, int h, double* d_mom, float* d_ishift,
+                    const PaUpdate* up) {
   MF_REQUIRE(npair > 0, "no pairs to measure");
   CcImgs imgs{};
   imgs.tab = d_tab;
   // The same normalisation, filter and peak window as pa_measure.
   cc_run(ctx, st, imgs, npair, w, h, 1, 1.0f, 500.0f, 1,
-         cc_mask(ctx, w, h, true), d_mom, d_ishift);
+         cc_mask(ctx, w, h, true), d_mom, d_ishift, up);
 }
 
 }  // namespace mf

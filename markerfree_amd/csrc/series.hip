@@ -114,29 +114,6 @@ PaDev pa_layout(mf_series* s) {
   return d;
 }
 
-// ClacAlignProj's parameter update (ProjAlign.cpp:276-296) for the step's
-// targets (thread j: target tj with ishift[j]; one thread per target):
-// shift = RotShift(ishift * bin, axis) + Z-induced; param += shift.  The
-// target's correction parameters are refreshed for its re-correction.
-__global__ void k_pa_update(PaDev d, int t0, int t1, float binx, float biny,
-                            CorrProj* cp) {
-  const int j = threadIdx.x;
-  const int t = j ? t1 : t0;
-  const float2 ish = d.ishift[j];
-  const float a0 = ish.x * binx, a1 = ish.y * biny;
-  d.meas[t] = make_float2(a0, a1);
-  d.err[t] = (float)sqrt((double)(a0 * a0 + a1 * a1));
-  const float c = d.rcs[t].x, sn = d.rcs[t].y;
-  float r0 = a0 * c - a1 * sn;
-  float r1 = a0 * sn + a1 * c;
-  r0 += d.ind[t].x;
-  r1 += d.ind[t].y;
-  d.sx[t] += (double)r0;
-  d.sy[t] += (double)r1;
-  cp[t].shift_x = (float)d.sx[t];
-  cp[t].shift_y = (float)d.sy[t];
-}
-
 void rot_shift(const float* in, float angle, float* out) {
   // ProjAlign::RotShift (ProjAlign.cpp:448-457).
   float c = (float)std::cos(angle * kD2R), s = (float)std::sin(angle * kD2R);
@@ -495,17 +472,27 @@ float series_projalign_iter(mf_series* s) {
                        reproj + k * plane, slab);
       skip[ts[k]] = 0;  // the next target's range sees this one aligned
     }
+    // The correlation's fit also applies each target's parameter update.
+    PaUpdate up{};
+    up.meas = (float*)d.meas;
+    up.err = d.err;
+    up.ind = (const float*)d.ind;
+    up.rcs = (const float*)d.rcs;
+    up.sx = d.sx;
+    up.sy = d.sy;
+    up.cp = cp;
+    up.t[0] = st.a;
+    up.t[1] = st.b;
+    up.binx = binx;
+    up.biny = biny;
     if (n == 2) {
       pa_measure_tab(ctx, s->st, d_tab + 4 * paired, 2, g.bx, g.by, mom,
-                     (float*)d.ishift);
+                     (float*)d.ishift, &up);
       paired++;
     } else {
       pa_measure(ctx, s->st, reproj, corr + st.a * plane, g.bx, g.by, work,
-                 mom, (float*)d.ishift);
+                 mom, (float*)d.ishift, &up);
     }
-    hipLaunchKernelGGL(k_pa_update, dim3(1), dim3(n), 0, s->st, d, st.a,
-                       st.b, binx, biny, cp);
-    MF_HIP(hipGetLastError());
     // Re-correct the step's targets (one launch for a pair).
     correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, st.a, n,
                     true, corr + st.a * plane, n == 2 ? st.b - st.a : 1,
