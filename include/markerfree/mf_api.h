@@ -175,6 +175,44 @@ mf_status mf_common_lines(mf_ctx* ctx, const float* d_stack, int nx, int ny,
                           const float* shift_y, float axis0, float range,
                           int num, float* d_lines, int* line_len);
 
+/* CalcTIltAxis::Radon + FFT1d + findTiltAxis/CalcScore
+ * (method/Rotate.cu:397-561) for the candidate grid of CLsetup (:339-374)
+ * around axis0: the leave-one-out line-correlation score of every candidate
+ * into scores (host, num floats), the best candidate before the wrap rules
+ * of DoIt (:316-322) into *best_axis (the first maximum above 0, as
+ * findTiltAxis), and, when d_spectra (device) is non-NULL, the line spectra
+ * [nz][num][L/2+1] complex values (float pairs, unnormalised R2C of the
+ * mean-removed, tapered lines).  Any output pointer may be NULL. */
+mf_status mf_line_scores(mf_ctx* ctx, const float* d_stack, int nx, int ny,
+                         int nz, const float* tilts, const float* shift_x,
+                         const float* shift_y, float axis0, float range,
+                         int num, float* scores, float* d_spectra,
+                         int* line_len, float* best_axis);
+
+/* FindOffset::Correlate (method/FindOffset.cu:176-238) of one w x h pair:
+ * d_img stretched towards d_ref by cos(ref_tilt)/cos(tilt) (axis 0, no
+ * fill), the six masked moments of GConv2D (:7-47) -- sums of v1, v2, v1^2,
+ * v2^2, v1*v2 and the count of pixels valid in both -- into moments6 (host),
+ * and the reference's correlation with quirk Q7 into *cc.  Either output
+ * may be NULL. */
+mf_status mf_ncc_moments(mf_ctx* ctx, const float* d_ref, const float* d_img,
+                         int w, int h, float ref_tilt, float tilt,
+                         double* moments6, float* cc);
+
+/* FindOffset::SearchOffset + CalcAverangedCC (method/FindOffset.cu:129-174)
+ * over a bin-4 corrected stack d_corr [nz][h][w] (device): the mean
+ * correlation of each of the 31 one-degree steps into ccs (host, 31 floats,
+ * may be NULL) and the best step (strict maximum above 0) into *best. */
+mf_status mf_offset_scan(mf_ctx* ctx, const float* d_corr, int w, int h,
+                         int nz, const float* tilts, float* ccs, float* best);
+
+/* Peak search and parabolic sub-pixel fit on a w x h correlation map
+ * d_map (device, row pitch 'pitch' floats): mode 0 CalcCC::FindPeak
+ * (method/CalcCC.cu:381-441), mode 1 ProjAlign::PAFindPeak
+ * (method/ProjAlign.cpp:325-375).  shift2 (host) = peak - (w/2, h/2). */
+mf_status mf_peak2d(mf_ctx* ctx, const float* d_map, int w, int h, int pitch,
+                    int mode, float* shift2);
+
 /* ------------------------------------------------------------- a series */
 
 /* Flags of the reference command line (main/opts.h:59-84, -g and -p). */
@@ -263,6 +301,35 @@ mf_status mf_correct(mf_series* s, int out_bin, float* out, int* bx, int* by);
 /* Process::DoIt (main/Process.cpp:34-92) minus file I/O: preprocess, coarse
  * alignment, axis and offset searches, projection matching. */
 mf_status mf_run(mf_series* s, const mf_options* o);
+
+/* The last projection-matching sweep's measured shift of every target (raw
+ * pixels, [nz][2], before RotShift; the zero tilt holds 0) and its length
+ * |shift| (ProjAlign::ClacAlignProj, method/ProjAlign.cpp:181-241, the values
+ * the sweep's verbose table prints).  Either pointer may be NULL. */
+mf_status mf_pa_measurements(mf_series* s, float* meas, float* err);
+/* Scores of the last mf_tiltaxis (*num = candidates; up to cap written) and
+ * of the last mf_find_offset (31 values); NULL pointers are skipped. */
+mf_status mf_axis_scores(mf_series* s, float* scores, int cap, int* num);
+mf_status mf_offset_scores(mf_series* s, float* ccs);
+
+/* Stage trace: every stage call of a series appends one event (kind,
+ * value) -- the stage sequence Process::DoIt drives (main/Process.cpp:34-92,
+ * 238-319), with the values its stdout lines print.  mf_series_trace copies
+ * up to cap events and sets *count to the number recorded. */
+enum {
+  MF_EV_PREPROCESS = 1,   /* value 0 */
+  MF_EV_COARSE = 2,       /* Transform::DoIt; value maxerr */
+  MF_EV_AXIS = 3,         /* CalcTIltAxis::DoIt; value the new axis */
+  MF_EV_OFFSET = 4,       /* FindOffset::DoIt; value the scan's best step */
+  MF_EV_APPLY_OFFSET = 5, /* value the total offset */
+  MF_EV_PA_SETUP = 6,     /* ProjAlign::Setup; value the slab thickness */
+  MF_EV_PA_ITER = 7,      /* ProjAlign::DoIt; value maxerr */
+  MF_EV_ONCE_BEGIN = 8,   /* ProjAlignOnce entered; value 0 */
+  MF_EV_ONCE_END = 9      /* value the number of DoIt calls it made */
+};
+mf_status mf_series_trace(mf_series* s, int* kinds, float* values, int cap,
+                          int* count);
+mf_status mf_series_trace_clear(mf_series* s);
 
 /* ------------------------------------------- batched and served forms
  * No counterpart in the reference, which aligns one series per process;

@@ -175,6 +175,28 @@ class DeviceArray:
             self.ctx._check(_lib.mf_dev_free(self.ctx.h, self.ptr))
             self.ptr = None
 
+    def view(self, offset, shape) -> "DeviceView":
+        """A part of this buffer, 'offset' elements in, of 'shape'; it does
+        not own the memory and must not outlive the buffer."""
+        n = int(np.prod(shape))
+        if offset < 0 or (offset + n) * self.dtype.itemsize > self.nbytes:
+            raise ValueError("view outside the buffer")
+        return DeviceView(self, offset, shape)
+
+
+class DeviceView(DeviceArray):
+    """A non-owning window into a DeviceArray (see DeviceArray.view)."""
+
+    def __init__(self, base: DeviceArray, offset, shape):
+        self.ctx = base.ctx
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.dtype = base.dtype
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = base.ptr + int(offset) * self.dtype.itemsize
+
+    def free(self):
+        self.ptr = None
+
 
 class Context:
     """mf_ctx: one device, one stream, an HBM budget and kernel timers."""
@@ -330,6 +352,22 @@ _sig("mf_projalign_iter", C.c_int, _vp, _fp)
 _sig("mf_projalign_once", C.c_int, _vp)
 _sig("mf_correct", C.c_int, _vp, C.c_int, _fp, _ip, _ip)
 _sig("mf_run", C.c_int, _vp, C.POINTER(Options))
+_sig("mf_pa_measurements", C.c_int, _vp, _fp, _fp)
+_sig("mf_axis_scores", C.c_int, _vp, _fp, C.c_int, _ip)
+_sig("mf_offset_scores", C.c_int, _vp, _fp)
+_sig("mf_series_trace", C.c_int, _vp, _ip, _fp, C.c_int, _ip)
+_sig("mf_series_trace_clear", C.c_int, _vp)
+_sig("mf_line_scores", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _fp, _fp,
+     _fp, C.c_float, C.c_float, C.c_int, _fp, _vp, _ip, _fp)
+_sig("mf_ncc_moments", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.c_float,
+     C.c_float, _dp, _fp)
+_sig("mf_offset_scan", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _fp, _fp,
+     _fp)
+_sig("mf_peak2d", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _fp)
+
+# Stage trace kinds (mf_api.h MF_EV_*).
+EV_PREPROCESS, EV_COARSE, EV_AXIS, EV_OFFSET, EV_APPLY_OFFSET, EV_PA_SETUP, \
+    EV_PA_ITER, EV_ONCE_BEGIN, EV_ONCE_END = range(1, 10)
 
 
 def default_options(**kw) -> Options:
@@ -382,6 +420,71 @@ def _ctx_common_lines(self, d_stack, nx, ny, nz, tilts, sx, sy, axis0,
         out.free()
 
 
+def _ctx_line_scores(self, d_stack, nx, ny, nz, tilts, sx, sy, axis0, arange,
+                     num=100, spectra=False):
+    """Common-line score of every axis candidate: (scores, best candidate
+    before the wrap rules, L, spectra [nz][num][L/2+1] complex64 or None)."""
+    a = [np.ascontiguousarray(v, np.float32) for v in (tilts, sx, sy)]
+    scores = np.zeros(num, np.float32)
+    L = C.c_int()
+    best = C.c_float()
+    d_spec = None
+    if spectra:
+        n = _line_len(self, d_stack, nx, ny, nz, a, axis0, arange, num)
+        d_spec = self.empty((nz, num, n + 2))
+    try:
+        self._check(_lib.mf_line_scores(
+            self.h, d_stack.ptr, nx, ny, nz, *map(f32p, a), axis0, arange,
+            num, f32p(scores), d_spec.ptr if d_spec else None, C.byref(L),
+            C.byref(best)))
+        spec = None
+        if d_spec is not None:
+            spec = d_spec.download().view(np.complex64)
+        return scores, float(best.value), int(L.value), spec
+    finally:
+        if d_spec is not None:
+            d_spec.free()
+
+
+def _line_len(ctx, d_stack, nx, ny, nz, a, axis0, arange, num):
+    L = C.c_int()
+    ctx._check(_lib.mf_common_lines(ctx.h, d_stack.ptr, nx, ny, nz,
+                                    *map(f32p, a),
+                                    axis0, arange, num, None, C.byref(L)))
+    return int(L.value)
+
+
+def _ctx_ncc_moments(self, d_ref, d_img, w, h, ref_tilt, tilt):
+    """FindOffset::Correlate of one pair: (six moments, Q7 correlation)."""
+    m = np.zeros(6, np.float64)
+    cc = C.c_float()
+    self._check(_lib.mf_ncc_moments(self.h, d_ref.ptr, d_img.ptr, w, h,
+                                    ref_tilt, tilt, m.ctypes.data_as(_dp),
+                                    C.byref(cc)))
+    return m, float(cc.value)
+
+
+def _ctx_offset_scan(self, d_corr, w, h, nz, tilts):
+    """SearchOffset over a bin-4 corrected stack: (best, ccs[31])."""
+    t = np.ascontiguousarray(tilts, np.float32)
+    ccs = np.zeros(31, np.float32)
+    best = C.c_float()
+    self._check(_lib.mf_offset_scan(self.h, d_corr.ptr, w, h, nz, f32p(t),
+                                    f32p(ccs), C.byref(best)))
+    return float(best.value), ccs
+
+
+def _ctx_peak2d(self, d_map, w, h, pitch, mode):
+    out = np.zeros(2, np.float32)
+    self._check(_lib.mf_peak2d(self.h, d_map.ptr, w, h, pitch, mode,
+                               f32p(out)))
+    return float(out[0]), float(out[1])
+
+
+Context.line_scores = _ctx_line_scores
+Context.ncc_moments = _ctx_ncc_moments
+Context.offset_scan = _ctx_offset_scan
+Context.peak2d = _ctx_peak2d
 Context.correct_proj = _ctx_correct_proj
 Context.common_lines = _ctx_common_lines
 Context.stretch_cc = _ctx_stretch_cc
@@ -525,6 +628,38 @@ class Series:
     def run(self, options: Options = None, **kw):
         o = options if options is not None else default_options(**kw)
         self._check(_lib.mf_run(self.h, C.byref(o)))
+
+    def pa_measurements(self):
+        """The last sweep's measured shifts [nz][2] (raw px) and lengths."""
+        meas = np.zeros((self.nz, 2), np.float32)
+        err = np.zeros(self.nz, np.float32)
+        self._check(_lib.mf_pa_measurements(self.h, f32p(meas), f32p(err)))
+        return meas, err
+
+    def axis_scores(self):
+        n = C.c_int()
+        self._check(_lib.mf_axis_scores(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.float32)
+        self._check(_lib.mf_axis_scores(self.h, f32p(out), n.value, None))
+        return out
+
+    def offset_scores(self):
+        out = np.zeros(31, np.float32)
+        self._check(_lib.mf_offset_scores(self.h, f32p(out)))
+        return out
+
+    def trace(self):
+        """Stage trace [(kind, value), ...] (EV_* kinds)."""
+        n = C.c_int()
+        self._check(_lib.mf_series_trace(self.h, None, None, 0, C.byref(n)))
+        k = np.zeros(n.value, np.int32)
+        v = np.zeros(n.value, np.float32)
+        self._check(_lib.mf_series_trace(self.h, k.ctypes.data_as(_ip),
+                                         f32p(v), n.value, None))
+        return list(zip(k.tolist(), v.tolist()))
+
+    def trace_clear(self):
+        self._check(_lib.mf_series_trace_clear(self.h))
 
 
 # ------------------------------------------------ batched and served forms

@@ -457,6 +457,82 @@ mf_status mf_common_lines(mf_ctx* ctx, const float* d_stack, int nx, int ny,
   });
 }
 
+mf_status mf_line_scores(mf_ctx* ctx, const float* d_stack, int nx, int ny,
+                         int nz, const float* tilts, const float* shift_x,
+                         const float* shift_y, float axis0, float range,
+                         int num, float* scores, float* d_spectra,
+                         int* line_len, float* best_axis) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(d_stack && tilts && shift_x && shift_y && nx > 0 && ny > 0 &&
+                   nz > 1 && num >= 2,
+               "bad arguments");
+    DeviceScope d(ctx);
+    const AxisSetup a = AxisSetup::make(nx, ny, axis0, range, num);
+    MF_REQUIRE(a.L >= 4, "common-line length too small for this image size");
+    std::vector<double> sx(shift_x, shift_x + nz), sy(shift_y, shift_y + nz);
+    AxisWork ws;
+    const AxisResult r = axis_search(ctx, ctx->stream, d_stack, nx, ny, nz,
+                                     tilts, sx.data(), sy.data(), axis0, range,
+                                     num, ws);
+    if (scores) std::copy(r.scores.begin(), r.scores.end(), scores);
+    if (line_len) *line_len = r.L;
+    if (best_axis) *best_axis = r.axis;
+    if (d_spectra) {
+      // The spectra sit in the lines' buffer at the lines' pitch (L + 2
+      // floats), which is the spectra's own (L/2 + 1 complex values).
+      MF_HIP(hipMemcpyAsync(d_spectra, ws.lines.get(),
+                            (size_t)nz * num * (r.L + 2) * sizeof(float),
+                            hipMemcpyDeviceToDevice, ctx->stream));
+      MF_HIP(hipStreamSynchronize(ctx->stream));
+    }
+  });
+}
+
+mf_status mf_ncc_moments(mf_ctx* ctx, const float* d_ref, const float* d_img,
+                         int w, int h, float ref_tilt, float tilt,
+                         double* moments6, float* cc) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(d_ref && d_img && w >= 2 && h >= 2, "bad arguments");
+    DeviceScope d(ctx);
+    OffsetWork ws;
+    double m[6];
+    ncc_moments(ctx, ctx->stream, d_ref, d_img, w, h, ref_tilt, tilt, ws, m);
+    if (moments6) std::copy(m, m + 6, moments6);
+    if (cc) *cc = offset_cc(m);
+  });
+}
+
+mf_status mf_offset_scan(mf_ctx* ctx, const float* d_corr, int w, int h,
+                         int nz, const float* tilts, float* ccs, float* best) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(d_corr && tilts && best && w >= 2 && h >= 2 && nz >= 2,
+               "bad arguments");
+    DeviceScope d(ctx);
+    OffsetWork ws;
+    std::vector<float> v;
+    *best = offset_search(ctx, ctx->stream, d_corr, w, h, nz, tilts, ws, &v);
+    if (ccs) std::copy(v.begin(), v.end(), ccs);
+  });
+}
+
+mf_status mf_peak2d(mf_ctx* ctx, const float* d_map, int w, int h, int pitch,
+                    int mode, float* shift2) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(d_map && shift2, "bad arguments");
+    DeviceScope d(ctx);
+    DevBuf ws, out;
+    float* dsh = (float*)out.ensure(ctx, 2 * sizeof(float));
+    peak2d(ctx, ctx->stream, d_map, w, h, pitch, mode, ws, dsh);
+    MF_HIP(hipMemcpyAsync(shift2, dsh, 2 * sizeof(float),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    MF_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 void mf_options_default(mf_options* o) {
   if (!o) return;
   o->offset = 0.0f;
@@ -678,6 +754,49 @@ mf_status mf_run(mf_series* s, const mf_options* o) {
                "reproj_mode is 0..5");
     series_run(s, o);
   });
+}
+
+mf_status mf_pa_measurements(mf_series* s, float* meas, float* err) {
+  return sguard(s, [&] {
+    MF_REQUIRE(!s->pa_err.empty(), "no projection-matching sweep has run");
+    if (meas) std::copy(s->pa_meas.begin(), s->pa_meas.end(), meas);
+    if (err) std::copy(s->pa_err.begin(), s->pa_err.end(), err);
+  });
+}
+
+mf_status mf_axis_scores(mf_series* s, float* scores, int cap, int* num) {
+  return sguard(s, [&] {
+    MF_REQUIRE(!s->axis_scores.empty(), "no axis search has run");
+    if (num) *num = (int)s->axis_scores.size();
+    if (scores)
+      std::copy(s->axis_scores.begin(),
+                s->axis_scores.begin() +
+                    std::min((size_t)std::max(cap, 0), s->axis_scores.size()),
+                scores);
+  });
+}
+
+mf_status mf_offset_scores(mf_series* s, float* ccs) {
+  return sguard(s, [&] {
+    MF_REQUIRE(!s->offset_scores.empty(), "no offset scan has run");
+    if (ccs) std::copy(s->offset_scores.begin(), s->offset_scores.end(), ccs);
+  });
+}
+
+mf_status mf_series_trace(mf_series* s, int* kinds, float* values, int cap,
+                          int* count) {
+  return sguard(s, [&] {
+    const int n = (int)s->trace.size();
+    for (int i = 0; i < n && i < cap; i++) {
+      if (kinds) kinds[i] = s->trace[i].first;
+      if (values) values[i] = s->trace[i].second;
+    }
+    if (count) *count = n;
+  });
+}
+
+mf_status mf_series_trace_clear(mf_series* s) {
+  return sguard(s, [&] { s->trace.clear(); });
 }
 
 }  // extern "C"

@@ -478,20 +478,20 @@ __global__ void k_line_score(const float* ccv, int nz, int num, float* score) {
 }
 
 struct OffJob {
-  int ref, img;
+  const float* ref;
+  const float* img;
   float m[3];
 };
 
 // FindOffset::Correlate (FindOffset.cu:204-266): stretch (bilinear, no
 // fill) and the six masked moments over pixels valid in both images.
-__global__ __launch_bounds__(256) void k_offset_moments(const float* corr,
-                                                        int w, int h,
+__global__ __launch_bounds__(256) void k_offset_moments(int w, int h,
                                                         const OffJob* jobs,
                                                         double* out) {
   __shared__ double sh[6][4];
   const OffJob jb = jobs[blockIdx.x];
-  const float* ref = corr + (size_t)jb.ref * w * h;
-  const float* img = corr + (size_t)jb.img * w * h;
+  const float* ref = jb.ref;
+  const float* img = jb.img;
   const float hw = 0.5f * (float)w, hh = 0.5f * (float)h;
   double m[6] = {0, 0, 0, 0, 0, 0};
   for (int i = threadIdx.x; i < w * h; i += blockDim.x) {
@@ -687,10 +687,47 @@ AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   return res;
 }
 
+float offset_cc(const double m[6]) {
+  // FindOffset::Correlate's finish (FindOffset.cu:217-237): moments
+  // rounded to float as the reference reads them back, quirk Q7.
+  float a[6];
+  for (int t = 0; t < 6; t++) a[t] = (float)m[t];
+  for (int t = 0; t < 5; t++) a[t] /= a[5];
+  double s1 = a[2] - a[0], s2 = a[3] - a[1];
+  double cc = a[4] - a[0] * a[1];
+  if (s1 < 0) s1 = 0;
+  if (s2 < 0) s2 = 0;
+  s1 = std::sqrt(s1);
+  s2 = std::sqrt(s2);
+  cc = cc / (s1 * s2 + 1e-30);
+  if (cc < -1) cc = -1;
+  else if (cc > 1) cc = 1;
+  return (float)cc;
+}
+
+void ncc_moments(mf_ctx* ctx, hipStream_t st, const float* d_ref,
+                 const float* d_img, int w, int h, float ref_tilt, float tilt,
+                 OffsetWork& ws, double m6[6]) {
+  OffJob j;
+  j.ref = d_ref;
+  j.img = d_img;
+  const double ds = std::cos(kD2R * ref_tilt) / std::cos(kD2R * tilt);
+  stretch_matrix((float)ds, 0.0f, j.m);
+  char* buf = (char*)ws.buf.ensure(ctx, sizeof(OffJob) + 6 * 8);
+  OffJob* d_job = (OffJob*)buf;
+  double* d_mom = (double*)(buf + sizeof(OffJob));
+  MF_HIP(hipMemcpyAsync(d_job, &j, sizeof(OffJob), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_offset_moments, dim3(1), dim3(256), 0, st, w, h, d_job,
+                     d_mom);
+  MF_HIP(hipGetLastError());
+  MF_HIP(hipMemcpyAsync(m6, d_mom, 6 * 8, hipMemcpyDeviceToHost, st));
+  MF_HIP(hipStreamSynchronize(st));
+}
+
 float offset_search(mf_ctx* ctx, hipStream_t st, const float* d_corr, int w,
                     int h, int nz, const float* tilts, OffsetWork& ws,
                     std::vector<float>* ccs) {
-  // SearchOffset / CalcAverangedCC (FindOffset.cu:157-202) with the tilt copy
+  // SearchOffset / CalcAverangedCC (FindOffset.cu:129-174) with the tilt copy
   // drifting exactly as the reference's member copy does.
   const int num = 31;
   std::vector<float> ang(tilts, tilts + nz);
@@ -714,8 +751,8 @@ float offset_search(mf_ctx* ctx, hipStream_t st, const float* d_corr, int w,
       if (k == z0) continue;
       const int r = (k < z0) ? k + 1 : k - 1;
       OffJob j;
-      j.ref = r;
-      j.img = k;
+      j.ref = d_corr + (size_t)r * w * h;
+      j.img = d_corr + (size_t)k * w * h;
       double ds = std::cos(kD2R * ang[r]) / std::cos(kD2R * ang[k]);
       stretch_matrix((float)ds, 0.0f, j.m);
       jobs.push_back(j);
@@ -729,7 +766,7 @@ float offset_search(mf_ctx* ctx, hipStream_t st, const float* d_corr, int w,
   MF_HIP(hipMemcpyAsync(d_jobs, jobs.data(), nj * sizeof(OffJob),
                         hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_offset_moments, dim3((unsigned)nj), dim3(256), 0, st,
-                     d_corr, w, h, d_jobs, d_mom);
+                     w, h, d_jobs, d_mom);
   MF_HIP(hipGetLastError());
   std::vector<double> mom(nj * 6);
   MF_HIP(hipMemcpyAsync(mom.data(), d_mom, nj * 6 * 8, hipMemcpyDeviceToHost,
@@ -743,20 +780,8 @@ float offset_search(mf_ctx* ctx, hipStream_t st, const float* d_corr, int w,
     int count = 0;
     for (int k = 0; k < nz; k++) {
       if (k == z0s[i]) continue;
-      const double* m = &mom[6 * q++];
-      float a[6];
-      for (int t = 0; t < 6; t++) a[t] = (float)m[t];
-      for (int t = 0; t < 5; t++) a[t] /= a[5];
-      double s1 = a[2] - a[0], s2 = a[3] - a[1];  // quirk Q7
-      double cc = a[4] - a[0] * a[1];
-      if (s1 < 0) s1 = 0;
-      if (s2 < 0) s2 = 0;
-      s1 = std::sqrt(s1);
-      s2 = std::sqrt(s2);
-      cc = cc / (s1 * s2 + 1e-30);
-      if (cc < -1) cc = -1;
-      else if (cc > 1) cc = 1;
-      sum += (float)cc;
+      const float cc = offset_cc(&mom[6 * q++]);
+      sum += cc;
       count++;
     }
     const float mean = sum / (float)count;

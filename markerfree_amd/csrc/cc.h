@@ -75,6 +75,14 @@ void pa_measure_tab(mf_ctx* ctx, hipStream_t st, const CcImg* d_tab,
                     int npair, int w, int h, double* d_mom, float* d_ishift,
                     const PaUpdate* up = nullptr);
 
+// Peak search and sub-pixel fit on a w x h correlation map (row pitch
+// 'pitch' floats) in device memory: mode 0 CalcCC::FindPeak
+// (CalcCC.cu:381-441), mode 1 PAFindPeak (ProjAlign.cpp:325-375); d_shift
+// receives the float2 offset of the peak from the map centre.  The same
+// window, tie rule and fit as the fused correlation's.  Asynchronous.
+void peak2d(mf_ctx* ctx, hipStream_t st, const float* d_map, int w, int h,
+            int pitch, int mode, DevBuf& ws, float* d_shift);
+
 size_t stretch_cc_work_floats(int w, int h, int npair);
 size_t cc_scratch_doubles(int npair, int h);
 

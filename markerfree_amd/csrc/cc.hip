@@ -429,6 +429,80 @@ __device__ Best block_best(Best b, Best* sh) {
   return b;
 }
 
+// The reference's parabolic sub-pixel fit around the integer peak b of a
+// w x h map with row pitch 'pitch' (CalcCC::FindPeak, CalcCC.cu:405-435: the
+// peak clamped to [1, n-2], FP32 terms; PAFindPeak, ProjAlign.cpp:349-370:
+// FP64 terms); returns the fitted peak's offset from the map centre.  When
+// nothing in the window beat the initial value, mode 0 keeps the reference's
+// (0, 0) (clamped to (1, 1)) and mode 1 takes the window's corner, where the
+// reference would read outside the map.
+__device__ float2 peak_fit(const float* cc, int pitch, int w, int h, int mode,
+                           const PeakWin& win, Best b) {
+  int px = mode == 0 ? 0 : win.x0, py = mode == 0 ? 0 : win.y0;
+  if (b.i != 0x7fffffff) {
+    px = b.i % w;
+    py = b.i / w;
+  }
+  auto at = [&](int x, int y) { return cc[(size_t)y * pitch + x]; };
+  double A, B, C, D;
+  if (mode == 0) {
+    if (px < 1) px = 1;
+    else if (px > w - 2) px = w - 2;
+    if (py < 1) py = 1;
+    else if (py > h - 2) py = h - 2;
+    A = (at(px + 1, py) + at(px - 1, py)) * 0.5f - at(px, py);
+    B = (at(px + 1, py) - at(px - 1, py)) * 0.5f;
+    C = (at(px, py + 1) + at(px, py - 1)) * 0.5f - at(px, py);
+    D = (at(px, py + 1) - at(px, py - 1)) * 0.5f;
+  } else {
+    A = (double)(at(px + 1, py) + at(px - 1, py)) * 0.5 - at(px, py);
+    B = (double)(at(px + 1, py) - at(px - 1, py)) * 0.5;
+    C = (double)(at(px, py + 1) + at(px, py - 1)) * 0.5 - at(px, py);
+    D = (double)(at(px, py + 1) - at(px, py - 1)) * 0.5;
+  }
+  double dx = -B / (2 * A + 1e-30), dy = -D / (2 * C + 1e-30);
+  if (fabs(dx) > 1) dx = 0;
+  if (fabs(dy) > 1) dy = 0;
+  const float fpx = (float)(px + dx), fpy = (float)(py + dy);
+  return make_float2(fpx - (float)(w / 2), fpy - (float)(h / 2));
+}
+
+// Standalone peak search of one map (the entry point for maps made
+// elsewhere): argmax partials of nl rows of the window per workgroup, each
+// thread visiting its pixels in increasing row-major order.
+__global__ __launch_bounds__(256) void k_peak_rows(const float* map, int w,
+                                                   int h, int pitch, int mode,
+                                                   int nl, Best* part) {
+  __shared__ Best sh[16];
+  const PeakWin win(w, h, mode);
+  const int y0 = win.y0 + blockIdx.x * nl;
+  const int y1 = min(y0 + nl, win.y1), wx = win.x1 - win.x0;
+  Best best{win.init, 0x7fffffff};
+  const long long n = (long long)max(y1 - y0, 0) * wx;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+    const int y = y0 + (int)(i / wx), x = win.x0 + (int)(i % wx);
+    const float v = map[(size_t)y * pitch + x];
+    if (v > best.v) best = Best{v, y * w + x};
+  }
+  best = block_best(best, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = best;
+}
+
+__global__ __launch_bounds__(256) void k_peak_fit(const float* map, int w,
+                                                  int h, int pitch, int mode,
+                                                  const Best* part, int nparts,
+                                                  float* out) {
+  __shared__ Best sh[16];
+  const PeakWin win(w, h, mode);
+  Best b{win.init, 0x7fffffff};
+  for (int k = threadIdx.x; k < nparts; k += blockDim.x) b = best_of(b, part[k]);
+  b = block_best(b, sh);
+  if (threadIdx.x != 0) return;
+  const float2 r = peak_fit(map, pitch, w, h, mode, win, b);
+  out[0] = r.x;
+  out[1] = r.y;
+}
+
 // Row pass of the inverse 2-D transform (cufftExecC2R, unnormalised) of nl
 // rows of the pair's correlation map per workgroup, written back in place,
 // with the argmax of these rows inside the peak window as one partial.
@@ -510,33 +584,8 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_fit(const CcImgs imgs,
     b = best_of(b, part[(size_t)blockIdx.x * nparts + k]);
   b = block_best(b, sh);
   if (threadIdx.x != 0) return;
-  int px = mode == 0 ? 0 : win.x0, py = mode == 0 ? 0 : win.y0;
-  if (b.i != 0x7fffffff) {
-    px = b.i % w;
-    py = b.i / w;
-  }
-  auto at = [&](int x, int y) { return cc[(size_t)y * pitch + x]; };
-  double A, B, C, D;
-  if (mode == 0) {
-    if (px < 1) px = 1;
-    else if (px > w - 2) px = w - 2;
-    if (py < 1) py = 1;
-    else if (py > h - 2) py = h - 2;
-    A = (at(px + 1, py) + at(px - 1, py)) * 0.5f - at(px, py);
-    B = (at(px + 1, py) - at(px - 1, py)) * 0.5f;
-    C = (at(px, py + 1) + at(px, py - 1)) * 0.5f - at(px, py);
-    D = (at(px, py + 1) - at(px, py - 1)) * 0.5f;
-  } else {
-    A = (double)(at(px + 1, py) + at(px - 1, py)) * 0.5 - at(px, py);
-    B = (double)(at(px + 1, py) - at(px - 1, py)) * 0.5;
-    C = (double)(at(px, py + 1) + at(px, py - 1)) * 0.5 - at(px, py);
-    D = (double)(at(px, py + 1) - at(px, py - 1)) * 0.5;
-  }
-  double dx = -B / (2 * A + 1e-30), dy = -D / (2 * C + 1e-30);
-  if (fabs(dx) > 1) dx = 0;
-  if (fabs(dy) > 1) dy = 0;
-  const float fpx = (float)(px + dx), fpy = (float)(py + dy);
-  const float ishx = fpx - (float)(w / 2), ishy = fpy - (float)(h / 2);
+  const float2 pk = peak_fit(cc, pitch, w, h, mode, win, b);
+  const float ishx = pk.x, ishy = pk.y;
   out[2 * blockIdx.x] = ishx;
   out[2 * blockIdx.x + 1] = ishy;
   if (up.sx) {
@@ -584,7 +633,7 @@ Shape cc_shape(mf_ctx* ctx, const char* family, int w, int h, int npair,
   const int t = (s.nl * std::max(n / 8, 1) + 63) / 64 * 64;
   s.threads = std::max(128, std::min(t, kCcThreads));
   s.threads = tune_get(ctx, key + "threads", s.threads);
-  MF_REQUIRE(s.threads % 64 == 0 && s.threads <= kCcThreads,
+  MF_REQUIRE(s.threads >= 64 && s.threads % 64 == 0 && s.threads <= kCcThreads,
              "workgroup size must be a multiple of 64 up to 256");
   return s;
 }
@@ -747,6 +796,24 @@ void pa_measure_tab(mf_ctx* ctx, hipStream_t st, const CcImg* d_tab,
   // The same normalisation, filter and peak window as pa_measure.
   cc_run(ctx, st, imgs, npair, w, h, 1, 1.0f, 500.0f, 1,
          cc_mask(ctx, w, h, true), d_mom, d_ishift, up);
+}
+
+void peak2d(mf_ctx* ctx, hipStream_t st, const float* d_map, int w, int h,
+            int pitch, int mode, DevBuf& ws, float* d_shift) {
+  MF_REQUIRE(w >= 4 && h >= 4 && pitch >= w && (mode == 0 || mode == 1),
+             "bad peak-search arguments");
+  if (mode == 1)
+    MF_REQUIRE(w > 2 * std::max(w / 20, 3) && h > 2 * std::max(h / 20, 3),
+               "map too small for the projection-matching peak window");
+  const int nl = 8;
+  const int rows = h;  // an upper bound of the window's rows
+  const int nparts = (rows + nl - 1) / nl;
+  Best* part = (Best*)ws.ensure(ctx, nparts * sizeof(Best));
+  hipLaunchKernelGGL(k_peak_rows, dim3(nparts), dim3(256), 0, st, d_map, w, h,
+                     pitch, mode, nl, part);
+  hipLaunchKernelGGL(k_peak_fit, dim3(1), dim3(256), 0, st, d_map, w, h, pitch,
+                     mode, part, nparts, d_shift);
+  MF_HIP(hipGetLastError());
 }
 
 }  // namespace mf

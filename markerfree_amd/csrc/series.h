@@ -43,6 +43,14 @@ struct mf_series {
   std::vector<float> pa_angles;
   mf::CorrGeom pa_geom{};
   mf::DevBuf pa_corr, pa_reproj, pa_work, pa_mom, pa_dev, pa_slab, pa_tab;
+  // The last sweep's measured shift (raw pixels, [nz][2]) and its length
+  // per target; targets the sweep did not measure hold 0.
+  std::vector<float> pa_meas, pa_err;
+  // Scores of the last axis search (per candidate) and offset scan (31).
+  std::vector<float> axis_scores, offset_scores;
+  // Stage trace: one (kind, value) event per stage call (MF_EV_*).
+  std::vector<std::pair<int, float>> trace;
+  void ev(int kind, float value) { trace.emplace_back(kind, value); }
   std::string last_error;
   ~mf_series();
 };

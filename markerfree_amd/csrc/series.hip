@@ -169,6 +169,7 @@ void series_preprocess(mf_series* s) {
   s->axw.t_of = nullptr;  // the stack's pixels are new
   s->axw.neg_of = nullptr;
   s->preprocessed = true;
+  s->ev(MF_EV_PREPROCESS, 0.0f);
 }
 
 void series_coarse_setup(mf_series* s) {
@@ -265,6 +266,7 @@ float series_coarse_iter(mf_series* s) {
     s->sy[i] += sum[1];
   }
   if (s->verbose) printf("maxerr: %8.2f\n\n", maxerr);
+  s->ev(MF_EV_COARSE, maxerr);
   return maxerr;
 }
 
@@ -276,12 +278,14 @@ float series_tilt_axis(mf_series* s, float range, int num) {
   AxisResult r = axis_search(s->ctx, s->st, (const float*)s->stack.get(), s->nx,
                              s->ny, s->nz, s->tilts.data(), s->sx.data(),
                              s->sy.data(), s->rot[z0], range, num, s->axw);
+  s->axis_scores = r.scores;
   float axis = r.axis;
   if (axis < -45 && s->rot[0] == 0) axis += 180.0f;
   if (axis > 135) axis -= 180.0f;
   for (int i = 0; i < s->nz; i++) s->rot[i] = axis;
   if (s->verbose)
     printf("Initial estimate of tilt axes:\nNew tilt axis: %.2f\n\n", axis);
+  s->ev(MF_EV_AXIS, axis);
   return axis;
 }
 
@@ -307,15 +311,18 @@ float series_find_offset(mf_series* s) {
     }
     printf("Tilt offset: %8.2f,  CC: %.4f\n\n", best, mx);
   }
+  s->offset_scores = ccs;
   s->offset += best;
   if (s->verbose)
     printf("Tilt axis angle offset is: %f\n", s->offset);
+  s->ev(MF_EV_OFFSET, best);
   return best;
 }
 
 void series_apply_offset(mf_series* s) {
   // Process::mCoarseAlign (Process.cpp:266-269).
   for (int z = 0; z < s->nz; z++) s->tilts[z] += s->offset;
+  s->ev(MF_EV_APPLY_OFFSET, s->offset);
 }
 
 void series_projalign_setup(mf_series* s, float zshift, int align_z,
@@ -340,6 +347,7 @@ void series_projalign_setup(mf_series* s, float zshift, int align_z,
   s->pa_work.ensure(s->ctx, 4 * (size_t)g.by * padb * sizeof(float));
   s->pa_mom.ensure(s->ctx, cc_scratch_doubles(2, g.by) * sizeof(double));
   s->pa_ready = true;
+  s->ev(MF_EV_PA_SETUP, (float)s->pa_tb);
   if (s->verbose)
     printf("bin:%d   cmpsize:%d,%d\nbinsize:%d, %d\n", s->pa_bin, g.bx / 2 + 1,
            g.by, g.bx, g.by);
@@ -508,27 +516,37 @@ float series_projalign_iter(mf_series* s) {
   if (s->verbose)
     printf("# Projection matching measurements\n# tilt angle  x shift  y shift\n");
   float maxerr = -1e20f;
+  s->pa_meas.assign(2 * nz, 0.0f);
+  s->pa_err.assign(nz, 0.0f);
   for (int t : order) {
     if (s->verbose)
       printf("  %6.2f  %8.2f  %8.2f\n", ang[t], meas[2 * t], meas[2 * t + 1]);
     if (err[t] > maxerr) maxerr = err[t];
+    s->pa_meas[2 * t] = meas[2 * t];
+    s->pa_meas[2 * t + 1] = meas[2 * t + 1];
+    s->pa_err[t] = err[t];
   }
+  s->ev(MF_EV_PA_ITER, maxerr);
   return maxerr;
 }
 
 void series_projalign_once(mf_series* s) {
   // Process::ProjAlignOnce (Process.cpp:292-319); the saved parameters are
   // never restored (quirk Q15), so none are saved here.
+  s->ev(MF_EV_ONCE_BEGIN, 0.0f);
+  int calls = 1;
   float last = series_projalign_iter(s);
   if (s->verbose) printf("fLastErr为: %8.2f\n", last);
   for (int i = 1; i < 10; i++) {
     float err = series_projalign_iter(s);
+    calls++;
     if (s->verbose) printf("fErr为: %8.2f\n", err);
     if (err <= last) {
       if (err < 2.0f) break;
       last = err;
     }
   }
+  s->ev(MF_EV_ONCE_END, (float)calls);
 }
 
 void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by) {

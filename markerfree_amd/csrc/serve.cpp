@@ -165,29 +165,29 @@ struct mf_server {
 
 namespace {
 
-// Runs one series end to end; fills the request's results or error.
-void process(mf_ctx* ctx, PinnedPool* pool, Request& r) {
+// Runs one series end to end and fills the request's results.  Returns
+// false with the message in *err on failure.  It does not touch r.state or
+// the times: the worker publishes those under the server lock, so a reader
+// that sees MF_REQ_DONE also sees the results and t_done.
+bool process(mf_ctx* ctx, PinnedPool* pool, Request& r, std::string* err) {
   mf_series* s = nullptr;
-  mf_status st = mf_series_create(ctx, r.nx, r.ny, r.nz, r.stack.data(),
-                                  r.tilts.data(), &s);
-  if (st != MF_OK) {
-    r.error = mf_last_error(ctx);
-    r.state = MF_REQ_FAILED;
-    return;
+  if (mf_series_create(ctx, r.nx, r.ny, r.nz, r.stack.data(), r.tilts.data(),
+                       &s) != MF_OK) {
+    *err = mf_last_error(ctx);
+    return false;
   }
   auto fail = [&]() {
-    r.error = mf_series_last_error(s);
-    r.state = MF_REQ_FAILED;
+    *err = mf_series_last_error(s);
     mf_series_destroy(s);
+    return false;
   };
   if (mf_run(s, &r.opt) != MF_OK) return fail();
   if (mf_correct(s, r.opt.out_bin, nullptr, &r.bx, &r.by) != MF_OK)
     return fail();
   if (!r.aligned.resize(pool, (size_t)r.bx * r.by * r.nz)) {
     mf_series_destroy(s);
-    r.error = "no page-locked host memory for the aligned stack";
-    r.state = MF_REQ_FAILED;
-    return;
+    *err = "no page-locked host memory for the aligned stack";
+    return false;
   }
   if (mf_correct(s, r.opt.out_bin, r.aligned.data(), &r.bx, &r.by) != MF_OK)
     return fail();
@@ -198,7 +198,7 @@ void process(mf_ctx* ctx, PinnedPool* pool, Request& r) {
       MF_OK)
     return fail();
   mf_series_destroy(s);
-  r.state = MF_REQ_DONE;
+  return true;
 }
 
 void worker_loop(mf_server* sv) {
@@ -231,9 +231,12 @@ void worker_loop(mf_server* sv) {
       r->state = MF_REQ_RUNNING;
       r->t_start = seconds_since(sv->t0);
     }
-    process(sv->ctx, sv->pool.get(), *r);
+    std::string err;
+    const bool ok = process(sv->ctx, sv->pool.get(), *r, &err);
     {
       std::lock_guard<std::mutex> lk(sv->mu);
+      r->state = ok ? MF_REQ_DONE : MF_REQ_FAILED;
+      if (!ok) r->error = err;
       sv->reserved -= r->bytes;
       sv->running--;
       sv->completed++;
@@ -412,11 +415,14 @@ mf_status mf_fetch(mf_server* sv, long long id, double* shift_x,
     r = it->second;
     if (r->state == MF_REQ_QUEUED || r->state == MF_REQ_RUNNING)
       return MF_ESTATE;
-  }
-  if (times3) {
-    times3[0] = r->t_submit;
-    times3[1] = r->t_start;
-    times3[2] = r->t_done;
+    // State and times are published together under the lock; the results
+    // below were written before the state changed and are not modified
+    // after it, so they are read without the lock.
+    if (times3) {
+      times3[0] = r->t_submit;
+      times3[1] = r->t_start;
+      times3[2] = r->t_done;
+    }
   }
   if (r->state == MF_REQ_FAILED) {
     if (err && err_size) {

@@ -532,7 +532,7 @@ void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
   int nl = (int)std::max(1LL, std::min<long long>({want, (long long)fit, 16LL}));
   nl = std::max(1, std::min(tune_get(ctx, key + "lines", nl), fit));
   int threads = tune_get(ctx, key + "threads", kRowThreads);
-  MF_REQUIRE(threads % 64 == 0 && threads <= kRowThreads,
+  MF_REQUIRE(threads >= 64 && threads % 64 == 0 && threads <= kRowThreads,
              "workgroup size must be a multiple of 64 up to 256");
   const size_t lds = nl * per + (size_t)ls * sizeof(float2);
   MF_REQUIRE(lds <= 160 * 1024, "R-weighting row exceeds LDS");

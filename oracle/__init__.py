@@ -202,6 +202,66 @@ def offset_correlate(ref, img, ref_tilt, tilt):
                                            tilt))
 
 
+_sig("orc_offset_moments", None, _fp, _fp, C.c_int, C.c_int, C.c_float,
+     C.c_float, C.POINTER(C.c_double))
+_sig("orc_offset_scan", C.c_float, _fp, C.c_int, C.c_int, C.c_int, _fp, _fp)
+_sig("orc_peak2d", None, _fp, C.c_int, C.c_int, C.c_int, C.c_int, _fp)
+_sig("orc_line_spectra", C.c_int, _fp, C.c_int, C.c_int, C.c_int, _fp, _fp,
+     _fp, C.c_float, C.c_float, C.c_int, _fp)
+_sig("orc_projalign_once_script", C.c_int, _fp, C.c_int)
+
+
+def offset_moments(ref, img, ref_tilt, tilt):
+    """The six masked moments of FindOffset::Correlate (float64[6])."""
+    ref = np.ascontiguousarray(ref, np.float32)
+    img = np.ascontiguousarray(img, np.float32)
+    h, w = ref.shape
+    m = np.zeros(6, np.float64)
+    _lib.orc_offset_moments(_f(ref), _f(img), w, h, ref_tilt, tilt,
+                            m.ctypes.data_as(C.POINTER(C.c_double)))
+    return m
+
+
+def offset_scan(corr, tilts):
+    """SearchOffset over a bin-4 corrected stack [nz][h][w]: (best, ccs)."""
+    corr = np.ascontiguousarray(corr, np.float32)
+    nz, h, w = corr.shape
+    t = np.ascontiguousarray(tilts, np.float32)
+    ccs = np.zeros(31, np.float32)
+    best = _lib.orc_offset_scan(_f(corr), w, h, nz, _f(t), _f(ccs))
+    return float(best), ccs
+
+
+def peak2d(cc_map, mode, w=None):
+    """FindPeak (mode 0) / PAFindPeak (mode 1) on a map [h][pitch] whose
+    first w columns are the map."""
+    cc_map = np.ascontiguousarray(cc_map, np.float32)
+    h, pitch = cc_map.shape
+    w = pitch if w is None else w
+    out = np.zeros(2, np.float32)
+    _lib.orc_peak2d(_f(cc_map), w, h, pitch, mode, _f(out))
+    return float(out[0]), float(out[1])
+
+
+def line_spectra(stack, tilts, sx, sy, axis0, arange, num=100):
+    """Radon + FFT1d line spectra [nz][num][L/2+1] complex64."""
+    stack = np.ascontiguousarray(stack, np.float32)
+    nz, ny, nx = stack.shape
+    a = [np.ascontiguousarray(v, np.float32) for v in (tilts, sx, sy)]
+    L = _lib.orc_line_spectra(_f(stack), nx, ny, nz, *map(_f, a), axis0,
+                              arange, num, None)
+    out = np.zeros((nz, num, L // 2 + 1), np.complex64)
+    _lib.orc_line_spectra(_f(stack), nx, ny, nz, *map(_f, a), axis0, arange,
+                          num, out.view(np.float32).ctypes.data_as(_fp))
+    return out
+
+
+def projalign_once_script(errs):
+    """DoIt calls Process::ProjAlignOnce makes when its sweeps return errs."""
+    e = np.ascontiguousarray(errs, np.float32)
+    return int(_lib.orc_projalign_once_script(_f(e), len(e)))
+
+
 def find_offset(stack, tilts, sx, sy, rot):
     stack = np.ascontiguousarray(stack, np.float32)
     nz, ny, nx = stack.shape
@@ -212,6 +272,10 @@ def find_offset(stack, tilts, sx, sy, rot):
 
 
 # ------------------------------------------------------------------ series
+# Stage trace kinds (oracle.h ORC_EV_*).
+EV_PREPROCESS, EV_COARSE, EV_AXIS, EV_OFFSET, EV_APPLY_OFFSET, EV_PA_SETUP, \
+    EV_PA_ITER, EV_ONCE_BEGIN, EV_ONCE_END = range(1, 10)
+
 class Options(C.Structure):
     _fields_ = [("offset", C.c_float), ("pitch_angle", C.c_float),
                 ("zshift", C.c_float), ("thickness", C.c_int),
@@ -234,6 +298,9 @@ _sig("orc_series_find_offset", C.c_float, _vp)
 _sig("orc_apply_offset", None, _vp)
 _sig("orc_projalign_setup", None, _vp, C.c_float, C.c_int, C.c_int)
 _sig("orc_projalign_iter", C.c_float, _vp)
+_sig("orc_projalign_iter_forced", C.c_float, _vp, _fp, _fp)
+_sig("orc_series_trace", C.c_int, _vp, _ip, _fp, C.c_int)
+_sig("orc_series_trace_clear", None, _vp)
 _sig("orc_pa_z0", C.c_float, _vp)
 _sig("orc_projalign_once", None, _vp)
 _sig("orc_correct_final", None, _vp, C.c_int, _fp, _ip)
@@ -307,8 +374,30 @@ class Series:
     def projalign_iter(self):
         return float(_lib.orc_projalign_iter(self.h))
 
+    def projalign_iter_forced(self, forced=None):
+        """One sweep; 'forced' ([nz][2] raw px) replaces each measured shift
+        before it is applied.  Returns (maxerr, own measurements [nz][2])."""
+        own = np.zeros((self.nz, 2), np.float32)
+        f = None
+        if forced is not None:
+            forced = np.ascontiguousarray(forced, np.float32).reshape(self.nz, 2)
+            f = _f(forced)
+        e = _lib.orc_projalign_iter_forced(self.h, f, _f(own))
+        return float(e), own
+
     def projalign_once(self):
         _lib.orc_projalign_once(self.h)
+
+    def trace(self):
+        """Stage trace [(kind, value), ...] (ORC_EV_* kinds)."""
+        n = _lib.orc_series_trace(self.h, None, None, 0)
+        k = np.zeros(n, np.int32)
+        v = np.zeros(n, np.float32)
+        _lib.orc_series_trace(self.h, k.ctypes.data_as(_ip), _f(v), n)
+        return list(zip(k.tolist(), v.tolist()))
+
+    def trace_clear(self):
+        _lib.orc_series_trace_clear(self.h)
 
     def correct_final(self, out_bin=1):
         bx, by = binned_size(self.nx, self.ny, out_bin)

@@ -143,6 +143,27 @@ void image_lines(const AxisSetup& a, const std::vector<int>& reg,
   }
 }
 
+// Radon + FFT1d of every image (Rotate.cu:397-451): spec [nz][num][L/2+1]
+// complex values; returns L.
+int line_spectra(const float* stack, int nx, int ny, int nz,
+                 const float* tilts, const float* shift_x,
+                 const float* shift_y, float axis0, float range, int num,
+                 std::vector<float>& spec) {
+  const int z0 = zero_tilt_index(tilts, nz, 0.0f);
+  AxisSetup a = axis_setup(nx, ny, axis0, range, num);
+  const int L = a.L, nc = L / 2 + 1;
+  std::vector<int> reg;
+  common_region(a, nx, ny, shift_x[z0], shift_y[z0], reg);
+  const std::vector<float> taper = line_taper(L);
+  spec.assign((size_t)nz * num * nc * 2, 0.0f);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int z = 0; z < nz; z++)
+    image_lines(a, reg, stack + (size_t)z * nx * ny, nx, ny, tilts[z],
+                shift_x[z], shift_y[z], taper,
+                spec.data() + (size_t)z * num * nc * 2);
+  return L;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -165,6 +186,18 @@ extern "C" int orc_common_lines(const float* stack, int nx, int ny, int nz,
   return a.L;
 }
 
+extern "C" int orc_line_spectra(const float* stack, int nx, int ny, int nz,
+                                const float* tilts, const float* shift_x,
+                                const float* shift_y, float axis0, float range,
+                                int num, float* spec) {
+  if (!spec) return axis_setup(nx, ny, axis0, range, num).L;
+  std::vector<float> v;
+  const int L = line_spectra(stack, nx, ny, nz, tilts, shift_x, shift_y,
+                              axis0, range, num, v);
+  std::memcpy(spec, v.data(), v.size() * sizeof(float));
+  return L;
+}
+
 extern "C" float orc_tilt_axis(const float* stack, int nx, int ny, int nz,
                                const float* tilts, const float* shift_x,
                                const float* shift_y, const float* rotate,
@@ -172,17 +205,11 @@ extern "C" float orc_tilt_axis(const float* stack, int nx, int ny, int nz,
                                int* line_size) {
   const int z0 = zero_tilt_index(tilts, nz, 0.0f);
   AxisSetup a = axis_setup(nx, ny, rotate[z0], range, num);
-  const int L = a.L, nc = L / 2 + 1;
+  std::vector<float> spec;
+  const int L = line_spectra(stack, nx, ny, nz, tilts, shift_x, shift_y,
+                             rotate[z0], range, num, spec);
+  const int nc = L / 2 + 1;
   if (line_size) *line_size = L;
-  std::vector<int> reg;
-  common_region(a, nx, ny, shift_x[z0], shift_y[z0], reg);
-  const std::vector<float> taper = line_taper(L);
-  std::vector<float> spec((size_t)nz * num * nc * 2);
-#pragma omp parallel for schedule(dynamic, 1)
-  for (int z = 0; z < nz; z++)
-    image_lines(a, reg, stack + (size_t)z * nx * ny, nx, ny, tilts[z],
-                shift_x[z], shift_y[z], taper,
-                spec.data() + (size_t)z * num * nc * 2);
   // CalcScore / GCC1d (Rotate.cu:489-561); filter exp(-20 (j/(2(n-1)))^2).
   std::vector<float> filt(nc);
   for (int j = 1; j < nc; j++) {

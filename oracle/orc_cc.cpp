@@ -177,6 +177,71 @@ void stretch_plain(const float* in, int w, int h, float stretch,
   stretch_image(in, w, h, w, m, false, out);
 }
 
+// Peak search and parabolic sub-pixel fit on a w x h correlation map with
+// row pitch 'pitch'.  mode 0: CalcCC::FindPeak (CalcCC.cu:381-435), the
+// central window of w*8/20*2 x h*8/20*2 pixels, FP32 fit terms, the integer
+// peak clamped to [1, n-2].  mode 1: ProjAlign::PAFindPeak
+// (ProjAlign.cpp:325-375), the window [max(n/20, 3), n - that), FP64 fit
+// terms.  In both the first maximum in row-major order wins.  shift2 = peak
+// - (w/2, h/2).
+void find_peak(const float* cc, int w, int h, int pitch, int mode,
+               float* shift2) {
+  int x0, y0, x1, y1;
+  float best;
+  if (mode == 0) {
+    const int sx = w * 8 / 20 * 2, sy = h * 8 / 20 * 2;
+    x0 = (w - sx) / 2;
+    y0 = (h - sy) / 2;
+    x1 = x0 + sx;
+    y1 = y0 + sy;
+    best = -1e30f;
+  } else {
+    x0 = std::max(w / 20, 3);
+    y0 = std::max(h / 20, 3);
+    x1 = w - x0;
+    y1 = h - y0;
+    best = -1e20f;
+  }
+  int px = 0, py = 0;
+  for (int y = y0; y < y1; y++)
+    for (int x = x0; x < x1; x++) {
+      const float v = cc[(size_t)y * pitch + x];
+      if (best >= v) continue;
+      px = x;
+      py = y;
+      best = v;
+    }
+  auto at = [&](int x, int y) { return cc[(size_t)y * pitch + x]; };
+  double A, B, C, D;
+  if (mode == 0) {
+    if (px < 1) px = 1;
+    else if (px > w - 2) px = w - 2;
+    if (py < 1) py = 1;
+    else if (py > h - 2) py = h - 2;
+    A = (at(px + 1, py) + at(px - 1, py)) * 0.5f - at(px, py);
+    B = (at(px + 1, py) - at(px - 1, py)) * 0.5f;
+    C = (at(px, py + 1) + at(px, py - 1)) * 0.5f - at(px, py);
+    D = (at(px, py + 1) - at(px, py - 1)) * 0.5f;
+  } else {
+    // The reference reads outside the map when nothing in the window beats
+    // -1e20 (peak left at 0, 0); the window's corner is taken instead.
+    if (best == -1e20f) {
+      px = x0;
+      py = y0;
+    }
+    A = (at(px + 1, py) + at(px - 1, py)) * 0.5 - at(px, py);
+    B = (at(px + 1, py) - at(px - 1, py)) * 0.5;
+    C = (at(px, py + 1) + at(px, py - 1)) * 0.5 - at(px, py);
+    D = (at(px, py + 1) - at(px, py - 1)) * 0.5;
+  }
+  double dx = -B / (2 * A + 1e-30), dy = -D / (2 * C + 1e-30);
+  if (std::fabs(dx) > 1) dx = 0;
+  if (std::fabs(dy) > 1) dy = 0;
+  const float fpx = (float)(px + dx), fpy = (float)(py + dy);
+  shift2[0] = fpx - (float)(w / 2);
+  shift2[1] = fpy - (float)(h / 2);
+}
+
 void stretch_cc(const float* ref, const float* img, int w, int h,
                 float ref_tilt, float tilt, float axis_deg, float bfactor,
                 float* shift2) {
@@ -200,35 +265,7 @@ void stretch_cc(const float* ref, const float* img, int w, int h,
   r2c(pst, w, h, norm);
   spectral_cc(pref.data(), pst.data(), w, h, bfactor);
   orc_fft_c2r_2d(pst.data(), w, h);
-  // FindPeak (CalcCC.cu:381-435) on the w x h map (row pitch 'pitch').
-  const float* cc = pst.data();
-  int sx = w * 8 / 20 * 2, sy = h * 8 / 20 * 2;
-  int x0 = (w - sx) / 2, y0 = (h - sy) / 2;
-  int px = 0, py = 0;
-  float best = -1e30f;
-  for (int y = y0; y < y0 + sy; y++)
-    for (int x = x0; x < x0 + sx; x++) {
-      float v = cc[(size_t)y * pitch + x];
-      if (best >= v) continue;
-      px = x;
-      py = y;
-      best = v;
-    }
-  if (px < 1) px = 1;
-  else if (px > w - 2) px = w - 2;
-  if (py < 1) py = 1;
-  else if (py > h - 2) py = h - 2;
-  auto at = [&](int x, int y) { return cc[(size_t)y * pitch + x]; };
-  double a = (at(px + 1, py) + at(px - 1, py)) * 0.5f - at(px, py);
-  double b = (at(px + 1, py) - at(px - 1, py)) * 0.5f;
-  double c = (at(px, py + 1) + at(px, py - 1)) * 0.5f - at(px, py);
-  double d = (at(px, py + 1) - at(px, py - 1)) * 0.5f;
-  double dx = -b / (2 * a + 1e-30), dy = -d / (2 * c + 1e-30);
-  if (std::fabs(dx) > 1) dx = 0;
-  if (std::fabs(dy) > 1) dy = 0;
-  float fpx = (float)(px + dx), fpy = (float)(py + dy);
-  shift2[0] = fpx - (float)(w / 2);
-  shift2[1] = fpy - (float)(h / 2);
+  find_peak(pst.data(), w, h, pitch, 0, shift2);
 }
 
 void pa_measure(const float* reproj, const float* proj, int w, int h,
@@ -251,30 +288,7 @@ void pa_measure(const float* reproj, const float* proj, int w, int h,
   r2c(b, w, h, 1.0f);
   spectral_cc(a.data(), b.data(), w, h, 500.0f);
   orc_fft_c2r_2d(b.data(), w, h);
-  // PAFindPeak (ProjAlign.cpp:325-375).
-  const float* cc = b.data();
-  int sx = std::max(w / 20, 3), sy = std::max(h / 20, 3);
-  int px = 0, py = 0;
-  float best = -1e20f;
-  for (int y = sy; y < h - sy; y++)
-    for (int x = sx; x < w - sx; x++) {
-      float v = cc[(size_t)y * pitch + x];
-      if (best >= v) continue;
-      px = x;
-      py = y;
-      best = v;
-    }
-  auto at = [&](int x, int y) { return cc[(size_t)y * pitch + x]; };
-  double A = (at(px + 1, py) + at(px - 1, py)) * 0.5 - at(px, py);
-  double B = (at(px + 1, py) - at(px - 1, py)) * 0.5;
-  double C = (at(px, py + 1) + at(px, py - 1)) * 0.5 - at(px, py);
-  double D = (at(px, py + 1) - at(px, py - 1)) * 0.5;
-  double dx = -B / (2 * A + 1e-30), dy = -D / (2 * C + 1e-30);
-  if (std::fabs(dx) > 1) dx = 0;
-  if (std::fabs(dy) > 1) dy = 0;
-  float fpx = (float)(px + dx), fpy = (float)(py + dy);
-  shift2[0] = fpx - (float)(w / 2);
-  shift2[1] = fpy - (float)(h / 2);
+  find_peak(b.data(), w, h, pitch, 1, shift2);
 }
 
 }  // namespace orc
@@ -290,4 +304,9 @@ extern "C" void orc_stretch_cc(const float* ref, const float* img, int bx,
 extern "C" void orc_pa_measure(const float* reproj, const float* proj, int bx,
                                int by, float* shift2) {
   pa_measure(reproj, proj, bx, by, shift2);
+}
+
+extern "C" void orc_peak2d(const float* map, int w, int h, int pitch, int mode,
+                           float* shift2) {
+  find_peak(map, w, h, pitch, mode, shift2);
 }

@@ -6,15 +6,16 @@
 
 namespace orc {
 
-// FindOffset::Correlate (FindOffset.cu:204-266): stretch the image towards
-// its reference (axis 0, no fill), six masked moments over pixels valid in
-// both, and the reference's NCC with quirk Q7 ("std" = sqrt(E[x^2] - E[x])).
-float offset_correlate(const float* ref, const float* img, int w, int h,
-                       float ref_tilt, float tilt) {
+// FindOffset::Correlate (FindOffset.cu:176-238), first half: stretch the
+// image towards its reference (axis 0, no fill, CalcCC::Stretch) and the six
+// masked moments of GConv2D (FindOffset.cu:7-47) over pixels valid in both:
+// sums of v1, v2, v1^2, v2^2, v1 v2 and the count.
+void offset_moments(const float* ref, const float* img, int w, int h,
+                    float ref_tilt, float tilt, double m[6]) {
   double ds = std::cos(kD2R * ref_tilt) / std::cos(kD2R * tilt);
   std::vector<float> st((size_t)w * h);
   stretch_plain(img, w, h, (float)ds, 0.0f, st.data());
-  double m[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 6; i++) m[i] = 0.0;
   for (size_t i = 0; i < (size_t)w * h; i++) {
     float v1 = ref[i];
     if (v1 < -1e10f) continue;
@@ -27,6 +28,11 @@ float offset_correlate(const float* ref, const float* img, int w, int h,
     m[4] += (double)(v1 * v2);
     m[5] += 1.0;
   }
+}
+
+// Second half (FindOffset.cu:221-237): the reference's NCC with quirk Q7
+// ("std" = sqrt(E[x^2] - E[x])), clamped to [-1, 1].
+float offset_cc(const double m[6]) {
   float a[6];
   for (int i = 0; i < 6; i++) a[i] = (float)m[i];
   for (int i = 0; i < 5; i++) a[i] /= a[5];
@@ -43,7 +49,14 @@ float offset_correlate(const float* ref, const float* img, int w, int h,
   return (float)cc;
 }
 
-// SearchOffset + CalcAverangedCC (FindOffset.cu:157-202) over a bin-4
+float offset_correlate(const float* ref, const float* img, int w, int h,
+                       float ref_tilt, float tilt) {
+  double m[6];
+  offset_moments(ref, img, w, h, ref_tilt, tilt, m);
+  return offset_cc(m);
+}
+
+// SearchOffset + CalcAverangedCC (FindOffset.cu:129-174) over a bin-4
 // corrected stack [nz][h][w].  The tilt copy drifts by the float rounding of
 // += offset / -= offset exactly as the reference's member copy does.
 float offset_search(const float* corr, int w, int h, int nz,
@@ -91,7 +104,7 @@ extern "C" float orc_find_offset(const float* stack, int nx, int ny, int nz,
                                  const float* tilts, const float* shift_x,
                                  const float* shift_y, const float* rotate,
                                  float* ccs) {
-  // FOsetup (FindOffset.cu:126-155): bin 4 (quirk Q8), rotation, no random
+  // FOsetup (FindOffset.cu:98-127): bin 4 (quirk Q8), rotation, no random
   // fill, no R-weighting.
   std::vector<float> corr;
   CorrSizes s;
@@ -104,4 +117,15 @@ extern "C" float orc_offset_correlate(const float* ref, const float* img,
                                       int w, int h, float ref_tilt,
                                       float tilt) {
   return offset_correlate(ref, img, w, h, ref_tilt, tilt);
+}
+
+extern "C" void orc_offset_moments(const float* ref, const float* img, int w,
+                                   int h, float ref_tilt, float tilt,
+                                   double* m6) {
+  offset_moments(ref, img, w, h, ref_tilt, tilt, m6);
+}
+
+extern "C" float orc_offset_scan(const float* corr, int w, int h, int nz,
+                                 const float* tilts, float* ccs) {
+  return offset_search(corr, w, h, nz, tilts, ccs);
 }

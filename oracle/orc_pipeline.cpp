@@ -31,6 +31,9 @@ struct orc_series {
   std::vector<float> pa_angles, pa_corr;
   CorrSizes pa_sizes{};
   bool pa_ready = false;
+  // Stage trace: one (kind, value) event per stage call (ORC_EV_*).
+  std::vector<std::pair<int, float>> trace;
+  void ev(int kind, float value) { trace.emplace_back(kind, value); }
 };
 
 namespace {
@@ -148,6 +151,7 @@ extern "C" void orc_preprocess(orc_series* S) {
   }
   S->tilts = S->file_tilts;
   if (rev) std::reverse(S->tilts.begin(), S->tilts.end());
+  S->ev(ORC_EV_PREPROCESS, 0.0f);
 }
 
 extern "C" void orc_get_stack(const orc_series* S, float* out) {
@@ -236,6 +240,7 @@ extern "C" float orc_coarse_iter(orc_series* S) {
     S->sy[i] += sum[1];
   }
   if (S->verbose) printf("maxerr: %8.2f\n\n", maxerr);
+  S->ev(ORC_EV_COARSE, maxerr);
   return maxerr;
 }
 
@@ -246,6 +251,7 @@ extern "C" float orc_series_tilt_axis(orc_series* S, float range, int num) {
   for (int i = 0; i < S->nz; i++) S->rot[i] = axis;
   if (S->verbose)
     printf("Initial estimate of tilt axes:\nNew tilt axis: %.2f\n\n", axis);
+  S->ev(ORC_EV_AXIS, axis);
   return axis;
 }
 
@@ -264,12 +270,14 @@ extern "C" float orc_series_find_offset(orc_series* S) {
     printf("Tilt offset: %8.2f,  CC: %.4f\n\n", best, mx);
   }
   S->offset += best;
+  S->ev(ORC_EV_OFFSET, best);
   return best;
 }
 
 extern "C" void orc_apply_offset(orc_series* S) {
   // Process::mCoarseAlign (Process.cpp:266-269).
   for (int z = 0; z < S->nz; z++) S->tilts[z] += S->offset;
+  S->ev(ORC_EV_APPLY_OFFSET, S->offset);
 }
 
 extern "C" void orc_projalign_setup(orc_series* S, float zshift, int align_z,
@@ -283,6 +291,7 @@ extern "C" void orc_projalign_setup(orc_series* S, float zshift, int align_z,
   S->pa_nproj = nproj;
   S->pa_angles = S->tilts;
   S->pa_ready = true;
+  S->ev(ORC_EV_PA_SETUP, (float)S->pa_tb);
   if (S->verbose)
     printf("bin:%d   cmpsize:%d,%d\nbinsize:%d, %d\n", S->pa_bin,
            S->pa_sizes.bx / 2 + 1, S->pa_sizes.by, S->pa_sizes.bx,
@@ -292,7 +301,15 @@ extern "C" void orc_projalign_setup(orc_series* S, float zshift, int align_z,
 extern "C" float orc_pa_z0(const orc_series* S) { return S->pa_z0; }
 
 extern "C" float orc_projalign_iter(orc_series* S) {
-  // ProjAlign::DoIt / DoItBin (ProjAlign.cpp:205-257).
+  return orc_projalign_iter_forced(S, nullptr, nullptr);
+}
+
+extern "C" float orc_projalign_iter_forced(orc_series* S,
+                                           const float* forced_meas,
+                                           float* own_meas) {
+  // ProjAlign::DoIt / DoItBin (ProjAlign.cpp:127-179).  forced_meas (raw
+  // pixels, [nz][2]) replaces each target's measured shift before it is
+  // applied; own_meas receives the measurement this sweep made itself.
   const int nz = S->nz;
   const CorrSizes& s = S->pa_sizes;
   correct_stack(S->stack.data(), S->nx, S->ny, nz, S->pa_bin, S->sx.data(),
@@ -344,6 +361,14 @@ extern "C" float orc_projalign_iter(orc_series* S) {
     float sh[2] = {ish[0] * binx, ish[1] * biny};
     if (S->verbose) printf("  %6.2f  %8.2f  %8.2f\n", ang[t], sh[0], sh[1]);
     float err = (float)std::sqrt(sh[0] * sh[0] + sh[1] * sh[1]);
+    if (own_meas) {
+      own_meas[2 * t] = sh[0];
+      own_meas[2 * t + 1] = sh[1];
+    }
+    if (forced_meas) {
+      sh[0] = forced_meas[2 * t];
+      sh[1] = forced_meas[2 * t + 1];
+    }
     rot_shift(sh, S->rot[t], sh);
     float ind[2];
     z_induced(S, t, ind);
@@ -361,23 +386,64 @@ extern "C" float orc_projalign_iter(orc_series* S) {
     if (z0 + i < nz) align(z0 + i);
     if (z0 - i >= 0) align(z0 - i);
   }
+  S->ev(ORC_EV_PA_ITER, maxerr);
   return maxerr;
 }
 
-extern "C" void orc_projalign_once(orc_series* S) {
-  // Process::ProjAlignOnce (Process.cpp:292-319); the saved parameters are
-  // never restored (quirk Q15).
-  float last = orc_projalign_iter(S);
-  if (S->verbose) printf("fLastErr: %8.2f\n", last);
+namespace {
+
+// Process::ProjAlignOnce (Process.cpp:292-319): one DoIt, then up to nine
+// more; a sweep whose error does not exceed the last kept one ends the loop
+// when it is below 2 px, else becomes the last kept error.  The parameters
+// saved at that point are never restored (quirk Q15), so none are saved.
+// Returns the number of DoIt calls.
+template <class Iter>
+int projalign_once_loop(Iter iter, int verbose) {
+  float last = iter();
+  int calls = 1;
+  if (verbose) printf("fLastErr: %8.2f\n", last);
   for (int i = 1; i < 10; i++) {
-    float err = orc_projalign_iter(S);
-    if (S->verbose) printf("fErr: %8.2f\n", err);
+    float err = iter();
+    calls++;
+    if (verbose) printf("fErr: %8.2f\n", err);
     if (err <= last) {
       if (err < 2.0f) break;
       last = err;
     }
   }
+  return calls;
 }
+
+}  // namespace
+
+extern "C" void orc_projalign_once(orc_series* S) {
+  S->ev(ORC_EV_ONCE_BEGIN, 0.0f);
+  const int calls =
+      projalign_once_loop([&] { return orc_projalign_iter(S); }, S->verbose);
+  S->ev(ORC_EV_ONCE_END, (float)calls);
+}
+
+extern "C" int orc_projalign_once_script(const float* errs, int n) {
+  int k = 0;
+  return projalign_once_loop(
+      [&] {
+        if (k >= n) return 1e30f;  // past the script: never breaks
+        return errs[k++];
+      },
+      0);
+}
+
+extern "C" int orc_series_trace(const orc_series* S, int* kinds,
+                                float* values, int cap) {
+  const int n = (int)S->trace.size();
+  for (int i = 0; i < n && i < cap; i++) {
+    if (kinds) kinds[i] = S->trace[i].first;
+    if (values) values[i] = S->trace[i].second;
+  }
+  return n;
+}
+
+extern "C" void orc_series_trace_clear(orc_series* S) { S->trace.clear(); }
 
 extern "C" void orc_correct_final(orc_series* S, int out_bin, float* out,
                                   int* out_size) {

@@ -44,3 +44,35 @@ def rel_l2(a, b, mask=None):
         a, b = a[mask], b[mask]
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+# Workloads of the benchmark configurations (seeded; the generator is
+# deterministic across machines, checked by the stack checksums the
+# end-to-end fixtures record).
+@pytest.fixture(scope="session")
+def cfg1():
+    """512^2 x 41 float32, -60:3:60, axis 85.3, shifts U(-20,20), SNR 0.1
+    (run with AlignZ 100)."""
+    from markerfree_amd import synth
+    return synth.generate(synth.PRESETS["small"])
+
+
+@pytest.fixture(scope="session")
+def cfg2():
+    """1024^2 x 61 int16 counts, -60:2:60, axis U(80,95), shifts U(-40,40),
+    SNR 0.05 (run with AlignZ 300)."""
+    from markerfree_amd import synth
+    return synth.generate(synth.PRESETS["single"])
+
+
+@pytest.fixture(scope="session")
+def cfg1_pre(orc, cfg1):
+    """The oracle's preprocessed cfg1 stack and ascending tilts."""
+    O = orc.Series(cfg1.stack, cfg1.tilts)
+    O.preprocess()
+    return O.stack(), O.tilts()
+
+
+def flush_print(*a):
+    """Progress line that reaches the log at once (long GPU tests)."""
+    print(*a, flush=True)

@@ -136,103 +136,20 @@ def test_gpu_stages_match_oracle(ctx, mf, orc, series):
     G.close()
 
 
-def _axis_agrees(orc, O, axis_g, arange, state):
-    """Axis decisions agree when equal within AXIS_TOL, or when the GPU's
-    candidate ties the oracle's best on the oracle's own score curve
-    (relative score gap <= 1e-4; the common-line score is jagged in angle)."""
-    sx, sy, rot, _ = state
-    axis_o, scores, _ = orc.tilt_axis(O.stack(), O.tilts(), sx, sy, rot,
-                                      arange)
-    d = abs(((axis_g - axis_o) + 90.0) % 180.0 - 90.0)
-    if d <= AXIS_TOL:
-        return True, axis_o
-    z0 = int(np.argmin(np.abs(O.tilts())))
-    step = np.float32(arange) / np.float32(99)
-    cand = rot[z0] - step * 100 / 2 + step * np.arange(100)
-    k = int(np.argmin(np.abs(((cand - axis_g) + 90.0) % 180.0 - 90.0)))
-    gap = (scores.max() - scores[k]) / abs(scores.max())
-    return gap <= 1e-4, axis_o
-
-
 @pytest.mark.gpu
 def test_gpu_pipeline_lockstep(ctx, mf, orc, series):
-    """The whole Process::DoIt sequence on the GPU, with the oracle re-seeded
-    from the GPU state before every stage and run on the same inputs:
-    shifts <= 0.05 px per sweep, axis <= 0.05 deg (or a score tie), offset
-    identical, aligned stack <= 1e-6 relative."""
-    s = series
-    nz = len(s.tilts)
-    G = mf.Series(ctx, s.stack, s.tilts)
-    O = orc.Series(s.stack, s.tilts)
-    G.preprocess()
-    O.preprocess()
-    assert rel_l2(G.stack(), O.stack()) <= 1e-6
-
-    def sync():
-        sx, sy, rot, off = G.params()
-        O.set_params(sx, sy, rot, off)
-        O.set_tilts(G.tilts())
-        return sx, sy, rot, off
-
-    def check_shifts(tag):
-        gsx, gsy, _, _ = G.params()
-        osx, osy, _, _ = O.params()
-        d = max(np.abs(gsx - osx).max(), np.abs(gsy - osy).max())
-        assert d <= SHIFT_TOL, (tag, d)
-
-    G.coarse_setup()
-    O.coarse_setup()
-    for i in (1, 2, 3):
-        sync()
-        G.coarse_iter()
-        O.coarse_iter()
-        check_shifts(f"coarse {i}")
-        state = sync()
-        ag = G.tilt_axis(180.0 / i)
-        ok, ao = _axis_agrees(orc, O, ag, 180.0 / i, state)
-        assert ok, ("coarse axis", i, ag, ao)
-    sync()
-    assert G.find_offset() == O.find_offset()
-    G.apply_offset()
-    sx, sy, rot, off = G.params()
-    zero = np.zeros(nz)
-    G.set_params(zero, zero, rot, off)
-    sync()
-    G.projalign_setup(0.0, 96, 10)
-    O.projalign_setup(0.0, 96, 10)
-
-    def once():
-        # Process::ProjAlignOnce with the GPU's errors driving the breaks.
-        sync()
-        last = G.projalign_iter()
-        eo = O.projalign_iter()
-        check_shifts("pa")
-        assert abs(last - eo) <= SHIFT_TOL
-        for _ in range(1, 10):
-            sync()
-            err = G.projalign_iter()
-            eo = O.projalign_iter()
-            check_shifts("pa")
-            assert abs(err - eo) <= SHIFT_TOL
-            if err <= last:
-                if err < 2.0:
-                    break
-                last = err
-
-    once()
-    for i in (1, 2, 3, 4):
-        state = sync()
-        ag = G.tilt_axis(20.0 / i)
-        ok, ao = _axis_agrees(orc, O, ag, 20.0 / i, state)
-        assert ok, ("fine axis", i, ag, ao)
-        if i == 1:
-            once()
-    once()
-    sx, sy, rot, off = sync()
-    al = G.correct(1)
-    oal = O.correct_final(1)
-    assert rel_l2(al, oal) <= 1e-6
-    G.close()
+    """The whole Process::DoIt sequence on the GPU with the oracle re-seeded
+    from the GPU state before every stage (tests/lockstep.py): shifts <= 0.05
+    px per pair and per projection-matching target, axis <= 0.05 deg or a
+    tie of the two score curves, offset identical, aligned stack <= 1e-6
+    relative."""
+    from lockstep import Lockstep
+    L = Lockstep(mf, orc, ctx, series.stack, series.tilts)
+    try:
+        r = L.run(align_z=96)
+    finally:
+        L.close()
+    print("384^2 lockstep:", r.summary())
 
 
 @pytest.mark.gpu

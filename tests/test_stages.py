@@ -193,3 +193,65 @@ def test_gpu_pa_measure_matches_oracle(ctx, orc):
     assert abs(g[0] - o[0]) <= 0.05 and abs(g[1] - o[1]) <= 0.05, (g, o)
     d_r.free()
     d_i.free()
+
+
+# ------------------------------------------------------------ peak search
+
+def _peak_maps():
+    """Correlation-like maps: a smooth peak at a known sub-pixel place, a
+    flat plateau (the first maximum in row-major order must win), a peak on
+    the window's edge, and noise."""
+    rng = np.random.default_rng(9)
+    out = []
+    for h, w, pitch in ((128, 96, 98), (100, 140, 142), (64, 64, 64)):
+        yy, xx = np.mgrid[0:h, 0:w]
+        m = np.exp(-((xx - w / 2 - 3.3) ** 2 + (yy - h / 2 + 5.7) ** 2) / 18.0)
+        out.append((m, pitch))
+        p = np.zeros((h, w))
+        p[h // 2 - 2:h // 2 + 3, w // 2 - 4:w // 2 + 2] = 1.0
+        out.append((p, pitch))
+        e = rng.standard_normal((h, w)) * 0.01
+        e[max(h // 20, 3) + 1, w - max(w // 20, 3) - 1] = 5.0
+        e[(h - h * 8 // 20 * 2) // 2, (w - w * 8 // 20 * 2) // 2] = 4.0
+        out.append((e, pitch))
+    return out
+
+
+def test_oracle_peak_recovers_parabola_vertex(orc):
+    """A separable parabola sampled on the grid: the fit returns its vertex
+    exactly in both windows (the three-point fit is exact for quadratics)."""
+    h, w = 80, 96
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    vx, vy = w / 2 + 2.25, h / 2 - 3.75
+    m = (100.0 - 0.5 * (xx - vx) ** 2 - 0.25 * (yy - vy) ** 2).astype(np.float32)
+    for mode in (0, 1):
+        sx, sy = orc.peak2d(m, mode)
+        assert abs(sx - 2.25) < 1e-4 and abs(sy + 3.75) < 1e-4, (mode, sx, sy)
+
+
+def test_oracle_peak_first_maximum_wins(orc):
+    h, w = 64, 64
+    m = np.zeros((h, w), np.float32)
+    m[30, 40] = m[35, 20] = m[30, 45] = 2.0   # three equal maxima
+    for mode in (0, 1):
+        sx, sy = orc.peak2d(m, mode)
+        # The fit is flat around an isolated spike: dx = dy = 0.
+        assert (sx, sy) == (40 - 32, 30 - 32), (mode, sx, sy)
+
+
+@pytest.mark.gpu
+def test_gpu_peak2d_matches_oracle(ctx, orc):
+    """mf_peak2d against the oracle's FindPeak / PAFindPeak: identical
+    shifts (the same window, tie rule and fit arithmetic on the same
+    floats)."""
+    for m, pitch in _peak_maps():
+        h, w = m.shape
+        buf = np.zeros((h, pitch), np.float32)
+        buf[:, :w] = m
+        buf[:, w:] = 1e9   # pad columns are outside every window
+        d = ctx.array(buf)
+        for mode in (0, 1):
+            want = orc.peak2d(buf, mode, w=w)
+            got = ctx.peak2d(d, w, h, pitch, mode)
+            assert got == want, (h, w, mode, got, want)
+        d.free()

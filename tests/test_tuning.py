@@ -92,3 +92,21 @@ def test_gpu_correct_rweight_independent_of_launch_shape(fresh):
         np.testing.assert_array_equal(d_out.download(), base)
     d_raw.free()
     d_out.free()
+
+
+@pytest.mark.gpu
+def test_gpu_malformed_workgroup_size_is_an_error(fresh, mf):
+    """A cached workgroup size below one wave or not a multiple of 64 is
+    rejected with a clear error, not launched."""
+    h, w = 64, 96
+    ref = blobs(h, w, seed=6)
+    d_r, d_i = fresh.array(ref), fresh.array(ref)
+    for bad in (32, 96 + 1, 512):
+        fresh.tune_set(f"cc_rowsf/{w}x{h}x1/threads", bad)
+        with pytest.raises(mf.MFError, match="workgroup size"):
+            fresh.pa_measure(d_r, d_i, w, h)
+    fresh.tune_set(f"cc_rowsf/{w}x{h}x1/threads", 0)
+    sx, sy = fresh.pa_measure(d_r, d_i, w, h)
+    assert abs(sx) < 0.05 and abs(sy) < 0.05
+    d_r.free()
+    d_i.free()
