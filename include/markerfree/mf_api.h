@@ -213,6 +213,17 @@ mf_status mf_offset_scan(mf_ctx* ctx, const float* d_corr, int w, int h,
 mf_status mf_peak2d(mf_ctx* ctx, const float* d_map, int w, int h, int pitch,
                     int mode, float* shift2);
 
+/* mf_peak2d for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It allocates nothing: its partial results
+ * live in a fixed device array of the library, one per GPU, held until the
+ * process ends.  Maps of up to 32768 rows (h larger: MF_EINVAL).  Every
+ * call on a GPU shares that array, so calls that overlap on one GPU race;
+ * each call writes all it reads, so nothing carries over between calls.
+ * Same results as mf_peak2d. */
+mf_status mf_peak2d_noalloc(mf_ctx* ctx, const float* d_map, int w, int h,
+                            int pitch, int mode, float* shift2);
+
 /* ------------------------------------------------------------- a series */
 
 /* Flags of the reference command line (main/opts.h:59-84, -g and -p). */

@@ -364,6 +364,8 @@ _sig("mf_ncc_moments", C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, C.c_float,
 _sig("mf_offset_scan", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, _fp, _fp,
      _fp)
 _sig("mf_peak2d", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _fp)
+_sig("mf_peak2d_noalloc", C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int,
+     C.c_int, _fp)
 
 # Stage trace kinds (mf_api.h MF_EV_*).
 EV_PREPROCESS, EV_COARSE, EV_AXIS, EV_OFFSET, EV_APPLY_OFFSET, EV_PA_SETUP, \
@@ -484,7 +486,16 @@ def _ctx_peak2d(self, d_map, w, h, pitch, mode):
 Context.line_scores = _ctx_line_scores
 Context.ncc_moments = _ctx_ncc_moments
 Context.offset_scan = _ctx_offset_scan
+def _ctx_peak2d_noalloc(self, d_map, w, h, pitch, mode):
+    """mf_peak2d_noalloc: the same search without any device allocation."""
+    out = np.zeros(2, np.float32)
+    self._check(_lib.mf_peak2d_noalloc(self.h, d_map.ptr, w, h, pitch, mode,
+                                       f32p(out)))
+    return float(out[0]), float(out[1])
+
+
 Context.peak2d = _ctx_peak2d
+Context.peak2d_noalloc = _ctx_peak2d_noalloc
 Context.correct_proj = _ctx_correct_proj
 Context.common_lines = _ctx_common_lines
 Context.stretch_cc = _ctx_stretch_cc

@@ -71,6 +71,7 @@ mf_status mf_ctx_create(int device, size_t budget, mf_ctx** out) {
     MF_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->budget = budget;
     ctx->fft.reset(new FftCache(ctx));
+    ctx->peak_scratch = peak_scratch_address(device);
   });
   if (st != MF_OK) {
     delete ctx;
@@ -530,6 +531,16 @@ mf_status mf_peak2d(mf_ctx* ctx, const float* d_map, int w, int h, int pitch,
     MF_HIP(hipMemcpyAsync(shift2, dsh, 2 * sizeof(float),
                           hipMemcpyDeviceToHost, ctx->stream));
     MF_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+mf_status mf_peak2d_noalloc(mf_ctx* ctx, const float* d_map, int w, int h,
+                            int pitch, int mode, float* shift2) {
+  if (!ctx) return MF_EINVAL;
+  return guard(ctx, [&] {
+    MF_REQUIRE(d_map && shift2, "bad arguments");
+    DeviceScope d(ctx);
+    peak2d_noalloc(ctx, ctx->stream, d_map, w, h, pitch, mode, shift2);
   });
 }
 

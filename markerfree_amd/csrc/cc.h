@@ -83,6 +83,14 @@ void pa_measure_tab(mf_ctx* ctx, hipStream_t st, const CcImg* d_tab,
 void peak2d(mf_ctx* ctx, hipStream_t st, const float* d_map, int w, int h,
             int pitch, int mode, DevBuf& ws, float* d_shift);
 
+// peak2d for callers that cannot allocate: partials and result in a
+// file-scope device array (maps of up to 32768 rows), shift2 to the host;
+// synchronous.  peak_scratch_address(device) returns that array's address
+// on 'device', which must be current; mf_ctx_create keeps it in the context.
+void peak2d_noalloc(mf_ctx* ctx, hipStream_t st, const float* d_map, int w,
+                    int h, int pitch, int mode, float* h_shift);
+void* peak_scratch_address(int device);
+
 size_t stretch_cc_work_floats(int w, int h, int npair);
 size_t cc_scratch_doubles(int npair, int h);
 

@@ -7,6 +7,8 @@
 // pair of a coarse-alignment sweep is processed in the same launches, the map
 // never leaves HBM and the peak search and parabolic fit run on the device.
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -798,22 +800,81 @@ void pa_measure_tab(mf_ctx* ctx, hipStream_t st, const CcImg* d_tab,
          cc_mask(ctx, w, h, true), d_mom, d_ishift, up);
 }
 
-void peak2d(mf_ctx* ctx, hipStream_t st, const float* d_map, int w, int h,
-            int pitch, int mode, DevBuf& ws, float* d_shift) {
+namespace {
+
+void check_peak_args(int w, int h, int pitch, int mode) {
   MF_REQUIRE(w >= 4 && h >= 4 && pitch >= w && (mode == 0 || mode == 1),
              "bad peak-search arguments");
   if (mode == 1)
     MF_REQUIRE(w > 2 * std::max(w / 20, 3) && h > 2 * std::max(h / 20, 3),
                "map too small for the projection-matching peak window");
-  const int nl = 8;
-  const int rows = h;  // an upper bound of the window's rows
-  const int nparts = (rows + nl - 1) / nl;
-  Best* part = (Best*)ws.ensure(ctx, nparts * sizeof(Best));
+}
+
+constexpr int kPeakRowsPerPart = 8;
+
+// The two launches of a standalone peak search: cdiv(h, 8) partials in
+// 'part', the shift in d_out.
+void launch_peak(hipStream_t st, const float* d_map, int w, int h, int pitch,
+                 int mode, Best* part, float* d_out) {
+  const int nparts = cdiv(h, kPeakRowsPerPart);
   hipLaunchKernelGGL(k_peak_rows, dim3(nparts), dim3(256), 0, st, d_map, w, h,
-                     pitch, mode, nl, part);
+                     pitch, mode, kPeakRowsPerPart, part);
   hipLaunchKernelGGL(k_peak_fit, dim3(1), dim3(256), 0, st, d_map, w, h, pitch,
-                     mode, part, nparts, d_shift);
+                     mode, part, nparts, d_out);
   MF_HIP(hipGetLastError());
+}
+
+// Scratch of the allocation-free peak search (mf_peak2d_noalloc): the
+// partials of maps of up to kPeakNoAllocMaxRows rows, then the float2
+// result.  One copy per GPU, held until the process ends; every call on a
+// GPU shares it, so calls that overlap on one GPU race.  Each call writes
+// every partial and the result it reads, so nothing needs clearing.
+constexpr int kPeakNoAllocMaxRows = 32768;
+constexpr int kPeakNoAllocParts = kPeakNoAllocMaxRows / kPeakRowsPerPart;
+constexpr size_t kPeakResultOffset = kPeakNoAllocParts * sizeof(Best);
+static __device__ __align__(256) unsigned char
+    scratch[kPeakResultOffset + 2 * sizeof(float)];
+
+}  // namespace
+
+void peak2d(mf_ctx* ctx, hipStream_t st, const float* d_map, int w, int h,
+            int pitch, int mode, DevBuf& ws, float* d_shift) {
+  check_peak_args(w, h, pitch, mode);
+  Best* part = (Best*)ws.ensure(ctx, cdiv(h, kPeakRowsPerPart) * sizeof(Best));
+  launch_peak(st, d_map, w, h, pitch, mode, part, d_shift);
+}
+
+void* peak_scratch_address(int device) {
+  // The array's device address, looked up once per device with that device
+  // current (each GPU holds its own copy at its own address).
+  static std::mutex mu;
+  static std::map<int, void*> addr;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) return it->second;
+  int cur = 0;
+  MF_HIP(hipGetDevice(&cur));
+  MF_REQUIRE(cur == device, "peak scratch looked up with another device current");
+  void* p = nullptr;
+  MF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  MF_REQUIRE(p != nullptr, "no device address for the peak scratch");
+  addr[device] = p;
+  return p;
+}
+
+void peak2d_noalloc(mf_ctx* ctx, hipStream_t st, const float* d_map, int w,
+                    int h, int pitch, int mode, float* h_shift) {
+  check_peak_args(w, h, pitch, mode);
+  MF_REQUIRE(h <= kPeakNoAllocMaxRows,
+             "map taller than the allocation-free peak search supports "
+             "(32768 rows)");
+  MF_REQUIRE(ctx->peak_scratch != nullptr, "context has no peak scratch");
+  char* base = (char*)ctx->peak_scratch;
+  float* d_out = (float*)(base + kPeakResultOffset);
+  launch_peak(st, d_map, w, h, pitch, mode, (Best*)base, d_out);
+  MF_HIP(hipMemcpyAsync(h_shift, d_out, 2 * sizeof(float),
+                        hipMemcpyDeviceToHost, st));
+  MF_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace mf

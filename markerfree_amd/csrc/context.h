@@ -116,6 +116,9 @@ struct mf_ctx {
   std::unique_ptr<mf::FftCache> fft;
   std::map<std::string, std::unique_ptr<mf::DevBuf>> tables;
   std::map<std::string, int> tune;      // launch-shape tunables
+  // Device address of the allocation-free peak search's scratch array on
+  // this context's device (cc.hip).
+  void* peak_scratch = nullptr;
   // Per-kernel launch facts (ensure_dyn_lds, occupancy): host calls that
   // are not repeated per launch.
   std::map<const void*, size_t> dyn_lds;
