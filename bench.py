@@ -13,15 +13,22 @@ the timed region starts (each step gets its own series).
 Prints one JSON line with the series/hour, the backprojection kernel's
 roofline and a CPU baseline (the oracle on a bounded sample, extrapolated).
 
-Roofline: achieved = the backprojection's algorithmic bytes per target,
-4*(2bx*T*by) slab bytes + 4*n*bx*by sinogram bytes, over the average
-HIP-event-timed launch of the kernel that does the backprojection (events
-around every 8th launch in the timed steps).  In the
-default fused mode that kernel (bprp_fused) also does the ray sums and never
-writes the slab, so 'achieved' is the rate at which it delivers the
-backprojection the materialised form would write to HBM; 'traffic' (from
-the committed FETCH_SIZE/WRITE_SIZE counter summary) shows the bytes it
-actually moves.  --mode materialized times bp_slab, which writes the slab.
+Roofline of the dominant kernel, timed with HIP events around every 8th of
+its launches in the timed steps.  Default (fused) mode: bprp_fused, which
+backprojects and reprojects without a slab in HBM; its algorithmic bytes per
+target are what it must move, the n neighbour images it reads and the
+reprojection it writes, 4*(n+1)*bx*by.  That kernel is bound by VALU issue,
+not HBM: 'valu' reports its issue rate from the committed counter summary
+(SQ_INSTS_VALU over SQ_BUSY_CYCLES), and 'gvoxel_per_s' the backprojection
+it delivers.  --mode materialized times bp_slab, whose algorithmic bytes are
+the slab it writes plus the sinogram rows it reads, 4*(2bx*T*by) + 4*n*bx*by.
+'traffic' is the FETCH_SIZE + WRITE_SIZE counter reading per launch from the
+committed summaries.
+
+'value' counts series whose raw stacks are resident in HBM when the timed
+region starts; 'pcie_inclusive' times as many series again from a
+page-locked host stack to parameters and the aligned stack in host memory
+(the served form's unit).
 """
 from __future__ import annotations
 
@@ -42,23 +49,34 @@ TIME_EVERY = 8  # HIP events around every 8th launch of the timed kernel
 CALIB_BYTES = 1 << 30
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of 'kernel' from the committed counter summary
-    (profiles/traffic_<kernel>_series_1024x1024x61_T300.md, line
-    'traffic_bytes_per_launch: N'), or None when absent."""
-    p = ROOT / "profiles" / f"traffic_{kernel}_series_1024x1024x61_T300.md"
+def profile_value(name, key):
+    """A 'key: value' line of a committed profile summary
+    (profiles/<name>), or None when absent."""
+    p = ROOT / "profiles" / name
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return int(float(line.split(":")[1]))
+        if line.startswith(key + ":"):
+            return float(line.split(":", 1)[1])
     return None
 
 
-def pa_sweep_bytes(mf, tilts, bx, by, tb, nproj):
-    """Algorithmic bytes of the backprojection over one projection-matching
-    sweep: per target 4*(2bx*tb*by) slab bytes written + 4*n*bx*by sinogram
-    bytes read, with n from FindProjRange under the sweep's skip pattern."""
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def pa_sweep_bytes(mf, tilts, bx, by, tb, nproj, fused):
+    """Algorithmic bytes of the dominant kernel over one projection-matching
+    sweep, with n from FindProjRange under the sweep's skip pattern: per
+    target 4*(n+1)*bx*by for the fused kernel (n neighbour images read, the
+    reprojection written), 4*(2bx*tb*by) + 4*n*bx*by for bp_slab (the slab
+    written, the sinogram rows read)."""
     nz = len(tilts)
     z0 = int(np.argmin(np.abs(tilts)))
     skip = np.ones(nz, np.uint8)
@@ -72,7 +90,10 @@ def pa_sweep_bytes(mf, tilts, bx, by, tb, nproj):
     for t in order:
         lo, hi = mf.find_proj_range(tilts, skip, t, nproj)
         n = hi - lo + 1 if lo >= 0 else 0
-        total += 4 * (2 * bx * tb * by) + 4 * n * bx * by
+        if fused:
+            total += 4 * (n + 1) * bx * by
+        else:
+            total += 4 * (2 * bx * tb * by) + 4 * n * bx * by
         voxels += 2 * bx * tb * by
         targets += 1
         skip[t] = 0
@@ -128,7 +149,8 @@ def cpu_baseline(series, pa_targets, pa_sweeps, n_targets=20):
                   nz * t["correct_image"])
     sampled = sum(t.values()) + sum(per)
     return {"value": round(3600.0 / per_series, 3), "unit": "series/hour",
-            "cores": oracle.num_threads(), "kind": "port",
+            "cores": oracle.num_threads(), "cpu_model": cpu_model(),
+            "kind": "port",
             "sample": (f"oracle on the same 1024x1024x61 T=300 series, one of "
                        f"each stage ({sampled:.1f} s): preprocess "
                        f"{t['preprocess']:.2f} s, coarse sweep "
@@ -353,8 +375,8 @@ def main():
     s_last = pool[-1]
     tilts_final = s_last.tilts()
     bx = by = 1024
-    sweep_bytes, sweep_vox, targets = pa_sweep_bytes(mf, tilts_final, bx, by,
-                                                     align_z, nproj)
+    sweep_bytes, sweep_vox, targets = pa_sweep_bytes(
+        mf, tilts_final, bx, by, align_z, nproj, fused=mode != 0)
     sweeps = k_n / max(args.steps, 1) / max(targets, 1)
     avg_ms = k_ms / max(k_timed, 1)
     bytes_per_launch = sweep_bytes / targets
@@ -362,17 +384,22 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     for s in pool:
         s.close()
-    # One more series with its raw stack copied from (page-locked) host
-    # memory inside the timing (the PCIe-inclusive latency; not the reported
-    # value).
+    # As many series again, each from its raw stack in page-locked host
+    # memory to the parameters and the aligned stack in host memory (the
+    # upload inside the timing).
     raw = mf.pinned_empty(series.stack.shape)
     raw[:] = series.stack
-    tp = time.perf_counter()
-    s = mf.Series(ctx, raw, series.tilts)
-    step(s)
+    pcie = []
+    tp0 = time.perf_counter()
+    for _ in range(args.steps):
+        tp = time.perf_counter()
+        s = mf.Series(ctx, raw, series.tilts)
+        step(s)
+        s.params()
+        pcie.append(time.perf_counter() - tp)
+        s.close()
     ctx.synchronize()
-    pcie_ms = 1000.0 * (time.perf_counter() - tp)
-    s.close()
+    pcie_elapsed = time.perf_counter() - tp0
     ctx.close()
 
     value = 3600.0 * args.steps / elapsed
@@ -386,8 +413,14 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(1000.0 * elapsed / args.steps, 2),
         "latency_ms": {"p50": round(1000 * float(np.percentile(lat, 50)), 2),
-                       "p99": round(1000 * float(np.percentile(lat, 99)), 2),
-                       "pcie_inclusive": round(pcie_ms, 2)},
+                       "p99": round(1000 * float(np.percentile(lat, 99)), 2)},
+        "pcie_inclusive": {
+            "series_per_hour": round(3600.0 * len(pcie) / pcie_elapsed, 2),
+            "p50_ms": round(1000 * float(np.percentile(pcie, 50)), 2),
+            "p99_ms": round(1000 * float(np.percentile(pcie, 99)), 2),
+            "series": len(pcie),
+            "unit": "raw stack in page-locked host memory to parameters and "
+                    "aligned stack in host memory"},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -403,20 +436,33 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(kname),
+            "traffic": None,
             "avg_launch_ms": round(avg_ms, 4),
-            "gvoxel_per_s": round(vox_per_launch / (avg_ms * 1e-3) / 1e9, 1),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
+            "gvoxel_per_s": round(vox_per_launch / (avg_ms * 1e-3) / 1e9, 1),
             "launches": int(k_n),
             "timed_launches": int(k_timed),
             "pa_sweeps_per_series": round(sweeps, 2),
         },
     }
-    if achieved > HBM_PEAK_GBS:
+    prof = f"traffic_{kname}_series_1024x1024x61_T300.md"
+    tr = profile_value(prof, "traffic_bytes_per_launch")
+    out["roofline"]["traffic"] = int(tr) if tr is not None else None
+    if mode != 0:
+        cnt = "counters_bprp_free_series_1024x1024x61_T300.md"
+        frac = profile_value(cnt, "valu_issue_frac")
+        clock = profile_value(cnt, "clock_ghz")
+        out["roofline"]["valu"] = {
+            "bound": "valu issue",
+            "frac": frac,
+            "peak": "1024 SIMDs x clock / 4 wave-instructions/s",
+            "clock_ghz": clock,
+            "source": "profiles/" + cnt,
+        }
         out["roofline"]["note"] = (
-            "frac above 1: the fused kernel delivers the backprojection "
-            "faster than writing its slab at HBM peak could; it never writes "
-            "the slab ('traffic' is what it moves) and is bound by VALU issue")
+            "the fused kernel reads n neighbour images and writes one; its "
+            "time goes to VALU issue ('valu'), so its HBM frac is low by "
+            "construction")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(series,
                                            int(k_n / max(args.steps, 1)),

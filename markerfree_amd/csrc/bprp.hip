@@ -14,7 +14,12 @@
 // FP32 evaluation order shared with the CPU oracle (index selection is
 // bit-identical; the reference's own FMA contraction is compiler-chosen):
 //   BP: rho = fma(x, cos_i, fma(z, sin_i, bx/2)); index (int)rho
-//   RP: X = fma(-t', sin, tx), Z = fma(t', cos, tz); index (int)Z, (int)X
+//   RP: X = fma(-t', sin, tx), Z = fma(t', cos, tz); index (int)Z, (int)X;
+//       the ray sum is an FP32 sum of the valid samples in increasing t and
+//       the output sum / count (the reference's FP32 tree order over 512
+//       threads per ray, CalcReproj.cu:68-124, is one admissible order of
+//       the same sum; a fixed sequential order makes every kernel form and
+//       the oracle agree bit for bit).
 //
 // The fused kernel is instantiated for 1..kMaxFixedN reference projections so
 // the per-projection cos/sin/weight tables sit in scalar registers; larger
@@ -324,7 +329,7 @@ __global__ __launch_bounds__(256) void k_rp_slab(const RpParams q) {
   const float iex = (float)(vx - 1), iez = (float)(q.tb - 1);
   int t0, t1;
   rp_sample_range(q, tx, tz, z0, &t0, &t1);
-  double sum = 0.0;
+  float sum = 0.0f;
   int cnt = 0;
 #pragma unroll 4
   for (int t = t0; t < t1; t++) {
@@ -335,11 +340,11 @@ __global__ __launch_bounds__(256) void k_rp_slab(const RpParams q) {
     const int ix = inb ? (int)X : 0, iz = inb ? (int)Z : 0;
     const float v = vol[(size_t)vx * iz + ix];
     const bool ok = inb && (v >= -1e10f);
-    sum += ok ? (double)v : 0.0;
+    sum += ok ? v : 0.0f;
     cnt += ok ? 1 : 0;
   }
   q.out[(size_t)y * q.bx + u] =
-      ((float)cnt < 0.8f * (float)q.L) ? kInvalid : (float)(sum / (double)cnt);
+      ((float)cnt < 0.8f * (float)q.L) ? kInvalid : sum / (float)cnt;
 }
 
 __device__ __forceinline__ int wave_max(int v) {
@@ -450,7 +455,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_fused(const BpParams p,
     const int ta = wave_max(fa), tb = wave_min(fb);
     const bool has_fast = fast_ok && !any_bad && ta < tb;
     const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
-    double sum = 0.0;
+    float sum = 0.0f;
     int cnt = 0;
     // General samples: [t0, a) and [b, t1); per-projection constants from
     // lane registers, not unrolled (a minority of the samples).
@@ -480,7 +485,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_fused(const BpParams p,
         }
         const float v = (wsum < 0.001f) ? kInvalid : acc / wsum;
         const bool ok = inb && (v >= -1e10f);
-        sum += ok ? (double)v : 0.0;
+        sum += ok ? v : 0.0f;
         cnt += ok ? 1 : 0;
       }
     };
@@ -496,7 +501,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_fused(const BpParams p,
           const float zsc = fmaf(vfz, p.s[i], cx);
           acc = fmaf(sv[i * rowp + (int)fmaf(vfx, p.c[i], zsc)], p.w[i], acc);
         }
-        sum += (double)div_by_rcp(acc, p.cnt_all, p.rcp_all);
+        sum += div_by_rcp(acc, p.cnt_all, p.rcp_all);
       }
       cnt += b - a;
     }
@@ -504,7 +509,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_fused(const BpParams p,
     if (live)
       q.out[(size_t)y * bx + u] = ((float)cnt < 0.8f * (float)q.L)
                                       ? kInvalid
-                                      : (float)(sum / (double)cnt);
+                                      : sum / (float)cnt;
   }
 }
 
@@ -608,7 +613,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_tab(const BpParams p,
     const int ta = wave_max(fa), tb = wave_min(fb);
     const bool has_fast = fast_ok && !any_bad && ta < tb;
     const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
-    double sum = 0.0;
+    float sum = 0.0f;
     int cnt = 0;
     // Samples that take the reference's tests.  A skipped sinogram sample
     // (stored as -inf) clears its projection's validity bit when the row
@@ -644,7 +649,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_tab(const BpParams p,
         const float v =
             (wr.x < 0.001f) ? kInvalid : div_by_rcp(acc, wr.x, wr.y);
         const bool ok = inb && (v >= -1e10f);
-        sum += ok ? (double)v : 0.0;
+        sum += ok ? v : 0.0f;
         cnt += ok ? 1 : 0;
       }
     };
@@ -663,7 +668,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_tab(const BpParams p,
           for (int i = 0; i < N; i++)
             acc = fmaf(sv[i * rowp + (int)fmaf(vfx, p.c[i], zs[i])], p.w[i],
                        acc);
-          sum += (double)div_by_rcp(acc, p.cnt_all, p.rcp_all);
+          sum += div_by_rcp(acc, p.cnt_all, p.rcp_all);
         }
         cnt += b - a;
       }
@@ -672,7 +677,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_tab(const BpParams p,
     if (live)
       q.out[(size_t)y * bx + u] = ((float)cnt < 0.8f * (float)q.L)
                                       ? kInvalid
-                                      : (float)(sum / (double)cnt);
+                                      : sum / (float)cnt;
   }
 }
 
@@ -811,11 +816,11 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
     const int ta = wave_max(fa), tb = wave_min(fb);
     const bool has_fast = fast_ok && !any_bad && ta < tb;
     const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
-    double sum[R];
+    float sum[R];
     int cnt[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      sum[r] = 0.0;
+      sum[r] = 0.0f;
       cnt[r] = 0;
     }
     // Samples that take the reference's tests.  With skipped samples in any
@@ -869,7 +874,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
           const float v =
               (wr.x < 0.001f) ? kInvalid : div_by_rcp(acc[r], wr.x, wr.y);
           const bool ok = inb && (v >= -1e10f);
-          sum[r] += ok ? (double)v : 0.0;
+          sum[r] += ok ? v : 0.0f;
           cnt[r] += ok ? 1 : 0;
         }
       }
@@ -896,7 +901,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
             float acc = 0.f;
 #pragma unroll
             for (int i = 0; i < N; i++) acc = fmaf(vals[i][r], p.w[i], acc);
-            sum[r] += (double)div_by_rcp(acc, p.cnt_all, p.rcp_all);
+            sum[r] += div_by_rcp(acc, p.cnt_all, p.rcp_all);
           }
         }
 #pragma unroll
@@ -911,7 +916,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
         if (R * gr + r < p.by)
           q.out[(size_t)y[r] * bx + u] =
               ((float)cnt[r] < lim) ? kInvalid
-                                    : (float)(sum[r] / (double)cnt[r]);
+                                    : sum[r] / (float)cnt[r];
     }
   }
 }
@@ -1049,7 +1054,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
     const int tb = __builtin_amdgcn_readfirstlane(wave_min(fb));
     const bool has_fast = fast_ok && ta < tb;
     const int a = has_fast ? ta : t1, b = has_fast ? tb : t1;
-    double sum[R];
+    float sum[R];
     int cnt[R];
     int nuse = 0;  // samples counted by every row (no skipped samples)
     // Samples that take the reference's tests, on the premise that no sample
@@ -1107,7 +1112,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         const unsigned um = use ? ~0u : 0u;
 #pragma unroll
         for (int r = 0; r < R; r++)
-          sum[r] += (double)__uint_as_float(
+          sum[r] += __uint_as_float(
               __float_as_uint(div_by_rcp(acc[r], ws, rc)) & um);
         nuse += use ? 1 : 0;
       }
@@ -1150,13 +1155,13 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         for (int r = 0; r < R; r++) {
           const float v = (ws[r] < 0.001f) ? kInvalid : acc[r] / ws[r];
           const bool ok = inb && (v >= -1e10f);
-          sum[r] += ok ? (double)v : 0.0;
+          sum[r] += ok ? v : 0.0f;
           cnt[r] += ok ? 1 : 0;
         }
       }
     };
 #pragma unroll
-    for (int r = 0; r < R; r++) sum[r] = 0.0;
+    for (int r = 0; r < R; r++) sum[r] = 0.0f;
     tested(t0, a);
     if (has_fast) {
       for (int t = ta; t < tb; t++) {
@@ -1181,7 +1186,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         }
 #pragma unroll
         for (int r = 0; r < R; r++)
-          sum[r] += (double)div_by_rcp(acc[r], p.cnt_all, p.rcp_all);
+          sum[r] += div_by_rcp(acc[r], p.cnt_all, p.rcp_all);
       }
       nuse += tb - ta;
     }
@@ -1192,7 +1197,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
     if (seen) {
 #pragma unroll
       for (int r = 0; r < R; r++) {
-        sum[r] = 0.0;
+        sum[r] = 0.0f;
         cnt[r] = 0;
       }
       exact(t0, t1);
@@ -1207,7 +1212,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         if (R * gr + r < p.by)
           q.out[(size_t)y[r] * bx + u] =
               ((float)cnt[r] < lim) ? kInvalid
-                                    : (float)(sum[r] / (double)cnt[r]);
+                                    : sum[r] / (float)cnt[r];
     }
   }
 }

@@ -188,10 +188,15 @@ extern "C" void orc_reproject(const float* stack, int bx, int by, int nz,
       }
       if (slab_out)
         std::memcpy(slab_out + (size_t)y * vsz, vol.data(), vsz * 4);
-      // Ray sums accumulate in FP64 in both modes (the reference's FP32 tree
-      // order is unpinned; a sequential FP64 sum is the exact-ish value).
-      reproject_row<double>(vol.data(), bx, tb, c, s, L,
-                            reproj + (size_t)y * bx);
+      // Ray sums: FP32 in increasing sample order in the FP32 mode (the
+      // reference sums in FP32 over a 512-thread tree, CalcReproj.cu:94-117,
+      // whose order is unpinned), FP64 in the FP64 mode.
+      if (acc64)
+        reproject_row<double>(vol.data(), bx, tb, c, s, L,
+                              reproj + (size_t)y * bx);
+      else
+        reproject_row<float>(vol.data(), bx, tb, c, s, L,
+                             reproj + (size_t)y * bx);
     }
   }
 }
