@@ -3,7 +3,7 @@
  * (libmarkerfree_hip.so).
  *
  * The reference exposes no plugin or FFI API: its boundary is the process
- * (Markerfree -i -o -a -g -p -s, main/main.cpp:6-44) and, inside it, the
+ * (Markerfree -i -o -a -g -p -s, main/main.cpp:6-43) and, inside it, the
  * method-layer C++ classes that Process::DoIt drives (main/Process.cpp:34-92).
  * Each entry point below replaces one of those seams; the comment on each
  * names the reference interface (file:line) it stands in for.
@@ -100,14 +100,14 @@ mf_status mf_kernel_time(mf_ctx* ctx, const char* name, double* total_ms,
 
 /* ------------------------------------------------- reprojection (PA core) */
 
-/* CalcReprojFBP::FindProjRange (method/CalcReproj.cu:281-334): the contiguous
+/* CalcReprojFBP::FindProjRange (method/CalcReproj.cu:206-259): the contiguous
  * range [range2[0], range2[1]] of reference projections for 'target'
  * (range2[0] = -1 when none qualifies).  Host-only. */
 mf_status mf_find_proj_range(const float* tilts, int nz,
                              const unsigned char* skip, int target, int nproj,
                              int* range2);
 
-/* CalcReprojFBP::DoIt (method/CalcReproj.cu:255-279) for one target:
+/* CalcReprojFBP::DoIt (method/CalcReproj.cu:180-204) for one target:
  * backproject the in-range rows of the corrected stack d_stack [nz][by][bx]
  * into per-row xz slabs of 2bx x tb voxels (mGBackProj, :26-64) and ray-sum
  * them at tilts[target] (mGForProj, :68-124) into d_reproj [by][bx].
@@ -125,16 +125,16 @@ mf_status mf_reproject(mf_ctx* ctx, const float* d_stack, int bx, int by,
                        int mode, float* d_reproj, float* d_slab);
 
 /* ------------------------------------------------------------------- FFT */
-/* Replace cuFFT as wrapped by cufft2D / cufft1D (method/Cufft1&2.cu:68-206).
+/* Replace cuFFT as wrapped by cufft2D / cufft1D (method/Cufft1&2.cu:17-165).
  * In-place, padded layout: real rows of n values at a pitch of (n/2+1)*2
  * floats, spectra of n/2+1 complex values in the same rows.  Unnormalised
  * like cufftExecR2C/C2R; 'scale' multiplies the forward result (1/(nx*ny)
- * reproduces cufft2D::ForwardFFT with bNorm, Cufft1&2.cu:88-103). */
+ * reproduces cufft2D::ForwardFFT with bNorm, Cufft1&2.cu:47-62). */
 mf_status mf_fft_r2c_2d(mf_ctx* ctx, float* d_buf, int nx, int ny, int nimg,
                         float scale);
 mf_status mf_fft_c2r_2d(mf_ctx* ctx, float* d_buf, int nx, int ny, int nimg);
 /* Batched 1-D transforms of length n on nlines lines 'pitch' floats apart
- * (cufft1D::Forward/Inverse, Cufft1&2.cu:181-206). */
+ * (cufft1D::Forward/Inverse, Cufft1&2.cu:140-165). */
 mf_status mf_fft_r2c_1d(mf_ctx* ctx, float* d_buf, int n, int nlines,
                         int pitch);
 mf_status mf_fft_c2r_1d(mf_ctx* ctx, float* d_buf, int n, int nlines,
@@ -142,7 +142,7 @@ mf_status mf_fft_c2r_1d(mf_ctx* ctx, float* d_buf, int n, int nlines,
 
 /* ----------------------------------------------------- stage-level kernels */
 
-/* CorrTomoStack::mCorrectProj (method/CorrStack.cu:503-530) on one raw
+/* CorrTomoStack::mCorrectProj (method/CorrStack.cu:370-397) on one raw
  * projection d_raw [ny][nx]: rotate by axis_deg (0 when shift_only) and
  * shift (nearest pixel), random fill or invalid outside, bin x bin sum
  * binning, optional R-weighting; d_out receives the unpadded [by][bx] image
@@ -159,12 +159,12 @@ mf_status mf_stretch_cc(mf_ctx* ctx, const float* d_ref, const float* d_img,
                         int bx, int by, float ref_tilt, float tilt,
                         float axis_deg, float bfactor, float* shift2);
 
-/* ProjAlign::MeaAlignProj (method/ProjAlign.cpp:321-375): correlation of a
+/* ProjAlign::MeaAlignProj (method/ProjAlign.cpp:243-278): correlation of a
  * reprojection with a corrected projection; shift2 (host) = ishift. */
 mf_status mf_pa_measure(mf_ctx* ctx, const float* d_reproj,
                         const float* d_proj, int bx, int by, float* shift2);
 
-/* CalcTIltAxis::Radon (method/Rotate.cu:397-428, GRadon :83-136) for the
+/* CalcTIltAxis::Radon (method/Rotate.cu:397-428, GRadon :42-95) for the
  * candidate grid of CLsetup (:339-374) around axis0 over 'range' degrees:
  * the mean line values of every image of d_stack [nz][ny][nx] for every
  * candidate, as [nz][num][L] floats into d_lines (device), L = *line_len.
@@ -226,7 +226,7 @@ mf_status mf_peak2d_noalloc(mf_ctx* ctx, const float* d_map, int w, int h,
 
 /* ------------------------------------------------------------- a series */
 
-/* Flags of the reference command line (main/opts.h:59-84, -g and -p). */
+/* Flags of the reference command line (main/opts.h:154-195, -g and -p). */
 typedef struct {
   float offset;      /* -g field 1: tilt-angle offset, 0 = search it */
   float pitch_angle; /* -g field 2: tilt axis, 0 = search it */
@@ -243,7 +243,7 @@ typedef struct {
                         Both give identical results. */
 } mf_options;
 
-/* Defaults of InitOpts (main/opts.h:110-124): AlignZ 600, OutBin 1, NPROJ 10;
+/* Defaults of InitOpts (main/opts.h:67-81): AlignZ 600, OutBin 1, NPROJ 10;
  * reproj_mode 1, pa_pairs 1. */
 void mf_options_default(mf_options* o);
 
@@ -285,27 +285,27 @@ mf_status mf_set_params(mf_series* s, const double* shift_x,
                         const double* shift_y, const float* rotate,
                         float angle_offset);
 
-/* Transform::Setup / DoIt (method/Transform.cpp:55-99): one coarse
+/* Transform::Setup / DoIt (method/Transform.cpp:24-97): one coarse
  * stretched-CC sweep; maxerr = largest measured pair shift (px). */
 mf_status mf_coarse_setup(mf_series* s);
 mf_status mf_coarse_iter(mf_series* s, float* maxerr);
 /* CalcTIltAxis::DoIt (method/Rotate.cu:292-337): common-line search over
  * 'num' candidates spanning 'range' degrees; sets every rotate[i]. */
 mf_status mf_tiltaxis(mf_series* s, float range, int num, float* axis);
-/* FindOffset::DoIt (method/FindOffset.cu:113-124): best tilt-angle offset
+/* FindOffset::DoIt (method/FindOffset.cu:85-96): best tilt-angle offset
  * of the 31-step scan, added to angle_offset. */
 mf_status mf_find_offset(mf_series* s, float* best);
 /* Adds angle_offset to the tilts (main/Process.cpp:266-269). */
 mf_status mf_apply_offset(mf_series* s);
-/* ProjAlign::Setup (method/ProjAlign.cpp:126-203). */
+/* ProjAlign::Setup (method/ProjAlign.cpp:48-125). */
 mf_status mf_projalign_setup(mf_series* s, float zshift, int align_z,
                              int nproj, int reproj_mode);
-/* ProjAlign::DoIt (method/ProjAlign.cpp:205-257): one projection-matching
+/* ProjAlign::DoIt (method/ProjAlign.cpp:127-179): one projection-matching
  * sweep over all targets; maxerr = largest measured shift (px). */
 mf_status mf_projalign_iter(mf_series* s, float* maxerr);
 /* Process::ProjAlignOnce (main/Process.cpp:292-319). */
 mf_status mf_projalign_once(mf_series* s);
-/* Process::CorrectStack (main/Process.cpp:329-351): the aligned stack
+/* Process::CorrectStack (main/Process.cpp:329-372): the aligned stack
  * [nz][by][bx] at OutBin 'out_bin' to the host (out may be NULL to query
  * bx, by). */
 mf_status mf_correct(mf_series* s, int out_bin, float* out, int* bx, int* by);

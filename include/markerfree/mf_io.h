@@ -7,7 +7,7 @@
  *   MrcStackM           mrc/mrcstack.cpp:5-525 (read modes 0/1/2, write
  *                       float32, InitializeHeader :421-454, UpdateHeader
  *                       :456-500)
- *   Process::ReadAngles main/Process.cpp:190-206
+ *   Process::ReadAngles main/Process.cpp:186-206
  *   SaveParam           method/SaveParam.cpp:15-262 (.txt and .xf)
  */
 #ifndef MARKERFREE_MF_IO_H
@@ -28,9 +28,9 @@ typedef struct {
 
 const char* mfio_last_error(void);
 
-/* MrcStackM::ReadHeader (mrcstack.cpp:44-49). */
+/* MrcStackM::ReadHeader (mrcstack.cpp:43-49). */
 int mfio_mrc_read_info(const char* path, mfio_mrc_info* info);
-/* MrcStackM::ReadSlice for every slice (mrcstack.cpp:52-112): data after
+/* MrcStackM::ReadSlice for every slice (mrcstack.cpp:51-119): data after
  * 1024 + next bytes, converted to float (mode 0 unsigned, 1 signed int16,
  * 2 float32; mode 6 unsigned int16 in addition).  out: nx*ny*nz floats. */
 int mfio_mrc_read(const char* path, float* out);
@@ -38,7 +38,7 @@ int mfio_mrc_read(const char* path, float* out);
  * ingest (reads no other image data). */
 int mfio_mrc_read_slices(const char* path, int z0, int count, float* out);
 
-/* The aligned-stack writer of Process::CorrectStack (Process.cpp:333-351):
+/* The aligned-stack writer of Process::CorrectStack (Process.cpp:329-372):
  * InitializeHeader + SetSize (mrcstack.cpp:421-454), float32 data, then
  * UpdateHeader (mrcstack.cpp:456-500): amin/amax/amean from the middle
  * slice (index (int)(nz * 0.5)) skipping values < -1e10, amax starting at
@@ -47,17 +47,17 @@ int mfio_mrc_read_slices(const char* path, int z0, int count, float* out);
 int mfio_mrc_write(const char* path, const float* data, int nx, int ny,
                    int nz, float stats[3]);
 
-/* Process::ReadAngles (Process.cpp:190-206): whitespace-separated floats
+/* Process::ReadAngles (Process.cpp:186-206): whitespace-separated floats
  * until the first that fails to parse.  Returns the count (<= cap values
  * stored), or -1 if the file cannot be opened. */
 int mfio_read_angles(const char* path, float* out, int cap);
 
-/* SaveParam::GetParam's file name (SaveParam.cpp:125-170): the first ".mrc"
+/* SaveParam::GetParam's file name (SaveParam.cpp:132-136, 150-154): the first ".mrc"
  * in 'outfile' replaced by ".txt" (mode 0) or ".xf" (mode 1), else the
  * suffix appended.  Returns 0, or -1 if 'size' is too small. */
 int mfio_param_path(const char* outfile, int mode, char* buf, size_t size);
 
-/* SaveHeader + SaveAllParam + SaveTime (SaveParam.cpp:179-262): the .txt
+/* SaveHeader + SaveAllParam + SaveTime (SaveParam.cpp:185-262): the .txt
  * parameter file.  All arrays hold nz entries in ascending tilt order;
  * tilts are the angles as read from the file (ascending, no offset). */
 int mfio_save_txt(const char* path, int nx, int ny, int nz,
@@ -65,7 +65,7 @@ int mfio_save_txt(const char* path, int nx, int ny, int nz,
                   const double* shift_y, const float* tilts,
                   float angle_offset, int out_bin, float elapsed_s);
 
-/* SaveXf (SaveParam.cpp:15-55): one line per image, in file order when the
+/* SaveXf (SaveParam.cpp:15-120): one line per image, in file order when the
  * file's tilts descend ('inv' nonzero), with cx = cy = nx/2 (the reference
  * uses the width for both). */
 int mfio_save_xf(const char* path, int nx, int nz, const float* rotate,

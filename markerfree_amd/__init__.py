@@ -322,7 +322,7 @@ _sig("mf_series_upload", C.c_int, _vp, C.c_int, C.c_int, _fp)
 
 
 class Options(C.Structure):
-    """mf_options: the reference's -g / -p flags (main/opts.h:59-84)."""
+    """mf_options: the reference's -g / -p flags (main/opts.h:154-195)."""
     _fields_ = [("offset", C.c_float), ("pitch_angle", C.c_float),
                 ("zshift", C.c_float), ("thickness", C.c_int),
                 ("align_z", C.c_int), ("out_bin", C.c_int),

@@ -86,7 +86,7 @@ struct RadonImg {
   float ox, oy;  // nx/2 + shift_x, ny/2 + shift_y
 };
 
-// GRadon (Rotate.cu:83-136): mean of the nearest non-negative pixels along
+// GRadon (Rotate.cu:42-95): mean of the nearest non-negative pixels along
 // the candidate direction over the cos(tilt)-cropped common region; the sum
 // is sequential in x exactly as the reference thread sums it.  One-dimensional
 // grid over (candidate, line block, image), image slowest and candidate
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(256) void k_transpose(const float* in, float* out,
 
 constexpr int kLineThreads = 256;
 
-// GCalcMean + GRemoveMean + 1-D R2C (Rotate.cu:138-198, 430-451) for nl
+// GCalcMean + GRemoveMean + 1-D R2C (Rotate.cu:97-157, 430-451) for nl
 // lines per workgroup, in place (pitch = L + 2 floats).
 __global__ __launch_bounds__(kLineThreads) void k_line_fft(
     float* lines, int nlines, int L, int pitch, const float* taper,
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(kLineThreads) void k_line_fft(
 }
 
 // Sum over images of each candidate's spectrum, sequential in z as the
-// reference's GSumCmp accumulation (Rotate.cu:499-504).
+// reference's GSumCmp accumulation (Rotate.cu:498-503).
 __global__ void k_line_sum(const float2* spec, int nz, int num, int nc,
                            int cpitch, float2* S) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -441,7 +441,7 @@ __global__ void k_line_sum(const float2* spec, int nz, int num, int nc,
   S[i] = acc;
 }
 
-// GConv/GSum (Rotate.cu:220-292) for one (candidate, image): filtered
+// GConv/GSum (Rotate.cu:179-274) for one (candidate, image): filtered
 // normalised correlation of the line with the sum of the others.
 __global__ __launch_bounds__(256) void k_line_cc(const float2* spec,
                                                  const float2* S, int nz,
@@ -483,7 +483,7 @@ struct OffJob {
   float m[3];
 };
 
-// FindOffset::Correlate (FindOffset.cu:204-266): stretch (bilinear, no
+// FindOffset::Correlate (FindOffset.cu:176-238): stretch (bilinear, no
 // fill) and the six masked moments over pixels valid in both images.
 __global__ __launch_bounds__(256) void k_offset_moments(int w, int h,
                                                         const OffJob* jobs,
@@ -688,7 +688,7 @@ AxisResult axis_search(mf_ctx* ctx, hipStream_t st, const float* d_stack,
 }
 
 float offset_cc(const double m[6]) {
-  // FindOffset::Correlate's finish (FindOffset.cu:217-237): moments
+  // FindOffset::Correlate's finish (FindOffset.cu:218-235): moments
   // rounded to float as the reference reads them back, quirk Q7.
   float a[6];
   for (int t = 0; t < 6; t++) a[t] = (float)m[t];

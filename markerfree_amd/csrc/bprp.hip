@@ -2,7 +2,7 @@
 // matching: the roofline-graded kernels of the alignment path.
 //
 // Reference: CalcReprojFBP (method/CalcReproj.cu).  The reference launches one
-// backprojection and one reprojection per image row (CalcReproj.cu:265-269,
+// backprojection and one reprojection per image row (CalcReproj.cu:190-194,
 // by x (NPROJ+3) runtime calls per target); here one launch covers every row
 // of a target.
 //
@@ -1305,7 +1305,7 @@ const Kernels& kernels() {
 
 void find_proj_range(const float* tilts, int nz, const unsigned char* skip,
                      int target, int nproj, int range[2]) {
-  // CalcReproj.cu:281-334, float/double semantics kept.
+  // CalcReproj.cu:206-259, float/double semantics kept.
   const float kRange = 20.5f, kStretch = 1.2f;
   const float tilt = tilts[target];
   const float fc = (float)std::cos(tilt * kD2R);

@@ -10,7 +10,7 @@
 namespace mf {
 
 // One stretched pair: reference image and image indices into an image array,
-// and the inverse stretch matrix of CalcCC::Stretch (CalcCC.cu:318-335).
+// and the inverse stretch matrix of CalcCC::Stretch (CalcCC.cu:318-345).
 struct PairDesc {
   int ref, img;
   float m[3];
@@ -35,7 +35,7 @@ void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
                       DevBuf& tab, float* d_shift);
 
 // The per-target parameter update of a projection-matching step
-// (ClacAlignProj, ProjAlign.cpp:276-296; RotShift, :448-457), applied by the correlation's fit for map
+// (ClacAlignProj, ProjAlign.cpp:200-221; RotShift, :448-457), applied by the correlation's fit for map
 // j to target t[j]: meas = ishift * bin, err = |meas|, shift = RotShift(meas)
 // + Z-induced shift, (sx, sy) += shift, and the target's correction
 // parameters cp[t] refreshed.  sx == nullptr: no update.

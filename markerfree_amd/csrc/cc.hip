@@ -214,7 +214,7 @@ __device__ void cc_mean_sd(const double* part, unsigned b, int w, int h,
   }
 }
 
-// mNormalize / PANorm (GNormalize, CalcCC.cu:137-147; ProjAlign.cpp:272-312)
+// mNormalize / PANorm (GNormalize, CalcCC.cu:137-147; ProjAlign.cpp:295-312)
 // fused with GRoundEdge (mask table) and the row pass of the 2-D R2C: nl rows
 // of image blockIdx.y per workgroup are normalised (invalid pixels and masked
 // ones become 0), transformed and unpacked into the padded work image.
@@ -432,7 +432,7 @@ __device__ Best block_best(Best b, Best* sh) {
 }
 
 // The reference's parabolic sub-pixel fit around the integer peak b of a
-// w x h map with row pitch 'pitch' (CalcCC::FindPeak, CalcCC.cu:405-435: the
+// w x h map with row pitch 'pitch' (CalcCC::FindPeak, CalcCC.cu:406-435: the
 // peak clamped to [1, n-2], FP32 terms; PAFindPeak, ProjAlign.cpp:349-370:
 // FP64 terms); returns the fitted peak's offset from the map centre.  When
 // nothing in the window beat the initial value, mode 0 keeps the reference's
@@ -783,7 +783,7 @@ void pa_measure(mf_ctx* ctx, hipStream_t st, const float* d_reproj,
   imgs.src1 = d_proj;
   imgs.pitch1 = w;
   imgs.dst1 = d_work + img;
-  // PANorm: std := 1 (ProjAlign.cpp:272-293); unnormalised transforms and
+  // PANorm: std := 1 (ProjAlign.cpp:295-312); unnormalised transforms and
   // B = 500 (:258-271); PAFindPeak window (:325-375).
   cc_run(ctx, st, imgs, 1, w, h, 1, 1.0f, 500.0f, 1, cc_mask(ctx, w, h, true),
          d_mom, d_ishift, up);

@@ -41,7 +41,7 @@ struct Options {
                         // the materialised slab
 };
 
-// UsageDual (opts.h:43-64).
+// UsageDual (opts.h:43-65).
 void usage() {
   std::cout << "[-i INPUT FILENAME]\n"
             << "    MRC file for reconstruction\n"
@@ -65,7 +65,7 @@ void usage() {
             << "    Help\n";
 }
 
-// PrintOpts (opts.h:86-103).  iter/step are uninitialised in the reference;
+// PrintOpts (opts.h:83-100).  iter/step are uninitialised in the reference;
 // here they print 0.
 void print_opts(const Options& o) {
   std::cout << "pitch_angle = " << o.pitch_angle << "\n"
@@ -91,7 +91,7 @@ std::string first_word(const char* s) {
   return w;
 }
 
-// GetOpts (opts.h:105-243): 1 = run, 0 = help printed, -1 = error.
+// GetOpts (opts.h:102-243): 1 = run, 0 = help printed, -1 = error.
 int get_opts(int argc, char** argv, Options* o) {
   static struct option longopts[] = {
       {"help", no_argument, nullptr, 'h'},
@@ -244,7 +244,7 @@ int run(const Options& o) {
     return 0;
   }
   const int nx = info.nx, ny = info.ny, nz = info.nz;
-  // ReadAngles (Process.cpp:58-68, 190-206).
+  // ReadAngles (Process.cpp:58-68, 186-206).
   std::vector<float> angles(nz);
   const int na = mfio_read_angles(o.angle.c_str(), angles.data(), nz);
   if (na < 0) {
@@ -366,7 +366,7 @@ int run(const Options& o) {
 
   const double elapsed = now_s() - t_start;
   if (o.align_z != 0) {
-    // CollectParam (Process.cpp:396-416) + SaveParam::GetParam.
+    // CollectParam (Process.cpp:399-418) + SaveParam::GetParam.
     std::vector<double> sx(nz), sy(nz);
     std::vector<float> rot(nz), tilts(angles);
     float offset = 0;

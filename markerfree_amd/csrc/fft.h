@@ -1,7 +1,7 @@
 // Hand-written FFTs for gfx950: mixed-radix Stockham transforms staged in LDS,
 // Bluestein's chirp-z for lengths with a prime factor above kMaxDirectPrime,
 // and the real<->complex packing used by the in-place padded layout that the
-// reference gets from cuFFT (method/Cufft1&2.cu:68-117, :150-206).
+// reference gets from cuFFT (method/Cufft1&2.cu:17-76, :109-165).
 #pragma once
 #include <map>
 #include <memory>
@@ -54,7 +54,7 @@ int fft_lines_per_group(const FftDevPlan& p, size_t lds_budget, int max_lines);
 // 2-D in-place R2C of an nx x ny image with row pitch pad_x(nx) floats,
 // batched over nimg images spaced img_stride floats apart; the result is
 // multiplied by 'scale' (1 for cufftExecR2C, 1/(nx*ny) for the reference's
-// bNorm option, Cufft1&2.cu:88-103).
+// bNorm option, Cufft1&2.cu:47-62).
 void fft_r2c_2d(mf_ctx* ctx, hipStream_t st, float* buf, int nx, int ny,
                 int nimg, size_t img_stride, float scale);
 // 2-D in-place C2R (unnormalised, cufftExecC2R).

@@ -4,7 +4,7 @@
 // Differences in mechanism from the reference, none in result:
 //  - the preprocessed stack stays in HBM (the reference writes it to buf.mrc
 //    and re-reads one slice per correction, Process.cpp:161-170,
-//    CorrStack.cu:515);
+//    CorrStack.cu:382);
 //  - during a projection-matching sweep the alignment parameters live on the
 //    device (FP64) and each target is measured, applied and re-corrected
 //    without a host round trip; the host reads them back once per sweep;
@@ -173,7 +173,7 @@ void series_preprocess(mf_series* s) {
 }
 
 void series_coarse_setup(mf_series* s) {
-  // Transform::Setup (Transform.cpp:55-89).
+  // Transform::Setup (Transform.cpp:24-58).
   MF_REQUIRE(s->preprocessed, "preprocess the series first");
   int bx = (int)(s->nx / 1024.0f + 0.1f), by = (int)(s->ny / 1024.0f + 0.1f);
   s->tr_bin = std::max(std::max(bx, by), 1);
@@ -185,7 +185,7 @@ void series_coarse_setup(mf_series* s) {
 }
 
 float series_coarse_iter(mf_series* s) {
-  // Transform::DoIt / Measure / UpdateShift (Transform.cpp:91-162).
+  // Transform::DoIt / Measure / UpdateShift (Transform.cpp:60-131).
   if (!s->tr_ready) throw Error(MF_ESTATE, "coarse alignment not set up");
   const int nz = s->nz;
   const CorrGeom g = CorrGeom::make(s->nx, s->ny, s->tr_bin);
@@ -236,7 +236,7 @@ float series_coarse_iter(mf_series* s) {
     s->psy[i] -= s->psy[z0];
   }
   // Accumulate outward from the zero tilt, un-stretching the running sum
-  // into each image's frame (UnStretch, Transform.cpp:164-187), in FP64.
+  // into each image's frame (UnStretch, Transform.cpp:133-157), in FP64.
   auto unstretch = [&](int ref, int f, double* sum) {
     double ds = std::cos(s->tr_angles[ref] * kD2R) / std::cos(s->tr_angles[f] * kD2R);
     double d2t = 2 * kD2R * s->rot[f];
@@ -290,7 +290,7 @@ float series_tilt_axis(mf_series* s, float range, int num) {
 }
 
 float series_find_offset(mf_series* s) {
-  // FindOffset::DoIt / FOsetup (FindOffset.cu:113-155): bin 4 (quirk Q8),
+  // FindOffset::DoIt / FOsetup (FindOffset.cu:85-127): bin 4 (quirk Q8),
   // rotation, no random fill, no R-weighting.
   MF_REQUIRE(s->preprocessed, "preprocess the series first");
   const CorrGeom g = CorrGeom::make(s->nx, s->ny, 4);
@@ -327,7 +327,7 @@ void series_apply_offset(mf_series* s) {
 
 void series_projalign_setup(mf_series* s, float zshift, int align_z,
                             int nproj) {
-  // ProjAlign::Setup / Setsize (ProjAlign.cpp:126-203).
+  // ProjAlign::Setup / Setsize (ProjAlign.cpp:48-125).
   MF_REQUIRE(s->preprocessed, "preprocess the series first");
   MF_REQUIRE(align_z > 0, "AlignZ must be > 0");
   MF_REQUIRE(nproj > 0 && nproj <= kMaxRefProj, "NPROJ out of range");
@@ -354,7 +354,7 @@ void series_projalign_setup(mf_series* s, float zshift, int align_z,
 }
 
 float series_projalign_iter(mf_series* s) {
-  // ProjAlign::DoIt / DoItBin (ProjAlign.cpp:205-257).
+  // ProjAlign::DoIt / DoItBin (ProjAlign.cpp:127-179).
   if (!s->pa_ready) throw Error(MF_ESTATE, "projection matching not set up");
   mf_ctx* ctx = s->ctx;
   const int nz = s->nz;
@@ -471,7 +471,7 @@ float series_projalign_iter(mf_series* s) {
   std::vector<int> order;
   int paired = 0;
   for (const Step& st : steps) {
-    // ClacAlignProj (ProjAlign.cpp:259-319), for one target or a pair.
+    // ClacAlignProj (ProjAlign.cpp:181-241), for one target or a pair.
     const int n = st.b >= 0 ? 2 : 1;
     const int ts[2] = {st.a, st.b};
     for (int k = 0; k < n; k++) {
@@ -550,7 +550,7 @@ void series_projalign_once(mf_series* s) {
 }
 
 void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by) {
-  // Process::CorrectStack (Process.cpp:329-351): rotation + shift, random
+  // Process::CorrectStack (Process.cpp:329-372): rotation + shift, random
   // fill, no R-weighting, OutBin sum binning.
   MF_REQUIRE(s->preprocessed, "preprocess the series first");
   MF_REQUIRE(out_bin >= 1, "OutBin must be >= 1");

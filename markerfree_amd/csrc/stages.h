@@ -8,7 +8,7 @@
 namespace mf {
 
 // ---------------------------------------------------------------- sizes
-// One CorrTomoStack configuration (method/CorrStack.cu:419-461).
+// One CorrTomoStack configuration (method/CorrStack.cu:298-328).
 struct CorrGeom {
   int nx, ny;   // raw image
   int cx;       // corrected width ((pad_x(nx)/2 - 1) * 2)
@@ -42,7 +42,7 @@ void scale_slices(mf_ctx* ctx, hipStream_t st, float* d, size_t plane, int nz,
                   DevBuf& tmp, const float* h_scale);
 
 // ----------------------------------------------------------- correction
-// mGCorrect + mGRandom + mGBinImage (CorrStack.cu:116-312) for 'count'
+// mGCorrect + mGRandom + mGBinImage (CorrStack.cu:40-63, 116-179) for 'count'
 // projections starting at index z0 of the raw stack; params from d_proj
 // (device CorrProj[nz]).  Writes binned images with row pitch out_pitch and
 // image stride out_stride (floats).
@@ -52,7 +52,7 @@ void correct_bin(mf_ctx* ctx, hipStream_t st, const float* d_raw,
                  const CorrGeom& g, const CorrProj* d_proj, int z0, int count,
                  bool random_fill, float* d_out, int out_pitch,
                  size_t out_stride, const float* d_raw_t = nullptr);
-// RWeight (CorrStack.cu:532-595) + Unpad: each padded binned row (pitch
+// RWeight (CorrStack.cu:415-462) + Unpad: each padded binned row (pitch
 // pad_x(bx)) is R2C'd, ramp-filtered, C2R'd and written unpadded to d_out.
 void rweight_rows(mf_ctx* ctx, hipStream_t st, const float* d_pad, int bx,
                   int nrows, float* d_out);

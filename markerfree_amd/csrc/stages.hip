@@ -262,7 +262,7 @@ __global__ __launch_bounds__(kRowThreads) void k_rweight(
   for (LineIter it(xs); it.l < nl; it.next()) {
     int l = it.l, k = it.k;
     r2c_unpack(Z + l * ls, N, P.half, 1.0f, k, [&](int kk, float2 v) {
-      // GRWeight (CorrStack.cu:564-579): x /= N, then x *= ramp.
+      // GRWeight (CorrStack.cu:431-446): x /= N, then x *= ramp.
       const float f = filt[kk];
       X[l * xs + kk] = make_float2((v.x / fn) * f, (v.y / fn) * f);
     });

@@ -5,9 +5,9 @@
 // so projections are exact (no voxel projector).  The aligned projection at
 // tilt t places a blob centred at (x0, y0, z0) at detector position
 // u0 = x0 cos t + z0 sin t, v0 = y0 -- the geometry the reference's
-// backprojector assumes (method/CalcReproj.cu:117-127).  The raw image is the
+// backprojector assumes (method/CalcReproj.cu:37-50).  The raw image is the
 // exact inverse of the reference's correction mGCorrect
-// (method/CorrStack.cu:275-312): raw(q) = P(R(axis)^-1 (q - shift - c)).
+// (method/CorrStack.cu:142-179): raw(q) = P(R(axis)^-1 (q - shift - c)).
 //
 // Random numbers: PCG32 and Box-Muller written here, so every platform draws
 // the same series.

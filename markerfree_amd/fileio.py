@@ -4,7 +4,7 @@ declared in ``include/markerfree/mf_io.h``) and a numpy MRC writer for
 preparing inputs.
 
 References: mrc/mrcheader.h:23-96, mrc/mrcstack.cpp:5-525,
-main/Process.cpp:190-206 (ReadAngles), method/SaveParam.cpp:15-262.
+main/Process.cpp:186-206 (ReadAngles), method/SaveParam.cpp:15-262.
 """
 from __future__ import annotations
 

@@ -70,7 +70,7 @@ void common_region(const AxisSetup& a, int nx, int ny, float sx, float sy,
   }
 }
 
-// Taper of GRemoveMean (Rotate.cu:185-198): (1 - (0.5(1-cos(pi r)))^100)^2.
+// Taper of GRemoveMean (Rotate.cu:144-157): (1 - (0.5(1-cos(pi r)))^100)^2.
 std::vector<float> line_taper(int L) {
   std::vector<float> t(L);
   const float half = 0.5f * (float)L;
@@ -83,7 +83,7 @@ std::vector<float> line_taper(int L) {
   return t;
 }
 
-// GRadon (Rotate.cu:83-136) of one image: the mean line values of candidate
+// GRadon (Rotate.cu:42-95) of one image: the mean line values of candidate
 // k, L values into line.
 void radon_lines(const AxisSetup& a, const std::vector<int>& reg,
                  const float* img, int nx, int ny, float tilt, float sx,
@@ -126,7 +126,7 @@ void image_lines(const AxisSetup& a, const std::vector<int>& reg,
   std::vector<float> line(L);
   for (int k = 0; k < num; k++) {
     radon_lines(a, reg, img, nx, ny, tilt, sx, sy, k, line.data());
-    // GCalcMean + GRemoveMean (Rotate.cu:138-198).
+    // GCalcMean + GRemoveMean (Rotate.cu:97-157).
     double ms = 0.0;
     int mc = 0;
     for (int y = 0; y < L; y++)

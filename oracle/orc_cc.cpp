@@ -8,7 +8,7 @@
 namespace orc {
 
 void stretch_matrix(float stretch, float axis_deg, float m[3]) {
-  // CalcCC::Stretch (CalcCC.cu:318-335).
+  // CalcCC::Stretch (CalcCC.cu:318-345).
   double d2t = 2 * kD2R * axis_deg;
   double s2 = std::sin(d2t), c2 = std::cos(d2t);
   double P = 0.5 * ((double)stretch + 1), M = 0.5 * ((double)stretch - 1);

@@ -48,7 +48,7 @@ void correct_one(const float* raw, const CorrSizes& s, float shift_x,
                  float shift_y, float axis_deg, bool shift_only,
                  bool random_fill, bool rweight, float* out) {
   const int nx = s.nx, ny = s.ny;
-  // mCorrectProj (CorrStack.cu:503-518): angle in radians as float.
+  // mCorrectProj (CorrStack.cu:370-397): angle in radians as float.
   float theta = shift_only ? 0.0f : (float)(axis_deg * kD2R);
   const float c = fcos(theta), sn = fsin(theta);
   std::vector<float> corr((size_t)s.pad * ny, 0.0f);
@@ -59,7 +59,7 @@ void correct_one(const float* raw, const CorrSizes& s, float shift_x,
 #pragma omp parallel for schedule(static)
   for (int y = 0; y < ny; y++) {
     for (int i = 0; i < s.cx; i++) {
-      // mGCorrect (CorrStack.cu:275-312).
+      // mGCorrect (CorrStack.cu:142-179).
       float px = (float)i + 0.5f - hx;
       float py = (float)y + 0.5f - hy;
       float qx = px * c - py * sn;
@@ -77,7 +77,7 @@ void correct_one(const float* raw, const CorrSizes& s, float shift_x,
       corr[(size_t)y * s.pad + i] = v;
     }
   }
-  // mGBinImage (CorrStack.cu:173-196): b x b sum, -1e30 if any invalid.
+  // mGBinImage (CorrStack.cu:40-63): b x b sum, -1e30 if any invalid.
   std::vector<float> bin((size_t)s.padb * s.by, 0.0f);
 #pragma omp parallel for schedule(static)
   for (int y = 0; y < s.by; y++) {
@@ -99,7 +99,7 @@ void correct_one(const float* raw, const CorrSizes& s, float shift_x,
     }
   }
   if (rweight) {
-    // RWeight::DoIt (CorrStack.cu:564-595): per row R2C (length bx), scale
+    // RWeight::DoIt (CorrStack.cu:431-462): per row R2C (length bx), scale
     // by 1/N and the ramp 2f(0.55 + 0.45 cos 2 pi f), f = k/N; C2R.
     const int n = s.bx, nc = n / 2 + 1;
     const float fn = 2.0f * ((float)nc - 1.0f);

@@ -5,7 +5,7 @@
 // radix decimation in time for factors up to 61 and Bluestein's chirp-z for
 // larger primes.  Pinned in tests against numpy.fft and the direct sum.
 // cuFFT's R2C/C2R conventions (unnormalised, n/2+1 outputs, in-place padded
-// rows) follow method/Cufft1&2.cu:68-117 and :150-206.
+// rows) follow method/Cufft1&2.cu:17-76 and :109-165.
 #include <map>
 #include <mutex>
 

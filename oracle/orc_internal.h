@@ -28,7 +28,7 @@ int zero_tilt_index(const float* tilts, int nz, float tilt = 0.0f);
 // FindRefIndex (Util.cpp:22-35).
 int find_ref_index(const float* tilts, int nz, int z);
 
-// Stretch matrix of CalcCC::Stretch (CalcCC.cu:318-335): inverse of
+// Stretch matrix of CalcCC::Stretch (CalcCC.cu:318-345): inverse of
 // R diag(s,1) R^T computed in double, cast to float.
 void stretch_matrix(float stretch, float axis_deg, float m[3]);
 

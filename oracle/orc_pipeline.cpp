@@ -22,10 +22,10 @@ struct orc_series {
   std::vector<float> sx, sy, rot;  // AlignParam
   float offset = 0.0f;
   int verbose = 0;
-  // Transform state (Transform.cpp:55-89).
+  // Transform state (Transform.cpp:24-58).
   int tr_bin = 1;
   std::vector<float> psx, psy, tr_angles;
-  // ProjAlign state (ProjAlign.cpp:126-203).
+  // ProjAlign state (ProjAlign.cpp:48-125).
   int pa_bin = 1, pa_tb = 0, pa_nproj = 10;
   float pa_z0 = 0.0f;
   std::vector<float> pa_angles, pa_corr;
@@ -57,7 +57,7 @@ void z_induced(const orc_series* S, int i, float* out) {
 }
 
 void unstretch(const orc_series* S, int ref, int f, float* sum) {
-  // Transform::UnStretch (Transform.cpp:164-187).
+  // Transform::UnStretch (Transform.cpp:133-157).
   double ds = std::cos(S->tr_angles[ref] * kD2R) / std::cos(S->tr_angles[f] * kD2R);
   double d2t = 2 * kD2R * S->rot[f];
   double s2 = std::sin(d2t), c2 = std::cos(d2t);
@@ -179,7 +179,7 @@ extern "C" void orc_set_params(orc_series* S, const float* sx, const float* sy,
 }
 
 extern "C" void orc_coarse_setup(orc_series* S) {
-  // Transform::Setup (Transform.cpp:55-89).
+  // Transform::Setup (Transform.cpp:24-58).
   int bx = (int)(S->nx / 1024.0f + 0.1f), by = (int)(S->ny / 1024.0f + 0.1f);
   S->tr_bin = std::max(std::max(bx, by), 1);
   S->psx.assign(S->nz, 0.0f);
@@ -189,7 +189,7 @@ extern "C" void orc_coarse_setup(orc_series* S) {
 }
 
 extern "C" float orc_coarse_iter(orc_series* S) {
-  // Transform::DoIt / Measure / UpdateShift (Transform.cpp:91-162).
+  // Transform::DoIt / Measure / UpdateShift (Transform.cpp:60-131).
   const int nz = S->nz;
   std::vector<float> corr;
   CorrSizes s;
@@ -282,7 +282,7 @@ extern "C" void orc_apply_offset(orc_series* S) {
 
 extern "C" void orc_projalign_setup(orc_series* S, float zshift, int align_z,
                                     int nproj) {
-  // ProjAlign::Setup / Setsize (ProjAlign.cpp:126-203).
+  // ProjAlign::Setup / Setsize (ProjAlign.cpp:48-125).
   S->pa_z0 = zshift;
   int bx = (int)(S->nx / 2048.0f + 0.5f), by = (int)(S->ny / 2048.0f + 0.5f);
   S->pa_bin = std::max(std::max(bx, by), 1);
@@ -353,7 +353,7 @@ extern "C" float orc_projalign_iter_forced(orc_series* S,
   std::vector<float> rp(plane);
   const float binx = S->nx / (float)s.bx, biny = S->ny / (float)s.by;
   auto align = [&](int t) {
-    // ClacAlignProj (ProjAlign.cpp:259-319).
+    // ClacAlignProj (ProjAlign.cpp:181-241).
     orc_reproject(S->pa_corr.data(), s.bx, s.by, nz, S->pa_tb, ang,
                   skip.data(), t, S->pa_nproj, 0, rp.data(), nullptr);
     float ish[2];
@@ -447,7 +447,7 @@ extern "C" void orc_series_trace_clear(orc_series* S) { S->trace.clear(); }
 
 extern "C" void orc_correct_final(orc_series* S, int out_bin, float* out,
                                   int* out_size) {
-  // Process::CorrectStack (Process.cpp:329-351): rotation + shift, random
+  // Process::CorrectStack (Process.cpp:329-372): rotation + shift, random
   // fill, no R-weighting, OutBin sum binning.
   std::vector<float> corr;
   CorrSizes s;

@@ -39,7 +39,7 @@ int find_ref_index(const float* tilts, int nz, int z) {
 extern "C" void orc_find_proj_range(const float* tilts, int nz,
                                     const unsigned char* skip, int target,
                                     int nproj, int* range2) {
-  // CalcReproj.cu:281-334.
+  // CalcReproj.cu:206-259.
   const float ref_range = 20.5f, ref_stretch = 1.2f;
   float tilt = tilts[target];
   float fc = (float)std::cos(tilt * kD2R);
@@ -167,7 +167,7 @@ extern "C" void orc_reproject(const float* stack, int bx, int by, int nz,
     tc[k] = fcos(v);
     ts[k] = fsin(v);
   }
-  // ForwardProj (CalcReproj.cu:374-386): exact D2R, host cos/sin.
+  // ForwardProj (CalcReproj.cu:299-311): exact D2R, host cos/sin.
   float c = (float)std::cos(kD2R * proj_angle);
   float s = (float)std::sin(kD2R * proj_angle);
   int L = (int)((float)tb / c + 0.5f);

@@ -4,7 +4,7 @@
 
 namespace orc {
 
-// Sizes of one CorrTomoStack configuration (CorrStack.cu:419-461).
+// Sizes of one CorrTomoStack configuration (CorrStack.cu:298-328).
 struct CorrSizes {
   int nx, ny;    // raw image
   int pad;       // padded row pitch of the corrected image

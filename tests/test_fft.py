@@ -68,7 +68,7 @@ def test_gpu_fft2d(ctx, nx, ny):
 
 @pytest.mark.gpu
 def test_gpu_fft2d_scale(ctx):
-    """bNorm: forward result multiplied by 1/(nx*ny) (Cufft1&2.cu:97)."""
+    """bNorm: forward result multiplied by 1/(nx*ny) (Cufft1&2.cu:53-60)."""
     nx, ny = 64, 40
     img = np.random.default_rng(1).standard_normal((ny, nx)).astype(np.float32)
     d = ctx.array(padded(img))

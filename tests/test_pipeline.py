@@ -60,7 +60,7 @@ def test_synthetic_series_geometry(series):
     """The generator is the inverse of the correction: correcting the
     noise-free raw image with the true shift and axis reproduces the aligned
     projection, up to the correction's pixel-centre convention -- output
-    pixel i samples p = i + 0.5 - W/2 (CorrStack.cu:290-303) while the
+    pixel i samples p = i + 0.5 - W/2 (CorrStack.cu:156-170) while the
     generator's aligned pixel i sits at i - W/2 -- i.e. a -0.5 px offset on
     both axes, measured here with the correlation peak fit."""
     import oracle

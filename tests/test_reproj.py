@@ -63,7 +63,7 @@ def test_find_proj_range_matches_product(mf, orc):
 def test_zero_tilt_identity(orc):
     """One reference at 0 deg, target at 0 deg: RP(BP(p)) == p exactly,
     except the last detector column: its voxels have rho = bx - 0.5, above
-    the reference's bound rho <= bx - 1 (CalcReproj.cu:124), so the column is
+    the reference's bound rho <= bx - 1 (CalcReproj.cu:17-24), so the column is
     never backprojected and reprojects as invalid."""
     bx, by, tb = 32, 4, 12
     rng = np.random.default_rng(0)

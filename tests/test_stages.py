@@ -75,7 +75,7 @@ def test_binning_sums(orc):
 
 def test_rweight_is_ramp_filter(orc):
     """R2C, x (1/N) ramp 2f(0.55+0.45cos 2pi f), C2R (unnormalised) equals
-    irfft(rfft(x) * ramp) -- the filter of GRWeight (CorrStack.cu:564-579)."""
+    irfft(rfft(x) * ramp) -- the filter of GRWeight (CorrStack.cu:431-446)."""
     rng = np.random.default_rng(3)
     raw = rng.standard_normal((12, 40)).astype(np.float32)
     out = orc.correct_proj(raw, -1.0, -1.0, 0.0, True, False, 1, True)
