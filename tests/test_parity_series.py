@@ -176,6 +176,38 @@ def test_gpu_lockstep_cfg2_stages(ctx, mf, orc, cfg2):
     flush_print("cfg2 stages:", r.summary())
 
 
+@pytest.mark.gpu
+def test_gpu_lockstep_cfg3_stages(ctx, mf, orc):
+    """The large-detector workload's geometry (5760 x 4092 float32, Poisson
+    at 20 e-/px, 0.5% magnification jitter, axis 86 deg, shifts U(-80,80),
+    AlignZ 300) on the seven tilts -6..6 deg, one of each stage: the coarse
+    sweep at bin 5 (1152 x 818), the common-line search at full resolution
+    (over 180 and 20 deg), the offset scan at bin 4, a projection-matching
+    sweep at bin 3 (1920 x 1364, slab 100) and the final correction."""
+    from markerfree_amd import synth
+    cfg = synth.SeriesConfig(nx=5760, ny=4092, nz=7, vol=(5760, 4092, 300),
+                             nblobs=12000, tilt_min=-6.0, tilt_step=2.0,
+                             axis=(86.0, 86.0), shift_max=80.0,
+                             mag_jitter=0.005, noise="poisson")
+    s = synth.generate(cfg)
+    L = Lockstep(mf, orc, ctx, s.stack, s.tilts)
+    try:
+        L.preprocess()
+        L.coarse_setup()
+        L.coarse_iter()
+        L.tilt_axis(180.0)
+        L.find_offset()
+        L.apply_offset()
+        L.projalign_setup(300)
+        L.projalign_iter()
+        L.tilt_axis(20.0)
+        L.correct(1)
+        r = L.r
+    finally:
+        L.close()
+    flush_print("cfg3 stages:", r.summary())
+
+
 # ------------------------------------------------------------ end to end
 
 def _e2e_diffs(res, fx):
