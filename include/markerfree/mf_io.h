@@ -71,6 +71,38 @@ int mfio_save_txt(const char* path, int nx, int ny, int nz,
 int mfio_save_xf(const char* path, int nx, int nz, const float* rotate,
                  const double* shift_x, const double* shift_y, int inv);
 
+/* Reads a .txt parameter file written by SaveAllParam (SaveParam.cpp:236-256;
+ * this library's mfio_save_txt): the "SEC ROT TX TY TILT" rows in order, up
+ * to cap of them, and the angleoffset and Output Bin lines (either pointer
+ * may be NULL).  Returns the number of rows, or -1 if the file cannot be
+ * opened. */
+int mfio_read_txt(const char* path, int cap, float* rotate, double* shift_x,
+                  double* shift_y, float* tilts, float* angle_offset,
+                  int* out_bin);
+
+/* Fiducial residuals of an alignment: the evaluation the reference's cal_res
+ * tool makes (tool/src/main/main.cpp:35-237, triangulation
+ * tool/src/nbundle/bundle_core.cpp:148-216) from marker tracks given as
+ * coordinates (its OpenCV marker detection is not part of this library).
+ * raw_xy [nmarkers][nz][2]: each marker's raw-image position per section
+ * (pixel k at position k; NaN where it is not seen).  Each observation goes
+ * to the aligned frame by the inverse of the stack correction,
+ * p = R(-rotate) (q - (nx/2, ny/2) - shift); each marker seen in at least
+ * min_views sections gets the 3-D position (X, Y, Z) minimising
+ * sum |p - (X cos t + Z sin t + d, Y)|^2 over its views, t = tilts (with the
+ * angle offset), d one x offset shared by all (fitted when fit_centre is
+ * nonzero, else 0: the alignment leaves the rotation centre's x free).
+ * Outputs: per_section[nz] RMS residual (px, NaN if unseen; may be NULL),
+ * marker_xyz[nmarkers][3] (NaN for markers not used; may be NULL) and
+ * summary[6] = mean, RMS and largest residual (px), observations used,
+ * markers used, d.  Returns 0, or -1 on bad arguments. */
+int mfio_fiducial_residuals(int nx, int ny, int nz, const float* rotate,
+                            const double* shift_x, const double* shift_y,
+                            const float* tilts, int nmarkers,
+                            const float* raw_xy, int min_views,
+                            int fit_centre, double* per_section,
+                            double* marker_xyz, double* summary);
+
 #ifdef __cplusplus
 }
 #endif

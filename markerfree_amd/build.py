@@ -86,9 +86,13 @@ def build(jobs: int | None = None, verbose: bool = False) -> None:
     # File formats of the command line (host only), then the drop-in CLI
     # linked against both libraries.
     _run(["g++", *COMMON, "-Wall", "-shared", str(CSRC / "cli" / "io.cpp"),
-          "-o", str(IO)])
+          str(CSRC / "cli" / "residuals.cpp"), "-o", str(IO)])
     _run(["g++", *COMMON, "-Wall", str(CSRC / "cli" / "cli.cpp"), "-o",
           str(CLI), f"-L{LIB.parent}", "-lmarkerfree_hip", "-lmf_io",
+          "-Wl,-rpath,$ORIGIN/../lib"])
+    # The fiducial residual evaluator's command line (host only).
+    _run(["g++", *COMMON, "-Wall", str(CSRC / "cli" / "cal_res.cpp"), "-o",
+          str(CLI.parent / "cal_res"), f"-L{LIB.parent}", "-lmf_io",
           "-Wl,-rpath,$ORIGIN/../lib"])
     if verbose:
         print(f"built {LIB}, {SYNTH}, {IO}, {CLI}")

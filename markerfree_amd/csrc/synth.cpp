@@ -37,6 +37,12 @@ typedef struct {
 int mfs_generate(const mfs_config* cfg, float* stack, float* tilts,
                  float* shift_x, float* shift_y, float* axis_out,
                  float* scale_out);
+// The raw-image positions of the first 'count' blob centres in every
+// section (pixel x at position x, as the images are drawn), NaN where the
+// centre falls outside the image: raw_xy [count][nz][2].  The same seeds
+// give the same blobs and geometry as mfs_generate.  Returns the number of
+// blobs written.
+int mfs_fiducials(const mfs_config* cfg, int count, float* raw_xy);
 }
 
 namespace {
@@ -84,32 +90,76 @@ struct Blob {
   double x, y, z, sigma, amp;
 };
 
+// The phantom and the geometry the seeds draw (shared by mfs_generate and
+// mfs_fiducials).
+struct Scene {
+  std::vector<Blob> blobs;
+  double axis;
+  std::vector<double> sx, sy, mag;
+  Scene(const mfs_config* c) {
+    Pcg32 rp(c->seed_phantom, 1), rg(c->seed_geom, 2);
+    blobs.resize(c->nblobs);
+    for (auto& b : blobs) {
+      b.x = rp.uniform(-0.5, 0.5) * c->vol_x;
+      b.y = rp.uniform(-0.5, 0.5) * c->vol_y;
+      b.z = rp.uniform(-0.5, 0.5) * c->vol_z;
+      b.sigma = rp.uniform(c->sigma_min, c->sigma_max);
+      b.amp = rp.uniform(c->amp_min, c->amp_max);
+    }
+    axis = c->axis_min == c->axis_max ? c->axis_min
+                                      : rg.uniform(c->axis_min, c->axis_max);
+    const int nz = c->nz;
+    sx.resize(nz);
+    sy.resize(nz);
+    mag.resize(nz);
+    for (int i = 0; i < nz; i++) {
+      sx[i] = rg.uniform(-c->shift_max, c->shift_max);
+      sy[i] = rg.uniform(-c->shift_max, c->shift_max);
+      mag[i] = c->mag_jitter > 0
+                   ? 1.0 + rg.uniform(-c->mag_jitter, c->mag_jitter)
+                   : 1.0;
+    }
+  }
+  // Raw position of blob b in section i: q = R(axis) p + shift + c, with p
+  // the aligned detector position (x0 cos t + z0 sin t, y0), magnified.
+  void raw_pos(const mfs_config* c, const Blob& b, int i, double* qx,
+               double* qy) const {
+    const double t = (c->tilt_min + i * c->tilt_step) * M_PI / 180.0;
+    const double ca = std::cos(axis * M_PI / 180.0),
+                 sa = std::sin(axis * M_PI / 180.0);
+    const double u0 = (b.x * std::cos(t) + b.z * std::sin(t)) * mag[i];
+    const double v0 = b.y * mag[i];
+    *qx = ca * u0 - sa * v0 + sx[i] + c->nx * 0.5;
+    *qy = sa * u0 + ca * v0 + sy[i] + c->ny * 0.5;
+  }
+};
+
 }  // namespace
+
+int mfs_fiducials(const mfs_config* c, int count, float* raw_xy) {
+  if (!c || !raw_xy || count < 0) return -1;
+  const Scene sc(c);
+  const int n = std::min(count, (int)sc.blobs.size());
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < c->nz; i++) {
+      double qx, qy;
+      sc.raw_pos(c, sc.blobs[k], i, &qx, &qy);
+      const bool in = qx >= 0 && qx <= c->nx - 1 && qy >= 0 && qy <= c->ny - 1;
+      raw_xy[2 * ((size_t)k * c->nz + i)] = in ? (float)qx : NAN;
+      raw_xy[2 * ((size_t)k * c->nz + i) + 1] = in ? (float)qy : NAN;
+    }
+  return n;
+}
 
 int mfs_generate(const mfs_config* c, float* stack, float* tilts,
                  float* shift_x, float* shift_y, float* axis_out,
                  float* scale_out) {
   if (!c || !stack || c->nx <= 0 || c->ny <= 0 || c->nz <= 0) return 1;
   const int nx = c->nx, ny = c->ny, nz = c->nz;
-  Pcg32 rp(c->seed_phantom, 1), rg(c->seed_geom, 2);
-  std::vector<Blob> blobs(c->nblobs);
-  for (auto& b : blobs) {
-    b.x = rp.uniform(-0.5, 0.5) * c->vol_x;
-    b.y = rp.uniform(-0.5, 0.5) * c->vol_y;
-    b.z = rp.uniform(-0.5, 0.5) * c->vol_z;
-    b.sigma = rp.uniform(c->sigma_min, c->sigma_max);
-    b.amp = rp.uniform(c->amp_min, c->amp_max);
-  }
-  const double axis = c->axis_min == c->axis_max
-                          ? c->axis_min
-                          : rg.uniform(c->axis_min, c->axis_max);
-  std::vector<double> sx(nz), sy(nz), mag(nz);
-  for (int i = 0; i < nz; i++) {
-    sx[i] = rg.uniform(-c->shift_max, c->shift_max);
-    sy[i] = rg.uniform(-c->shift_max, c->shift_max);
-    mag[i] = c->mag_jitter > 0 ? 1.0 + rg.uniform(-c->mag_jitter, c->mag_jitter)
-                               : 1.0;
-  }
+  const Scene sc(c);
+  const std::vector<Blob>& blobs = sc.blobs;
+  const double axis = sc.axis;
+  const std::vector<double>&sx = sc.sx, &sy = sc.sy, &mag = sc.mag;
   if (axis_out) *axis_out = (float)axis;
   const double ca = std::cos(axis * M_PI / 180.0), sa = std::sin(axis * M_PI / 180.0);
   const double cxr = nx * 0.5, cyr = ny * 0.5;

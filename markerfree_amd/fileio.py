@@ -49,6 +49,11 @@ def lib():
                                     C.c_int, C.c_float]
         L.mfio_save_xf.argtypes = [C.c_char_p, C.c_int, C.c_int, f32p, f64p,
                                    f64p, C.c_int]
+        L.mfio_read_txt.argtypes = [C.c_char_p, C.c_int, f32p, f64p, f64p,
+                                    f32p, f32p, C.POINTER(C.c_int)]
+        L.mfio_fiducial_residuals.argtypes = [
+            C.c_int, C.c_int, C.c_int, f32p, f64p, f64p, f32p, C.c_int, f32p,
+            C.c_int, C.c_int, f64p, f64p, f64p]
         _lib = L
     return _lib
 
@@ -131,6 +136,52 @@ def save_xf(path, nx, rotate, shift_x, shift_y, inv):
     sy = np.ascontiguousarray(shift_y, np.float64)
     _check(lib().mfio_save_xf(str(path).encode(), nx, len(r), _f32(r),
                               _f64(sx), _f64(sy), int(inv)))
+
+
+def read_txt(path, nz):
+    """The .txt parameter file: (rotate, shift_x, shift_y, tilts,
+    angle_offset, out_bin), rows in ascending tilt order."""
+    rot = np.zeros(nz, np.float32)
+    sx = np.zeros(nz, np.float64)
+    sy = np.zeros(nz, np.float64)
+    t = np.zeros(nz, np.float32)
+    off = C.c_float()
+    b = C.c_int()
+    n = lib().mfio_read_txt(str(path).encode(), nz, _f32(rot), _f64(sx),
+                            _f64(sy), _f32(t), C.byref(off), C.byref(b))
+    if n < 0:
+        raise OSError(f"cannot read {path}")
+    if n != nz:
+        raise ValueError(f"{path}: {n} rows, expected {nz}")
+    return rot, sx, sy, t, float(off.value), int(b.value)
+
+
+def fiducial_residuals(nx, ny, rotate, shift_x, shift_y, tilts, raw_xy,
+                       min_views=2, fit_centre=True):
+    """Reprojection residuals of marker tracks raw_xy [n][nz][2] (NaN where
+    unseen) under an alignment (mfio_fiducial_residuals): returns a dict
+    with mean, rms, max (px), observations, markers, centre offset, the
+    per-section RMS and the markers' 3-D positions."""
+    rot = np.ascontiguousarray(rotate, np.float32)
+    nz = len(rot)
+    sx = np.ascontiguousarray(shift_x, np.float64)
+    sy = np.ascontiguousarray(shift_y, np.float64)
+    t = np.ascontiguousarray(tilts, np.float32)
+    xy = np.ascontiguousarray(raw_xy, np.float32)
+    n = xy.shape[0]
+    assert xy.shape == (n, nz, 2)
+    per = np.zeros(nz, np.float64)
+    xyz = np.zeros((n, 3), np.float64)
+    summ = np.zeros(6, np.float64)
+    st = lib().mfio_fiducial_residuals(nx, ny, nz, _f32(rot), _f64(sx),
+                                       _f64(sy), _f32(t), n, _f32(xy),
+                                       min_views, int(fit_centre), _f64(per),
+                                       _f64(xyz), _f64(summ))
+    if st:
+        raise ValueError("bad fiducial input")
+    return {"mean": summ[0], "rms": summ[1], "max": summ[2],
+            "observations": int(summ[3]), "markers": int(summ[4]),
+            "centre": summ[5], "per_section": per, "xyz": xyz}
 
 
 # --------------------------------------------------------- numpy MRC writer

@@ -44,6 +44,8 @@ def _get():
         _lib.mfs_generate.restype = C.c_int
         fp = C.POINTER(C.c_float)
         _lib.mfs_generate.argtypes = [C.POINTER(_Cfg), fp, fp, fp, fp, fp, fp]
+        _lib.mfs_fiducials.restype = C.c_int
+        _lib.mfs_fiducials.argtypes = [C.POINTER(_Cfg), C.c_int, fp]
     return _lib
 
 
@@ -95,7 +97,7 @@ class Series:
     config: SeriesConfig = field(default_factory=SeriesConfig)
 
 
-def generate(cfg: SeriesConfig) -> Series:
+def _cfg(cfg: SeriesConfig) -> _Cfg:
     c = _Cfg()
     c.nx, c.ny, c.nz = cfg.nx, cfg.ny, cfg.nz
     c.vol_x, c.vol_y, c.vol_z = cfg.vol
@@ -109,6 +111,21 @@ def generate(cfg: SeriesConfig) -> Series:
     c.snr, c.dose = cfg.snr, cfg.dose
     c.int16_counts = int(cfg.int16)
     c.seed_phantom, c.seed_geom, c.seed_noise = cfg.seeds
+    return c
+
+
+def fiducials(cfg: SeriesConfig, count: int) -> np.ndarray:
+    """Raw-image positions of the first 'count' blob centres in every
+    section, [count][nz][2] (NaN outside the image): point-like markers with
+    known 3-D positions for the fiducial residual evaluator."""
+    out = np.empty((count, cfg.nz, 2), np.float32)
+    n = _get().mfs_fiducials(C.byref(_cfg(cfg)), count,
+                             out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out[:n]
+
+
+def generate(cfg: SeriesConfig) -> Series:
+    c = _cfg(cfg)
     stack = np.empty((cfg.nz, cfg.ny, cfg.nx), np.float32)
     tilts = np.empty(cfg.nz, np.float32)
     sx = np.empty(cfg.nz, np.float32)
