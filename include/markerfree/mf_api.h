@@ -118,7 +118,10 @@ mf_status mf_find_proj_range(const float* tilts, int nz,
  * the one-row fused form without per-target tables (identical output; the
  * last fallback of mode 1); modes 3, 4 and 5 pin forms of mode 1 (one
  * row with tables, two rows with tables, two rows without; identical
- * output). */
+ * output); modes 6 and 7: materialised slab of R = 4 or 2 image rows per
+ * voxel, d_slab [cdiv(by, R)][tb][2bx][R] (image row R*g + r of group g at
+ * position r; the same voxel values as mode 0; MF_EINVAL when the R rows'
+ * sinogram rows do not fit in LDS). */
 mf_status mf_reproject(mf_ctx* ctx, const float* d_stack, int bx, int by,
                        int nz, int tb, const float* tilts,
                        const unsigned char* skip, int target, int nproj,
@@ -236,7 +239,7 @@ typedef struct {
   int out_bin;       /* -g field 6 */
   int nproj;         /* -p: neighbouring projections per target */
   int verbose;       /* print the reference's stage trace to stdout */
-  int reproj_mode;   /* mf_reproject mode 0..5: 0 slab, 1 fused (default) */
+  int reproj_mode;   /* mf_reproject mode 0..7: 0 slab, 1 fused (default) */
   int pa_pairs;      /* 1 (default): a projection-matching sweep measures the
                         pair of targets z0+i, z0-i together when neither
                         reads the other's image; 0: one target at a time.

@@ -1217,6 +1217,222 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
   }
 }
 
+// ------------------------------------------- materialised, R rows per voxel
+// The slab written for R image rows at once: a voxel's sinogram index depends
+// on (x, z) only, so one index computation and one LDS slot read serve the R
+// rows, and each voxel is stored as R consecutive floats,
+//   slab [cdiv(by, R)][tb][2bx][R]  (row y = R*g + r of group g at lane r).
+// Each voxel takes the same terms in the same order as k_bp_slab, so every
+// value is bit-identical to it; rows past by repeat the last row and are
+// never read back.  Workgroup g owns a contiguous range of (group, iz) slab
+// rows; no LDS is left beside the sinogram slots, so a skipped sample (-inf)
+// is found from the voxel it makes non-finite, and that block is computed
+// again with per-row validity.
+template <int N, int R>
+__global__ __launch_bounds__(kRowsThreads) void k_bp_slabr(const BpParams p,
+                                                           int per) {
+  typedef typename FreeSlot<R>::T Slot;
+  extern __shared__ float4 smem_slab[];
+  Slot* sq = reinterpret_cast<Slot*>(smem_slab);  // [n][bx]
+  constexpr int V = kBpBlock / 64;
+  const int n = N ? N : p.n;
+  const int bx = p.bx;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int vx = 2 * bx;
+  const float cx = (float)bx * 0.5f, iend = (float)(bx - 1);
+  const float hvx = (float)vx * 0.5f, htb = (float)p.tb * 0.5f;
+  const LaneProj lp(p, n, lane);
+  const bool all_fast_ok = p.cnt_all >= 0.001f;
+  const int groups = cdiv(p.by, R);
+  const int total = groups * p.tb;
+  const int qa = blockIdx.x * per, qb = min(total, qa + per);
+  Slot* slab = reinterpret_cast<Slot*>(p.slab);
+
+  // The reference's per-voxel loop with per-row validity (range, skipped
+  // samples) for the V voxels of this lane.
+  auto general = [&](const float* fx, float zf, unsigned long long m_out,
+                     float res[V][R]) {
+    float acc[V][R], cnt[V][R];
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int r = 0; r < R; r++) acc[v][r] = cnt[v][r] = 0.f;
+#pragma unroll 1
+    for (int i = 0; i < n; i++) {
+      if ((m_out >> i) & 1ull) continue;
+      const float c = lane_f(lp.c, i), w = lane_f(lp.w, i);
+      const float zsc = fmaf(zf, lane_f(lp.s, i), cx);
+      const Slot* row = sq + i * bx;
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        const float rr = fmaf(fx[v], c, zsc);
+        const bool inr = (rr >= 0.f) && (rr <= iend);
+        float val[R];
+        FreeSlot<R>::split(row[(int)__builtin_amdgcn_fmed3f(rr, 0.f, iend)],
+                           val);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const bool ok = inr && (val[r] != -INFINITY);
+          acc[v][r] = fmaf(ok ? val[r] : 0.f, w, acc[v][r]);
+          cnt[v][r] += ok ? w : 0.f;
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; v++)
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        res[v][r] = (cnt[v][r] < 0.001f) ? kInvalid : acc[v][r] / cnt[v][r];
+  };
+
+  int gcur = -1;
+  for (int q0 = qa; q0 < qb;) {
+    const int g = q0 / p.tb;
+    const int za = q0 - g * p.tb;
+    const int zb = min(p.tb, za + (qb - q0));
+    q0 += zb - za;
+    if (g != gcur) {
+      __syncthreads();  // the previous group's waves are done with the slots
+      int y[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) y[r] = min(R * g + r, p.by - 1);
+      load_sino_slots<R>(p, n, y, sq);
+      gcur = g;
+    }
+    for (int iz = za + wave; iz < zb; iz += nwaves) {
+      const float fz = (float)iz + 0.5f - htb;
+      const float lz = fmaf(fz, lp.s, cx);  // lane i: zsc of projection i
+      // With N fixed, the projections' z terms sit in registers and their
+      // cos and weights are kernel arguments.
+      float zs[N ? N : 1];
+#pragma unroll
+      for (int i = 0; i < (N ? N : 1); i++)
+        zs[i] = N ? fmaf(fz, p.s[i], cx) : 0.f;
+      Slot* orow = slab + ((size_t)g * p.tb + iz) * vx + lane;
+      for (int ix0 = 0; ix0 < vx; ix0 += kBpBlock) {
+        const int ixl = min(ix0 + kBpBlock - 1, vx - 1);
+        const float rf = fmaf((float)ix0 + 0.5f - hvx, lp.c, lz);
+        const float rl = fmaf((float)ixl + 0.5f - hvx, lp.c, lz);
+        const unsigned long long m_in =
+            __ballot(rf >= 0.f && rl <= iend) & lp.mask;
+        const unsigned long long m_out =
+            __ballot(rl < 0.f || rf > iend) & lp.mask;
+        float fx[V], res[V][R];
+#pragma unroll
+        for (int v = 0; v < V; v++)
+          fx[v] = (float)min(ix0 + lane + 64 * v, ixl) + 0.5f - hvx;
+        if (m_out == lp.mask) {
+#pragma unroll
+          for (int v = 0; v < V; v++)
+#pragma unroll
+            for (int r = 0; r < R; r++) res[v][r] = kInvalid;
+        } else if (m_in == lp.mask && all_fast_ok) {
+          // Every projection in range over the block: no tests, one index
+          // and one slot per lookup for the R rows.
+          float acc[V][R];
+#pragma unroll
+          for (int v = 0; v < V; v++)
+#pragma unroll
+            for (int r = 0; r < R; r++) acc[v][r] = 0.f;
+          auto lookup = [&](int i, float c, float w, float zsc) {
+            const Slot* row = sq + i * bx;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+              float val[R];
+              FreeSlot<R>::split(row[(int)fmaf(fx[v], c, zsc)], val);
+#pragma unroll
+              for (int r = 0; r < R; r++)
+                acc[v][r] = fmaf(val[r], w, acc[v][r]);
+            }
+          };
+          if (N) {
+#pragma unroll
+            for (int i = 0; i < (N ? N : 1); i++)
+              lookup(i, p.c[i], p.w[i], zs[i]);
+          } else {
+#pragma unroll 1
+            for (int i = 0; i < n; i++)
+              lookup(i, lane_f(lp.c, i), lane_f(lp.w, i), lane_f(lz, i));
+          }
+          // Valid samples give finite voxels, so a non-finite sum of the
+          // lane's voxels means a skipped sample (-inf) reached one of them
+          // (a false alarm only recomputes the block, exactly).
+          float chk = 0.f;
+#pragma unroll
+          for (int v = 0; v < V; v++)
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+              res[v][r] = div_by_rcp(acc[v][r], p.cnt_all, p.rcp_all);
+              chk += res[v][r];
+            }
+          const bool bad = !isfinite(chk);
+          // A skipped sample made some voxel -inf or NaN.
+          if (__ballot(bad)) general(fx, fz, m_out, res);
+        } else {
+          general(fx, fz, m_out, res);
+        }
+#pragma unroll
+        for (int v = 0; v < V; v++)
+          if (ix0 + lane + 64 * v < vx)
+            orow[ix0 + 64 * v] = FreeSlot<R>::make(res[v]);
+      }
+    }
+  }
+}
+
+// Ray sums over the R-row slab: one lane per detector position u and group
+// g, the R rows' samples read as one slot.  The samples are visited in the
+// order of k_rp_slab, so each row's output is bit-identical to it.
+template <int R>
+__global__ __launch_bounds__(256) void k_rp_slabr(const RpParams q) {
+  typedef typename FreeSlot<R>::T Slot;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (u >= q.bx) return;
+  const int vx = 2 * q.bx;
+  const Slot* vol =
+      reinterpret_cast<const Slot*>(q.slab) + (size_t)g * q.tb * vx;
+  const float xp = (float)u + 0.5f - (float)q.bx * 0.5f;
+  const float tx = fmaf(xp, q.c, (float)vx * 0.5f);
+  const float tz = fmaf(xp, q.s, (float)q.tb * 0.5f);
+  const float z0 = (-xp * q.s) / q.c - 0.5f * (float)q.L;
+  const float iex = (float)(vx - 1), iez = (float)(q.tb - 1);
+  int t0, t1;
+  rp_sample_range(q, tx, tz, z0, &t0, &t1);
+  float sum[R];
+  int cnt[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    sum[r] = 0.0f;
+    cnt[r] = 0;
+  }
+#pragma unroll 4
+  for (int t = t0; t < t1; t++) {
+    const float fz = (float)t + z0;
+    const float X = fmaf(-fz, q.s, tx);
+    const float Z = fmaf(fz, q.c, tz);
+    const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+    const int ix = inb ? (int)X : 0, iz = inb ? (int)Z : 0;
+    float v[R];
+    FreeSlot<R>::split(vol[(size_t)vx * iz + ix], v);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const bool ok = inb && (v[r] >= -1e10f);
+      sum[r] += ok ? v[r] : 0.0f;
+      cnt[r] += ok ? 1 : 0;
+    }
+  }
+  const float lim = 0.8f * (float)q.L;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int y = R * g + r;
+    if (y < q.by)
+      q.out[(size_t)y * q.bx + u] =
+          ((float)cnt[r] < lim) ? kInvalid : sum[r] / (float)cnt[r];
+  }
+}
+
 __global__ void k_fill(float* p, size_t n, float v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
@@ -1251,6 +1467,21 @@ struct TabKernelTable {
     TabKernelTable<N - 1>::fill(tk, tk2, f2, f4);
   }
 };
+typedef void (*SlabKernel)(BpParams, int);
+
+template <int N>
+struct SlabKernelTable {
+  static void fill(SlabKernel* s4, SlabKernel* s2) {
+    s4[N] = k_bp_slabr<N, 4>;
+    s2[N] = k_bp_slabr<N, 2>;
+    SlabKernelTable<N - 1>::fill(s4, s2);
+  }
+};
+template <>
+struct SlabKernelTable<-1> {
+  static void fill(SlabKernel*, SlabKernel*) {}
+};
+
 template <>
 struct TabKernelTable<0> {
   static void fill(FusedKernel* tk, FusedKernel* tk2, FusedKernel* f2,
@@ -1265,12 +1496,16 @@ struct Kernels {
   FusedKernel rows2[kTabMaxN + 1];
   FusedKernel free2[kTabMaxN + 1];
   FusedKernel free4[kTabMaxN + 1];
+  // Materialised R-row backprojection, n = 1..kTabMaxN fixed, 0 runtime.
+  SlabKernel slab4[kTabMaxN + 1];
+  SlabKernel slab2[kTabMaxN + 1];
   // Static LDS of every kernel above and of k_bp_slab, read once (block
   // reductions such as __syncthreads_or use some).
   std::map<const void*, size_t> static_lds;
   Kernels() {
     KernelTable<kMaxFixedN>::fill(fu);
     TabKernelTable<kTabMaxN>::fill(tab, rows2, free2, free4);
+    SlabKernelTable<kTabMaxN>::fill(slab4, slab2);
     auto note = [&](const void* k) {
       if (!k) return;
       hipFuncAttributes a;
@@ -1281,6 +1516,8 @@ struct Kernels {
     for (FusedKernel* t : {tab, rows2, free2, free4})
       for (int n = 0; n <= kTabMaxN; n++) note((const void*)t[n]);
     note((const void*)k_bp_slab);
+    for (SlabKernel* t : {slab4, slab2})
+      for (int n = 0; n <= kTabMaxN; n++) note((const void*)t[n]);
   }
 };
 
@@ -1388,7 +1625,7 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   const size_t lds = bp_lds_bytes(n, bx);
   MF_REQUIRE(lds <= 160 * 1024, "sinogram rows exceed LDS");
   const int kid = n <= kMaxFixedN ? n : 0;
-  if (mode != 0) {
+  if (mode >= 1 && mode <= 5) {
     if (n == 0) {
       hipLaunchKernelGGL(k_fill, dim3(cdiv((long long)bx * by, 256)),
                          dim3(256), 0, st, d_reproj, (size_t)bx * by,
@@ -1442,6 +1679,49 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
     TimedLaunch tl(ctx->timer, "bprp_fused", st);
     hipLaunchKernelGGL(k, dim3(cdiv(units, per)), dim3(threads), flds, st, bp,
                        rp, per);
+    MF_HIP(hipGetLastError());
+    return;
+  }
+  if (mode == 6 || mode == 7) {
+    // Materialised slab of R image rows per voxel (R = 4 or 2).
+    const int R = mode == 6 ? 4 : 2;
+    const int groups = cdiv(by, R);
+    const size_t elems = (size_t)groups * R * tb * 2 * bx;
+    float* slab = d_slab;
+    if (!slab) slab = (float*)ctx->slab_ws.ensure(ctx, elems * 4);
+    bp.slab = slab;
+    rp.slab = slab;
+    if (n == 0) {
+      hipLaunchKernelGGL(k_fill, dim3(4096), dim3(256), 0, st, slab, elems,
+                         kInvalid);
+    } else {
+      const int sid = n <= kTabMaxN ? n : 0;
+      const SlabKernel kf =
+          R == 4 ? kernels().slab4[sid] : kernels().slab2[sid];
+      const void* k = (const void*)kf;
+      const size_t flds = free_lds_bytes(R, n, bx);
+      MF_REQUIRE(lds_fits(k, flds),
+                 "sinogram rows of this many image rows exceed LDS");
+      ensure_dyn_lds(ctx, k, flds);
+      const int occ = occupancy(ctx, k, kRowsThreads, flds);
+      const long long rows = (long long)groups * tb;
+      const long long slots =
+          (long long)std::max(occ, 1) * std::max(ctx->num_cu, 1);
+      const int per = (int)cdiv(rows, std::min(rows, slots));
+      TimedLaunch tl(ctx->timer, "bp_slab_rows", st);
+      hipLaunchKernelGGL(kf, dim3((int)cdiv(rows, per)), dim3(kRowsThreads),
+                         flds, st, bp, per);
+    }
+    MF_HIP(hipGetLastError());
+    {
+      TimedLaunch tl(ctx->timer, "rp_slab_rows", st);
+      if (R == 4)
+        hipLaunchKernelGGL(k_rp_slabr<4>, dim3(cdiv(bx, 256), groups),
+                           dim3(256), 0, st, rp);
+      else
+        hipLaunchKernelGGL(k_rp_slabr<2>, dim3(cdiv(bx, 256), groups),
+                           dim3(256), 0, st, rp);
+    }
     MF_HIP(hipGetLastError());
     return;
   }

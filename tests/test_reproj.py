@@ -193,7 +193,7 @@ def _check_target(ctx, orc, stack, tilts, skip, target, nproj, tb, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("bx,by,nz,step,tb", [
     (128, 16, 21, 3.0, 40),
     (256, 8, 41, 3.0, 100),   # small-series binned geometry, reduced rows
@@ -213,11 +213,11 @@ def test_gpu_reproject_parity(ctx, orc, mode, bx, by, nz, step, tb):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_gpu_reproject_bitexact_fp32(ctx, orc, mode):
     """Against the oracle's FP32 mode (backprojection accumulated as
-    fma(v*cos, w, acc) in reference order, ray sums in FP64) the HIP
-    reprojection is bit-identical."""
+    fma(v*cos, w, acc) in reference order, ray sums in FP32 in increasing
+    sample order) the HIP reprojection is bit-identical."""
     stack, tilts = _series(160, 12, 41, 3.0, seed=21)
     stack[3, 2, 40:44] = INVALID
     skip = np.zeros(41, np.uint8)
@@ -257,10 +257,50 @@ def test_gpu_fast_paths_bitexact_wide(ctx, orc, bx, by, tb):
                           d_out, mode=0, d_slab=d_slab)
             np.testing.assert_array_equal(d_slab.download(), slab)
             np.testing.assert_array_equal(d_out.download(), ref)
-            for mode in (1, 2, 3, 4, 5):
+            for mode in (1, 2, 3, 4, 5, 6, 7):
                 ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
                               d_out, mode=mode)
                 np.testing.assert_array_equal(d_out.download(), ref)
+    for d in (d_stack, d_out, d_slab):
+        d.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bx,by,tb,rows", [(1024, 6, 300, 4), (512, 7, 150, 4),
+                                           (300, 5, 60, 2), (96, 1, 30, 4)])
+def test_gpu_slab_rows_bitexact(ctx, orc, bx, by, tb, rows):
+    """Modes 6 and 7 (materialised slab of 4 or 2 image rows per voxel): the
+    interleaved slab, read back row by row, equals the oracle's slab, and the
+    reprojection is bit-identical -- with row counts that leave a partial
+    last group, skipped samples in one row of a group (the backprojection's
+    fast path finds them from non-finite voxels), and a target with no
+    neighbour (an invalid slab)."""
+    stack, tilts = _series(bx, by, 41, 3.0, seed=3 * bx + by)
+    stack[20, by // 2, bx // 3: bx // 3 + 9] = INVALID
+    nz = 41
+    mode = 6 if rows == 4 else 7
+    groups = -(-by // rows)
+    d_stack = ctx.array(stack)
+    d_out = ctx.empty((by, bx))
+    d_slab = ctx.empty((groups, tb, 2 * bx, rows))
+    z0 = int(np.argmin(np.abs(tilts)))
+    cases = []
+    for k in (1, 4, 12):
+        skip = np.ones(nz, np.uint8)
+        skip[max(0, z0 - k + 1): z0 + k] = 0
+        skip[z0 + k] = 1
+        cases.append((skip, z0 + k))
+    lonely = np.ones(nz, np.uint8)
+    cases.append((lonely, 3))            # no neighbour: FindProjRange empty
+    for skip, target in cases:
+        ref, slab = orc.reproject(stack, tb, tilts, skip, target, 10,
+                                  acc64=False, want_slab=True)
+        ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10, d_out,
+                      mode=mode, d_slab=d_slab)
+        got = d_slab.download()               # [g][tb][2bx][R]
+        flat = got.transpose(0, 3, 1, 2).reshape(groups * rows, tb, 2 * bx)
+        np.testing.assert_array_equal(flat[:by], slab)
+        np.testing.assert_array_equal(d_out.download(), ref)
     for d in (d_stack, d_out, d_slab):
         d.free()
 
