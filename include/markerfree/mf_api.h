@@ -52,6 +52,10 @@ const char* mf_version(void);
 /* ---------------------------------------------------------- device memory */
 /* Plumbing for callers that own device buffers (tests, benchmarks). */
 mf_status mf_dev_malloc(mf_ctx* ctx, size_t bytes, void** d_out);
+/* Device bytes the context and its series hold now, and the most they have
+ * held at once (since creation or the last reset_peak). */
+mf_status mf_ctx_memory(mf_ctx* ctx, size_t* in_use, size_t* peak,
+                        int reset_peak);
 mf_status mf_dev_free(mf_ctx* ctx, void* d_ptr);
 /* Page-locked host memory (hipHostMalloc): stacks uploaded from it and
  * aligned stacks downloaded into it (mf_series_create, mf_correct,
@@ -244,10 +248,15 @@ typedef struct {
                         pair of targets z0+i, z0-i together when neither
                         reads the other's image; 0: one target at a time.
                         Both give identical results. */
+  int graphs;        /* 1: the projection-matching sweeps after the first of
+                        each setup replay a HIP graph captured from the
+                        second (fewer launches; identical results); 0
+                        (default): every sweep launches its kernels.  Sweeps
+                        run eagerly while kernel timing is on. */
 } mf_options;
 
 /* Defaults of InitOpts (main/opts.h:67-81): AlignZ 600, OutBin 1, NPROJ 10;
- * reproj_mode 1, pa_pairs 1. */
+ * reproj_mode 1, pa_pairs 1, graphs 0. */
 void mf_options_default(mf_options* o);
 
 typedef struct mf_series mf_series;
@@ -272,6 +281,18 @@ mf_status mf_series_upload(mf_series* s, int z0, int count, const float* host);
 void mf_series_destroy(mf_series* s);
 const char* mf_series_last_error(const mf_series* s);
 mf_status mf_series_set_verbose(mf_series* s, int verbose);
+/* Graph replay of projection-matching sweeps (mf_options.graphs) for the
+ * stage-by-stage calls; mf_run sets it from its options.  replays (may be
+ * NULL) receives how many sweeps of this series ran as a graph replay.  No
+ * counterpart in the reference. */
+mf_status mf_series_set_graphs(mf_series* s, int on);
+/* Releases the series' stage workspaces (coarse, axis, offset and
+ * projection-matching buffers, the transposed stack), keeping the
+ * preprocessed stack, tilts and parameters: what mf_correct and
+ * mf_get_params need.  A later stage call needs its setup again.  No
+ * counterpart in the reference. */
+mf_status mf_series_trim(mf_series* s);
+mf_status mf_series_graph_replays(const mf_series* s, long long* replays);
 
 /* PreProcess::SetPositive + MassNormalization (method/Preprocess.cu:103-177);
  * reverses descending files into ascending order; zeroes the parameters. */
@@ -352,7 +373,9 @@ mf_status mf_series_trace_clear(mf_series* s);
 
 /* mf_run on n series of one context (sizes and tilt counts may differ),
  * 'concurrency' at a time: each series has its own stream, so the kernels
- * of different series overlap on the device. */
+ * of different series overlap on the device.  Each series is trimmed
+ * (mf_series_trim) after its run, so a finished series holds only its
+ * stack until mf_correct. */
 mf_status mf_batch_run(mf_series** series, int n, const mf_options* o,
                        int concurrency);
 
@@ -380,6 +403,8 @@ void mf_server_destroy(mf_server* sv);
 mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
                     const float* stack_zyx, const float* tilts,
                     const mf_options* o, long long* id);
+/* Seconds since the server was created: the clock of mf_fetch's times3. */
+mf_status mf_server_clock(mf_server* sv, double* seconds);
 mf_status mf_poll(mf_server* sv, long long id, int* state);
 mf_status mf_wait(mf_server* sv, long long id, int* state);
 /* Results of a finished request (any pointer may be NULL); times3 =

@@ -82,7 +82,7 @@ int mfio_read_txt(const char* path, int cap, float* rotate, double* shift_x,
 
 /* Fiducial residuals of an alignment: the evaluation the reference's cal_res
  * tool makes (tool/src/main/main.cpp:35-237, triangulation
- * tool/src/nbundle/bundle_core.cpp:148-216) from marker tracks given as
+ * tool/src/nbundle/bundle_core.cpp:149-216) from marker tracks given as
  * coordinates (its OpenCV marker detection is not part of this library).
  * raw_xy [nmarkers][nz][2]: each marker's raw-image position per section
  * (pixel k at position k; NaN where it is not seen).  Each observation goes

@@ -54,6 +54,8 @@ _sig("mf_ctx_destroy", None, _vp)
 _sig("mf_last_error", C.c_char_p, _vp)
 _sig("mf_ctx_synchronize", C.c_int, _vp)
 _sig("mf_dev_malloc", C.c_int, _vp, C.c_size_t, C.POINTER(_vp))
+_sig("mf_ctx_memory", C.c_int, _vp, C.POINTER(C.c_size_t),
+     C.POINTER(C.c_size_t), C.c_int)
 _sig("mf_dev_free", C.c_int, _vp, _vp)
 _sig("mf_h2d", C.c_int, _vp, _vp, _vp, C.c_size_t)
 _sig("mf_d2h", C.c_int, _vp, _vp, _vp, C.c_size_t)
@@ -282,6 +284,14 @@ class Context:
     def timing_reset(self):
         self._check(_lib.mf_timing_reset(self.h))
 
+    def memory(self, reset_peak=False):
+        """(bytes held now, most bytes held at once) by the context and its
+        series; reset_peak restarts the peak from the current holding."""
+        u, p = C.c_size_t(), C.c_size_t()
+        self._check(_lib.mf_ctx_memory(self.h, C.byref(u), C.byref(p),
+                                       int(reset_peak)))
+        return u.value, p.value
+
     def kernel_time(self, name: str):
         ms = C.c_double()
         n = C.c_longlong()
@@ -327,7 +337,8 @@ class Options(C.Structure):
                 ("zshift", C.c_float), ("thickness", C.c_int),
                 ("align_z", C.c_int), ("out_bin", C.c_int),
                 ("nproj", C.c_int), ("verbose", C.c_int),
-                ("reproj_mode", C.c_int), ("pa_pairs", C.c_int)]
+                ("reproj_mode", C.c_int), ("pa_pairs", C.c_int),
+                ("graphs", C.c_int)]
 
 
 _sig("mf_options_default", None, C.POINTER(Options))
@@ -336,6 +347,9 @@ _sig("mf_series_create", C.c_int, _vp, C.c_int, C.c_int, C.c_int, _fp, _fp,
 _sig("mf_series_destroy", None, _vp)
 _sig("mf_series_last_error", C.c_char_p, _vp)
 _sig("mf_series_set_verbose", C.c_int, _vp, C.c_int)
+_sig("mf_series_set_graphs", C.c_int, _vp, C.c_int)
+_sig("mf_series_graph_replays", C.c_int, _vp, C.POINTER(C.c_longlong))
+_sig("mf_series_trim", C.c_int, _vp)
 _sig("mf_preprocess", C.c_int, _vp)
 _sig("mf_get_stack", C.c_int, _vp, _fp)
 _sig("mf_get_tilts", C.c_int, _vp, _fp)
@@ -672,6 +686,19 @@ class Series:
     def trace_clear(self):
         self._check(_lib.mf_series_trace_clear(self.h))
 
+    def trim(self):
+        """Release the stage workspaces; keep stack, tilts and parameters."""
+        self._check(_lib.mf_series_trim(self.h))
+
+    def set_graphs(self, on=True):
+        """Replay projection-matching sweeps as a captured HIP graph."""
+        self._check(_lib.mf_series_set_graphs(self.h, int(bool(on))))
+
+    def graph_replays(self):
+        n = C.c_longlong()
+        self._check(_lib.mf_series_graph_replays(self.h, C.byref(n)))
+        return n.value
+
 
 # ------------------------------------------------ batched and served forms
 
@@ -691,6 +718,7 @@ _sig("mf_fetch", C.c_int, _vp, C.c_longlong, C.POINTER(C.c_double),
      C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_int),
      C.POINTER(C.c_double), C.c_char_p, C.c_size_t)
 _sig("mf_release", C.c_int, _vp, C.c_longlong)
+_sig("mf_server_clock", C.c_int, _vp, C.POINTER(C.c_double))
 _sig("mf_server_stats", C.c_int, _vp, C.POINTER(C.c_size_t),
      C.POINTER(C.c_size_t), C.POINTER(C.c_int), C.POINTER(C.c_int),
      C.POINTER(C.c_longlong))
@@ -788,6 +816,12 @@ class Server:
     def release(self, rid):
         self.ctx._check(_lib.mf_release(self.h, rid))
         self._shapes.pop(rid, None)
+
+    def clock(self) -> float:
+        """Seconds since the server started (the clock of Result times)."""
+        t = C.c_double()
+        self.ctx._check(_lib.mf_server_clock(self.h, C.byref(t)))
+        return t.value
 
     def stats(self):
         b, r = C.c_size_t(), C.c_size_t()

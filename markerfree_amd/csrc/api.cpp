@@ -99,6 +99,16 @@ mf_status mf_ctx_synchronize(mf_ctx* ctx) {
   });
 }
 
+mf_status mf_ctx_memory(mf_ctx* ctx, size_t* in_use, size_t* peak,
+                        int reset_peak) {
+  if (!ctx) return MF_EINVAL;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (in_use) *in_use = ctx->in_use;
+  if (peak) *peak = ctx->peak;
+  if (reset_peak) ctx->peak = ctx->in_use;
+  return MF_OK;
+}
+
 mf_status mf_dev_malloc(mf_ctx* ctx, size_t bytes, void** d_out) {
   if (!ctx || !d_out) return MF_EINVAL;
   return guard(ctx, [&] {
@@ -187,6 +197,7 @@ mf_status mf_tune_set(mf_ctx* ctx, const char* key, int value) {
       ctx->tune.erase(key);
     else
       ctx->tune[key] = value;
+    ctx->tune_gen++;
   });
 }
 
@@ -248,6 +259,7 @@ mf_status mf_tune_load(mf_ctx* ctx, const char* path, int* count) {
     MF_REQUIRE(i < text.size(), "tuning file: unterminated object");
     std::lock_guard<std::mutex> g(ctx->tune_mu);
     for (auto& p : kv) ctx->tune[p.first] = p.second;
+    ctx->tune_gen++;
     if (count) *count = (int)kv.size();
   });
 }
@@ -556,6 +568,7 @@ void mf_options_default(mf_options* o) {
   o->verbose = 0;
   o->reproj_mode = 1;
   o->pa_pairs = 1;
+  o->graphs = 0;
 }
 
 mf_status mf_series_create(mf_ctx* ctx, int nx, int ny, int nz,
@@ -649,6 +662,20 @@ mf_status mf_series_upload(mf_series* s, int z0, int count, const float* host) {
 
 mf_status mf_series_set_verbose(mf_series* s, int verbose) {
   return sguard(s, [&] { s->verbose = verbose; });
+}
+
+mf_status mf_series_trim(mf_series* s) {
+  return sguard(s, [&] { series_trim(s); });
+}
+
+mf_status mf_series_set_graphs(mf_series* s, int on) {
+  return sguard(s, [&] { s->graphs = on != 0; });
+}
+
+mf_status mf_series_graph_replays(const mf_series* s, long long* replays) {
+  if (!s) return MF_EINVAL;
+  if (replays) *replays = s->pa_replays;
+  return MF_OK;
 }
 
 mf_status mf_preprocess(mf_series* s) {

@@ -5,7 +5,7 @@
 // (tool/src/main/main.cpp:35-237), which reads a Markerfree .txt alignment,
 // detects gold markers in the stack, tracks them and triangulates each track
 // under the alignment (PPBundleApp::Triangulate,
-// tool/src/nbundle/bundle_core.cpp:148-216: per view, the rotated and
+// tool/src/nbundle/bundle_core.cpp:149-216: per view, the rotated and
 // shifted observation equals the projection of the marker's 3-D position at
 // the tilt angle, solved by least squares; the error is the RMS reprojection
 // distance).  Marker detection and matching (OpenCV templates) are outside

@@ -1,3 +1,4 @@
+#include <algorithm>
 #include "context.h"
 
 #include "fft.h"
@@ -91,6 +92,7 @@ void* dev_alloc(mf_ctx* ctx, size_t bytes) {
     if (ctx->budget && ctx->in_use + bytes > ctx->budget)
       throw Error(MF_ENOMEM, "HBM budget of the context exceeded");
     ctx->in_use += bytes;
+    ctx->peak = std::max(ctx->peak, ctx->in_use);
   }
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes);

@@ -1,5 +1,6 @@
 // Library context: device, stream, HBM accounting, kernel timing, workspaces.
 #pragma once
+#include <atomic>
 #include <functional>
 #include <map>
 #include <tuple>
@@ -109,6 +110,7 @@ struct mf_ctx {
   hipStream_t stream = nullptr;
   size_t budget = 0;      // bytes the context may hold (0: unlimited)
   size_t in_use = 0;      // bytes currently allocated through the context
+  size_t peak = 0;        // largest in_use since creation or the last reset
   std::string last_error;
   mf::KernelTimer timer;
   std::map<void*, size_t> user_allocs;  // mf_dev_malloc bookkeeping
@@ -116,6 +118,7 @@ struct mf_ctx {
   std::unique_ptr<mf::FftCache> fft;
   std::map<std::string, std::unique_ptr<mf::DevBuf>> tables;
   std::map<std::string, int> tune;      // launch-shape tunables
+  std::atomic<unsigned long long> tune_gen{0};  // changes to 'tune'
   // Device address of the allocation-free peak search's scratch array on
   // this context's device (cc.hip).
   void* peak_scratch = nullptr;

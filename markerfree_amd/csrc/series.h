@@ -43,6 +43,16 @@ struct mf_series {
   std::vector<float> pa_angles;
   mf::CorrGeom pa_geom{};
   mf::DevBuf pa_corr, pa_reproj, pa_work, pa_mom, pa_dev, pa_slab, pa_tab;
+  // Captured sweep (mf_options.graphs): within one setup every sweep makes
+  // the same launches with the same arguments (only device-resident
+  // parameters change), so the second sweep is captured into a HIP graph
+  // and later sweeps replay it.  pa_ekey: what the last eager sweep's
+  // launches depended on; pa_gkey: the same for the captured graph.
+  bool graphs = false;
+  unsigned long long pa_gen = 0;  // setups so far
+  std::vector<unsigned long long> pa_ekey, pa_gkey;
+  hipGraphExec_t pa_graph = nullptr;
+  long long pa_replays = 0;       // sweeps run by graph replay
   // The last sweep's measured shift (raw pixels, [nz][2]) and its length
   // per target; targets the sweep did not measure hold 0.
   std::vector<float> pa_meas, pa_err;
@@ -71,6 +81,10 @@ void series_projalign_once(mf_series* s);
 // nullptr to query the size only.
 void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by);
 void series_run(mf_series* s, const mf_options* o);
+// Releases the stage workspaces (coarse, axis, offset and projection-matching
+// buffers, the transposed stack); keeps the preprocessed stack, tilts and
+// parameters.  A later stage call needs its setup again.
+void series_trim(mf_series* s);
 
 // Upper bound of the device bytes one series holds through series_run and
 // series_correct (admission control of the served form).

@@ -27,6 +27,7 @@ mf_series::~mf_series() {
     b->release();
   for (hipEvent_t e : up_ev)
     if (e) (void)hipEventDestroy(e);
+  if (pa_graph) (void)hipGraphExecDestroy(pa_graph);
   if (st) (void)hipStreamDestroy(st);
 }
 
@@ -346,12 +347,71 @@ void series_projalign_setup(mf_series* s, float zshift, int align_z,
   s->pa_reproj.ensure(s->ctx, 2 * plane * sizeof(float));
   s->pa_work.ensure(s->ctx, 4 * (size_t)g.by * padb * sizeof(float));
   s->pa_mom.ensure(s->ctx, cc_scratch_doubles(2, g.by) * sizeof(double));
+  // New angles and thickness: a captured sweep no longer applies.
+  s->pa_gen++;
+  s->pa_ekey.clear();
+  if (s->pa_graph) {
+    MF_HIP(hipStreamSynchronize(s->st));
+    MF_HIP(hipGraphExecDestroy(s->pa_graph));
+    s->pa_graph = nullptr;
+  }
+  s->pa_gkey.clear();
   s->pa_ready = true;
   s->ev(MF_EV_PA_SETUP, (float)s->pa_tb);
   if (s->verbose)
     printf("bin:%d   cmpsize:%d,%d\nbinsize:%d, %d\n", s->pa_bin, g.bx / 2 + 1,
            g.by, g.bx, g.by);
 }
+
+namespace {
+
+// Runs one sweep's launches on the series' stream: eagerly, or, with graphs
+// enabled and kernel timing off, by replaying the graph captured from an
+// earlier sweep with the same key.  The first sweep of a key runs eagerly
+// (it also creates the tables, transform plans and launch attributes the
+// launches use), the second is captured, instantiated and launched, later
+// ones are replayed.  The captured launches hold no memory set or copy.
+template <class F>
+void sweep_launches(mf_series* s, const std::vector<unsigned long long>& key,
+                    F&& launch) {
+  const bool can = s->graphs && !s->ctx->timer.enabled();
+  if (can && s->pa_graph && key == s->pa_gkey) {
+    MF_HIP(hipGraphLaunch(s->pa_graph, s->st));
+    s->pa_replays++;
+    return;
+  }
+  if (!can || key != s->pa_ekey) {
+    launch();
+    s->pa_ekey = key;
+    return;
+  }
+  if (s->pa_graph) {
+    MF_HIP(hipStreamSynchronize(s->st));
+    MF_HIP(hipGraphExecDestroy(s->pa_graph));
+    s->pa_graph = nullptr;
+    s->pa_gkey.clear();
+  }
+  hipGraph_t graph = nullptr;
+  MF_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+  try {
+    launch();
+  } catch (...) {
+    (void)hipStreamEndCapture(s->st, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  MF_HIP(hipStreamEndCapture(s->st, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  MF_HIP(e);
+  s->pa_graph = exec;
+  s->pa_gkey = key;
+  MF_HIP(hipGraphLaunch(exec, s->st));
+  s->pa_replays++;
+}
+
+}  // namespace
 
 float series_projalign_iter(mf_series* s) {
   // ProjAlign::DoIt / DoItBin (ProjAlign.cpp:127-179).
@@ -361,10 +421,7 @@ float series_projalign_iter(mf_series* s) {
   const CorrGeom& g = s->pa_geom;
   const size_t plane = (size_t)g.bx * g.by;
   float* corr = (float*)s->pa_corr.get();
-  // Correct + bin + R-weight the whole stack with the current parameters.
   CorrProj* cp = upload_corr(s, corr_params(s, false));
-  correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, 0, nz, true,
-                  corr, 1, stack_t(s));
   // FitRotCenterZ (ProjAlign.cpp:459-486) with dA starting at 0 (quirk Q13),
   // then RemoveOffsetZ(-1) (:488-501).
   const std::vector<float>& ang = s->pa_angles;
@@ -470,44 +527,68 @@ float series_projalign_iter(mf_series* s) {
                           hipMemcpyHostToDevice, s->st));
   }
   std::vector<int> order;
-  int paired = 0;
   for (const Step& st : steps) {
-    // ClacAlignProj (ProjAlign.cpp:181-241), for one target or a pair.
-    const int n = st.b >= 0 ? 2 : 1;
-    const int ts[2] = {st.a, st.b};
-    for (int k = 0; k < n; k++) {
-      reproject_target(ctx, s->st, corr, g.bx, g.by, nz, s->pa_tb, ang.data(),
-                       skip.data(), ts[k], s->pa_nproj, s->reproj_mode,
-                       reproj + k * plane, slab);
-      skip[ts[k]] = 0;  // the next target's range sees this one aligned
-    }
-    // The correlation's fit also applies each target's parameter update.
-    PaUpdate up{};
-    up.meas = (float*)d.meas;
-    up.err = d.err;
-    up.ind = (const float*)d.ind;
-    up.rcs = (const float*)d.rcs;
-    up.sx = d.sx;
-    up.sy = d.sy;
-    up.cp = cp;
-    up.t[0] = st.a;
-    up.t[1] = st.b;
-    up.binx = binx;
-    up.biny = biny;
-    if (n == 2) {
-      pa_measure_tab(ctx, s->st, d_tab + 4 * paired, 2, g.bx, g.by, mom,
-                     (float*)d.ishift, &up);
-      paired++;
-    } else {
-      pa_measure(ctx, s->st, reproj, corr + st.a * plane, g.bx, g.by, work,
-                 mom, (float*)d.ishift, &up);
-    }
-    // Re-correct the step's targets (one launch for a pair).
-    correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, st.a, n,
-                    true, corr + st.a * plane, n == 2 ? st.b - st.a : 1,
-                    stack_t(s));
-    for (int k = 0; k < n; k++) order.push_back(ts[k]);
+    order.push_back(st.a);
+    if (st.b >= 0) order.push_back(st.b);
   }
+  // The sweep's launches.  Their arguments depend only on the tilts, the
+  // schedule and the buffers in 'key'; the parameters they read and update
+  // (cp, d.*) are device-resident and uploaded above.
+  auto launch = [&] {
+    // Correct + bin + R-weight the whole stack with the current parameters.
+    correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, 0, nz,
+                    true, corr, 1, stack_t(s));
+    std::vector<unsigned char> sk = skip;
+    int paired = 0;
+    for (const Step& st : steps) {
+      // ClacAlignProj (ProjAlign.cpp:181-241), for one target or a pair.
+      const int n = st.b >= 0 ? 2 : 1;
+      const int ts[2] = {st.a, st.b};
+      for (int k = 0; k < n; k++) {
+        reproject_target(ctx, s->st, corr, g.bx, g.by, nz, s->pa_tb,
+                         ang.data(), sk.data(), ts[k], s->pa_nproj,
+                         s->reproj_mode, reproj + k * plane, slab);
+        sk[ts[k]] = 0;  // the next target's range sees this one aligned
+      }
+      // The correlation's fit also applies each target's parameter update.
+      PaUpdate up{};
+      up.meas = (float*)d.meas;
+      up.err = d.err;
+      up.ind = (const float*)d.ind;
+      up.rcs = (const float*)d.rcs;
+      up.sx = d.sx;
+      up.sy = d.sy;
+      up.cp = cp;
+      up.t[0] = st.a;
+      up.t[1] = st.b;
+      up.binx = binx;
+      up.biny = biny;
+      if (n == 2) {
+        pa_measure_tab(ctx, s->st, d_tab + 4 * paired, 2, g.bx, g.by, mom,
+                       (float*)d.ishift, &up);
+        paired++;
+      } else {
+        pa_measure(ctx, s->st, reproj, corr + st.a * plane, g.bx, g.by, work,
+                   mom, (float*)d.ishift, &up);
+      }
+      // Re-correct the step's targets (one launch for a pair).
+      correct_rweight(ctx, s->st, (const float*)s->stack.get(), g, cp, st.a, n,
+                      true, corr + st.a * plane, n == 2 ? st.b - st.a : 1,
+                      stack_t(s));
+    }
+  };
+  const std::vector<unsigned long long> key = {
+      s->pa_gen, ctx->tune_gen.load(), (unsigned long long)s->reproj_mode,
+      (unsigned long long)s->pa_pairs, (unsigned long long)s->pa_nproj,
+      (unsigned long long)(uintptr_t)s->stack.get(),
+      (unsigned long long)(uintptr_t)stack_t(s),
+      (unsigned long long)(uintptr_t)corr, (unsigned long long)(uintptr_t)cp,
+      (unsigned long long)(uintptr_t)reproj,
+      (unsigned long long)(uintptr_t)work, (unsigned long long)(uintptr_t)mom,
+      (unsigned long long)(uintptr_t)slab,
+      (unsigned long long)(uintptr_t)d_tab,
+      (unsigned long long)(uintptr_t)d.meas};
+  sweep_launches(s, key, launch);
   std::vector<float> err(nz), meas(2 * nz);
   MF_HIP(hipMemcpyAsync(s->sx.data(), d.sx, nz * 8, hipMemcpyDeviceToHost, s->st));
   MF_HIP(hipMemcpyAsync(s->sy.data(), d.sy, nz * 8, hipMemcpyDeviceToHost, s->st));
@@ -597,11 +678,33 @@ size_t series_hbm_bytes(int nx, int ny, int nz, const mf_options* o) {
   return 2 * raw + coarse + pa + axis + out + raw / 8 + ((size_t)64 << 20);
 }
 
+void series_trim(mf_series* s) {
+  MF_HIP(hipStreamSynchronize(s->st));
+  if (s->pa_graph) {
+    MF_HIP(hipGraphExecDestroy(s->pa_graph));
+    s->pa_graph = nullptr;
+  }
+  s->pa_gkey.clear();
+  s->pa_ekey.clear();
+  for (DevBuf* b : {&s->tmp, &s->tr_corr, &s->tr_work, &s->tr_mom,
+                    &s->tr_shift, &s->tr_tab, &s->cproj, &s->off_corr,
+                    &s->pa_corr, &s->pa_reproj, &s->pa_work, &s->pa_mom,
+                    &s->pa_dev, &s->pa_slab, &s->pa_tab, &s->axw.lines,
+                    &s->axw.reg, &s->axw.small, &s->axw.tstack, &s->axw.negf,
+                    &s->offw.buf})
+    b->release();
+  s->axw.t_of = nullptr;
+  s->axw.neg_of = nullptr;
+  s->tr_ready = false;
+  s->pa_ready = false;
+}
+
 void series_run(mf_series* s, const mf_options* o) {
   // Process::DoIt (Process.cpp:34-92) minus file I/O.
   s->verbose = o->verbose;
   s->reproj_mode = o->reproj_mode;
   s->pa_pairs = o->pa_pairs != 0;
+  s->graphs = o->graphs != 0;
   if (!s->preprocessed) series_preprocess(s);
   if (o->align_z == 0) return;
   // SetParam (Process.cpp:208-221).

@@ -42,11 +42,14 @@ double seconds_since(Clock::time_point t0) {
   return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
-// Free page-locked blocks kept for reuse, up to cap bytes.  The first host
-// write to newly page-locked memory costs about 0.3 ms per MB (page
+// Free page-locked blocks kept for reuse, at most 'cap' of them.  The first
+// host write to newly page-locked memory costs about 0.3 ms per MB (page
 // population), several times a plain copy, so a request's stack copy and
 // its aligned stack go into recycled blocks when one of a fitting size is
-// free (no more than twice the size asked for).
+// free (no more than twice the size asked for).  The server sets the cap to
+// the number of blocks its admitted and queued requests can hold at once
+// (two per running request, one per queued request): the pool keeps no more
+// blocks than the requests themselves can hold.
 class PinnedPool {
  public:
   explicit PinnedPool(size_t cap) : cap_(cap) {}
@@ -63,7 +66,6 @@ class PinnedPool {
       if (it != free_.end() && it->first <= 2 * n) {
         float* p = it->second;
         *got = it->first;
-        held_ -= it->first * sizeof(float);
         free_.erase(it);
         return p;
       }
@@ -76,9 +78,8 @@ class PinnedPool {
   void give(float* p, size_t n) {
     {
       std::lock_guard<std::mutex> lk(mu_);
-      if (held_ + n * sizeof(float) <= cap_) {
+      if (free_.size() < cap_) {
         free_.emplace(n, p);
-        held_ += n * sizeof(float);
         return;
       }
     }
@@ -88,7 +89,7 @@ class PinnedPool {
  private:
   std::mutex mu_;
   std::multimap<size_t, float*> free_;  // size in floats -> block
-  size_t cap_, held_ = 0;
+  size_t cap_;
 };
 
 // Page-locked float buffer from the pool: the request's stack copy and its
@@ -287,6 +288,8 @@ mf_status mf_batch_run(mf_series** series, int n, const mf_options* o,
         i = next++;
       }
       st[i] = mf_run(series[i], o);
+      // Only the stack and the parameters are needed from here on.
+      if (st[i] == MF_OK) st[i] = mf_series_trim(series[i]);
     }
   };
   std::vector<std::thread> pool;
@@ -312,8 +315,8 @@ mf_status mf_server_create(mf_ctx* ctx, int workers, size_t hbm_budget,
   mf_status st = guarded(&err, [&] {
     sv = new mf_server;
     sv->ctx = ctx;
-    // Recycled page-locked blocks: up to 4 GB of them.
-    sv->pool.reset(new PinnedPool((size_t)4 << 30));
+    // Recycled page-locked blocks: as many as the requests can hold.
+    sv->pool.reset(new PinnedPool((size_t)2 * workers + queue_cap));
     size_t budget = hbm_budget ? hbm_budget : ctx->budget;
     if (!budget) {
       size_t free_b = 0, total_b = 0;
@@ -377,6 +380,12 @@ mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
     *id = r->id;
   }
   sv->cv_work.notify_all();
+  return MF_OK;
+}
+
+mf_status mf_server_clock(mf_server* sv, double* seconds) {
+  if (!sv || !seconds) return MF_EINVAL;
+  *seconds = seconds_since(sv->t0);
   return MF_OK;
 }
 

@@ -202,6 +202,9 @@ int mfs_generate(const mfs_config* c, float* stack, float* tilts,
   }
 
   // Noise: Gaussian at a per-tilt SNR, or Poisson at 'dose' electrons/px.
+  // Each section draws from its own stream, so the sections run in parallel
+  // with the same result.
+#pragma omp parallel for schedule(dynamic, 1)
   for (int i = 0; i < nz; i++) {
     float* img = stack + (size_t)i * plane;
     Pcg32 rn(c->seed_noise + 7919ULL * (uint64_t)i, 3);
@@ -226,13 +229,16 @@ int mfs_generate(const mfs_config* c, float* stack, float* tilts,
   float scale = 1.0f;
   if (c->int16_counts) {
     float mn = stack[0], mx = stack[0];
-    for (size_t k = 0; k < plane * nz; k++) {
+    const long long total = (long long)plane * nz;
+#pragma omp parallel for reduction(min : mn) reduction(max : mx)
+    for (long long k = 0; k < total; k++) {
       mn = std::min(mn, stack[k]);
       mx = std::max(mx, stack[k]);
     }
     // Map [mn, mx] to [-30000, 30000] counts, rounded to integers.
     scale = 60000.0f / std::max(mx - mn, 1e-20f);
-    for (size_t k = 0; k < plane * nz; k++)
+#pragma omp parallel for
+    for (long long k = 0; k < total; k++)
       stack[k] = std::nearbyint((stack[k] - mn) * scale - 30000.0f);
   }
   if (scale_out) *scale_out = scale;

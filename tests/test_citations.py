@@ -1,6 +1,7 @@
 """Every reference citation in the repository (File.ext:N or File.ext:N-M,
-with same-file continuations ', N-M' or ', :N-M') names lines that exist in
-the cited reference file (line counts in tests/golden/
+with same-file continuations ', N-M' or ', :N-M'; the evaluation tool's
+files as tool/src/dir/File.ext) names lines that exist in the cited
+reference file (line counts in tests/golden/
 reference_line_counts.json)."""
 import json
 import os
@@ -10,8 +11,8 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent.parent
 COUNTS = json.loads((ROOT / "tests" / "golden" /
                      "reference_line_counts.json").read_text())
-PAT = re.compile(r"([A-Za-z0-9_&]+\.(?:cu|cpp|h)):(\d+)(?:-(\d+))?"
-                 r"((?:,\s*:?\d+(?:-\d+)?)*)")
+PAT = re.compile(r"((?:[A-Za-z0-9_]+/)*[A-Za-z0-9_&]+\.(?:cu|cpp|h)):(\d+)"
+                 r"(?:-(\d+))?((?:,\s*:?\d+(?:-\d+)?)*)")
 SKIP = {"SURVEY.md", "VERDICT.md", "ADVICE.md", "BASELINE.md"}
 
 
@@ -33,7 +34,10 @@ def test_cited_ranges_exist():
     for p in _sources():
         for ln, line in enumerate(p.read_text(errors="replace").splitlines(), 1):
             for m in PAT.finditer(line):
+                # The tool's files by path, the method's by file name.
                 f = m.group(1)
+                if not f.startswith("tool/src/"):
+                    f = f.rsplit("/", 1)[-1]
                 if f not in COUNTS:
                     continue
                 ranges = [(int(m.group(2)), int(m.group(3) or m.group(2)))]

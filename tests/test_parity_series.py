@@ -83,6 +83,14 @@ def _check_input(series, fx):
         "the generator did not reproduce the fixture's input"
 
 
+def test_generator_reproduces_fixture_input(cfg1):
+    """The seeded generator is deterministic (sections are projected and
+    given noise in parallel, each from its own random stream): the 512^2 x
+    41 workload regenerates, bit for bit, the stack the end-to-end fixture
+    was made from."""
+    _check_input(cfg1, _fixture("small"))
+
+
 @pytest.mark.gpu
 def test_gpu_run_stage_sequence(ctx, mf, orc, cfg1):
     """mf_run's stage trace on the 512^2 x 41 workload: the same stage

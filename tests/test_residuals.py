@@ -1,6 +1,6 @@
 """Fiducial residual evaluator (the cal_res tool's evaluation,
 tool/src/main/main.cpp:35-237, triangulation
-tool/src/nbundle/bundle_core.cpp:148-216), an accuracy check independent of
+tool/src/nbundle/bundle_core.cpp:149-216), an accuracy check independent of
 parity: marker tracks with known 3-D positions come from the generator, the
 alignment from the GPU run or from the oracle's run of the same series.
 

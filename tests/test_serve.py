@@ -149,3 +149,53 @@ def test_gpu_streamed_upload_equals_whole_stack(ctx, mf):
             G.upload(0, bufs[0][:1])
         G.close()
 
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [1, 6, 0])
+def test_gpu_admission_charge_bounds_device_use(ctx, mf, mode):
+    """The served form admits against mf_series_hbm_bytes: the most device
+    memory the context holds while one series is created, aligned, corrected
+    and destroyed (the context's own lazily made tables included) stays
+    within that bound, for each reprojection form and two shapes."""
+    for s, opts in ((_mix()[0], dict(OPTS)),
+                    (_mix()[1], dict(align_z=96, nproj=10, out_bin=2))):
+        nz, ny, nx = s.stack.shape
+        opts["reproj_mode"] = mode
+        bound = mf.series_hbm_bytes(nx, ny, nz, **opts)
+        ctx.synchronize()
+        u0, _ = ctx.memory(reset_peak=True)
+        G = mf.Series(ctx, s.stack, s.tilts)
+        G.run(**opts)
+        G.correct(opts.get("out_bin", 1))
+        _, peak = ctx.memory()
+        G.close()
+        used = peak - u0
+        print(f"{nx}x{ny}x{nz} mode {mode}: peak {used / 2**20:.1f} MiB, "
+              f"bound {bound / 2**20:.1f} MiB")
+        assert 0 < used <= bound, (used, bound)
+        assert ctx.memory()[0] <= u0 + (1 << 20)   # released again
+
+
+@pytest.mark.gpu
+def test_gpu_trim_keeps_stack_and_parameters(ctx, mf):
+    """mf_batch_run trims every series after its run: the series then holds
+    little more than its preprocessed stack, its correction and parameters
+    equal an untrimmed run's, stage calls ask for their setup again, and a
+    new setup works."""
+    s = _mix()[0]
+    ref = _alone(mf, ctx, s)
+    nz, ny, nx = s.stack.shape
+    ctx.synchronize()
+    u0, _ = ctx.memory()
+    G = mf.Series(ctx, s.stack, s.tilts)
+    mf.batch_run([G], concurrency=1, **OPTS)
+    held = ctx.memory()[0] - u0
+    assert held <= 4 * nx * ny * nz + (1 << 20), held
+    _same((G.params(), G.correct(1)), ref)
+    with pytest.raises(mf.MFError) as e:
+        G.projalign_iter()
+    assert e.value.code == mf.MF_ESTATE
+    G.projalign_setup(0.0, OPTS["align_z"], OPTS["nproj"])
+    assert np.isfinite(G.projalign_iter())
+    G.close()
