@@ -164,115 +164,224 @@ def cpu_baseline(series, pa_targets, pa_sweeps, n_targets=20):
                        f"{per_series:.0f} s/series")}
 
 
+MIX_SIZES = [(1024, 1024), (2048, 2048), (5760, 4092)]
+
+
 def mixture(synth, n, seed=7):
-    """Series of the batched configuration's mixture: sizes from {1024^2,
-    2048^2}, tilt counts U{35..61}, ranges +-U[50, 60] degrees, thickness
-    {150, 300}, NPROJ 10, seeded per series.  (The 4092x5760 size of the
-    mixture is left out of this bounded run.)"""
+    """Configurations of the batched workload's mixture: sizes uniform from
+    {1024^2, 2048^2, 5760x4092}, tilt counts U{35..61}, ranges +-U[50, 60]
+    degrees, thickness {150, 300}, NPROJ 10, float32, seeded per series.
+    Phantom and noise as the single-series workload (3000 blobs per 1024^2 of
+    area, at most 12000; Gaussian noise at SNR 0.05; axis U(80, 95)); shifts
+    U(-40, 40) px, U(-80, 80) px at 5760x4092."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
-        side = int(rng.choice([1024, 2048]))
+        nx, ny = MIX_SIZES[int(rng.integers(0, len(MIX_SIZES)))]
         nz = int(rng.integers(35, 62))
         rmax = float(rng.uniform(50, 60))
-        step = 2 * rmax / (nz - 1)
-        cfg = synth.SeriesConfig(nx=side, ny=side, nz=nz,
-                                 vol=(side, side, 300),
-                                 nblobs=3000 * (side // 1024) ** 2,
-                                 tilt_min=-rmax, tilt_step=step,
-                                 axis=(80.0, 95.0), shift_max=40.0, snr=0.05,
-                                 int16=True, seeds=(100 + i, 200 + i, 300 + i))
-        out.append((synth.generate(cfg), int(rng.choice([150, 300]))))
+        cfg = synth.SeriesConfig(
+            nx=nx, ny=ny, nz=nz, vol=(nx, ny, 300),
+            nblobs=min(3000 * nx * ny // (1 << 20), 12000),
+            tilt_min=-rmax, tilt_step=2 * rmax / (nz - 1), axis=(80.0, 95.0),
+            shift_max=80.0 if nx > 2048 else 40.0, snr=0.05,
+            seeds=(100 + i, 200 + i, 300 + i))
+        out.append((cfg, int(rng.choice([150, 300]))))
     return out
 
 
-def bench_batched(mf, synth, args):
-    """The batched form on a mixture: every series aligned and its stack
-    downloaded, 'concurrency' series at a time (mf_batch_run), against the
-    same series one after another (the plain per-series form)."""
-    mix = mixture(synth, args.batch)
-    ctx = mf.Context(0)
-    groups = {}
-    for s, tb in mix:
-        groups.setdefault(tb, []).append(s)
+def generate_all(synth, specs, label):
+    t0 = time.perf_counter()
+    out = []
+    for k, (cfg, tb) in enumerate(specs):
+        out.append((synth.generate(cfg), tb))
+        if (k + 1) % 8 == 0 or k + 1 == len(specs):
+            print(f"{label}: generated {k + 1}/{len(specs)} series "
+                  f"({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
+                  flush=True)
+    return out
 
-    def run(concurrency):
-        pool = [(mf.Series(ctx, s.stack, s.tilts), tb) for s, tb in mix]
+
+def _sample_sum(a):
+    # A sampled checksum of an aligned stack (every 61st row, 67th column).
+    return float(a[:, ::61, ::67].sum(dtype=np.float64))
+
+
+def bench_batched(mf, synth, args):
+    """The streaming batched form (mf_batch_run_each) on the mixture: every
+    series aligned, corrected and its aligned stack handed to the caller in
+    host memory, 'concurrency' series at a time, largest first; against the
+    same series one at a time (the plain per-series form).  Raw stacks are
+    resident in HBM before each timed pass."""
+    mix = generate_all(synth, mixture(synth, args.batch), "batched")
+    n = len(mix)
+    opts = [mf.default_options(align_z=tb, nproj=10) for _, tb in mix]
+    ctx = mf.Context(0)
+    # Warm-up (tables, transform plans, page-locked output blocks): one
+    # series of each size.
+    warm, seen = [], set()
+    for i, (s, _) in enumerate(mix):
+        if s.stack.shape[1:] not in seen:
+            seen.add(s.stack.shape[1:])
+            warm.append(i)
+    W = [mf.Series(ctx, mix[i][0].stack, mix[i][0].tilts) for i in warm]
+    mf.batch_run_each(W, [opts[i] for i in warm],
+                      concurrency=args.concurrency, correct=True)
+    for g in W:
+        g.close()
+
+    def run(concurrency, label):
+        pool = [mf.Series(ctx, s.stack, s.tilts) for s, _ in mix]
         ctx.synchronize()
+        ctx.memory(reset_peak=True)
+        sums, t_done = [None] * n, [None] * n
+        count = [0]
+
+        def on_done(i, st, a):
+            sums[i] = _sample_sum(a)
+            t_done[i] = time.perf_counter()
+            count[0] += 1
+            if count[0] % 8 == 0 or count[0] == n:
+                print(f"{label}: {count[0]}/{n} series aligned "
+                      f"({t_done[i] - t0:.1f} s)", file=sys.stderr,
+                      flush=True)
+
         t0 = time.perf_counter()
-        for tb in sorted(groups):
-            ser = [g for g, t in pool if t == tb]
-            mf.batch_run(ser, concurrency=concurrency, align_z=tb, nproj=10)
-        for g, _ in pool:
-            g.correct(1)
+        mf.batch_run_each(pool, opts, concurrency=concurrency, correct=True,
+                          on_done=on_done)
         ctx.synchronize()
         dt = time.perf_counter() - t0
-        params = [g.params() for g, _ in pool]
-        for g, _ in pool:
+        peak = ctx.memory()[1]
+        params = [g.params() for g in pool]
+        for g in pool:
             g.close()
-        return dt, params
+        return dt, params, sums, peak
 
-    run(1)                                  # warm-up (tables, plans)
-    t_plain, p_plain = run(1)
-    t_batch, p_batch = run(args.concurrency)
-    same = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
-               for a, b in zip(p_plain, p_batch))
+    t_plain, p_plain, s_plain, _ = run(1, "plain")
+    t_batch, p_batch, s_batch, peak = run(args.concurrency, "batched")
+    same = s_plain == s_batch and all(
+        all(np.array_equal(np.asarray(x), np.asarray(y)) for x, y in zip(a, b))
+        for a, b in zip(p_plain, p_batch))
     ctx.close()
-    n = len(mix)
+    counts = {f"{nx}x{ny}": sum(1 for s, _ in mix
+                                if s.stack.shape[1:] == (ny, nx))
+              for nx, ny in MIX_SIZES}
     out = {
         "metric": "tilt series aligned/hour (batched form)",
         "value": round(3600.0 * n / t_batch, 2), "unit": "series/hour",
         "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
-        "data": "synthetic mixture (seeded)",
-        "config": {"workload": f"{n} series, sizes {{1024^2, 2048^2}}, "
-                               "tilts U{35..61}, range +-U[50,60], T "
-                               "{150,300}, NPROJ 10",
-                   "concurrency": args.concurrency,
+        "data": "synthetic mixture (seeded per series)",
+        "config": {"workload": f"{n} series, sizes {{1024^2, 2048^2, "
+                               "5760x4092}, tilts U{35..61}, range "
+                               "+-U[50,60], T {150,300}, NPROJ 10, float32",
+                   "concurrency": args.concurrency, "series_per_size": counts,
                    "sizes": [[s.stack.shape[2], s.stack.shape[1],
                               s.stack.shape[0], tb] for s, tb in mix]},
+        "seconds": round(t_batch, 2),
+        "plain_same_mix_seconds": round(t_plain, 2),
         "plain_same_mix_series_per_hour": round(3600.0 * n / t_plain, 2),
         "speedup_vs_plain": round(t_plain / t_batch, 3),
+        "peak_hbm_gb": round(peak / 1e9, 1),
         "results_identical_to_plain": bool(same),
     }
     print(json.dumps(out), flush=True)
 
 
 def bench_served(mf, synth, args):
-    """The served form: 1024^2 x 61 series (T=300, NPROJ 10) arriving as a
-    Poisson stream at each rate; latency = submission to results in host
-    memory (parameters and aligned stack)."""
-    cfg = synth.PRESETS["single"]
-    base = synth.generate(cfg)
-    opts = dict(align_z=300, nproj=10)
+    """The served form: series drawn (seeded) from a pool of base series of
+    the batched mixture arrive as a Poisson stream at each rate.  Latency
+    runs from the scheduled arrival to the parameters and the aligned stack
+    in host memory; a submission the full queue refuses waits for the oldest
+    outstanding request and is retried, its wait counted in its latency.
+    The plain per-series form is the same server with one worker and one
+    request at a time."""
+    gen = generate_all(synth, mixture(synth, args.pool, seed=11), "served")
+    # The base stacks sit in page-locked memory, read in place by the server
+    # (mf_submit_borrowed), as a service reading files into its own staging
+    # buffers would hand them over.
+    base = []
+    for s, tb in gen:
+        pin = mf.pinned_empty(s.stack.shape)
+        pin[:] = s.stack
+        base.append((pin, s.tilts, tb))
+    del gen
+    opts = [mf.default_options(align_z=tb, nproj=10) for _, _, tb in base]
     ctx = mf.Context(0)
     rows = []
+    with mf.Server(ctx, workers=1, queue_cap=2) as sv:
+        for (st, tl, _), o in zip(base, opts):         # warm-up
+            rid = sv.submit(st, tl, o, borrow=True)
+            sv.wait(rid)
+            sv.release(rid)
+        plain = []
+        for (st, tl, _), o in zip(base, opts):
+            t = time.perf_counter()
+            rid = sv.submit(st, tl, o, borrow=True)
+            sv.wait(rid)
+            plain.append(time.perf_counter() - t)
+            sv.release(rid)
+    mean_plain = float(np.mean(plain))
+    print(json.dumps({"served_plain_s": [round(x, 3) for x in plain]}),
+          file=sys.stderr, flush=True)
     with mf.Server(ctx, workers=args.concurrency,
-                   queue_cap=args.requests + 1) as sv:
-        rid = sv.submit(base.stack, base.tilts, **opts)   # warm-up
-        sv.wait(rid)
-        sv.release(rid)
+                   queue_cap=args.queue) as sv:
+        budget = sv.stats()["budget"]
+        charges = {f"{st.shape[2]}x{st.shape[1]}x{st.shape[0]}":
+                   round(mf.series_hbm_bytes(st.shape[2], st.shape[1],
+                                             st.shape[0], o) / 1e9, 2)
+                   for (st, _, _), o in zip(base, opts)}
         for lam in [float(r) for r in args.rates.split(",")]:
             rng = np.random.default_rng(int(lam * 100))
-            gaps = rng.exponential(1.0 / lam, args.requests)
+            draw = rng.integers(0, len(base), args.requests)
+            sched = np.cumsum(rng.exponential(1.0 / lam, args.requests))
             t0 = time.perf_counter()
-            due = t0
-            ids = []
-            for g in gaps:
-                due += g
-                dt = due - time.perf_counter()
+            c0 = sv.clock()
+            ids, lat, done = [], [], []
+
+            def collect(block):
+                # Results in submission order, each released as soon as it
+                # is read (the server holds a result until released).
+                while len(done) < len(ids):
+                    r = ids[len(done)]
+                    if not block and sv.poll(r) not in (mf.REQ_DONE,
+                                                        mf.REQ_FAILED):
+                        return
+                    sv.wait(r)
+                    res = sv.fetch(r, aligned=False)
+                    sv.release(r)
+                    lat.append(res.t_done - (c0 + sched[len(done)]))
+                    done.append(res.t_done)
+                    if len(done) % 50 == 0:
+                        print(f"rate {lam}: completed {len(done)}/"
+                              f"{args.requests}", file=sys.stderr, flush=True)
+                    if block:
+                        return
+
+            for k in range(args.requests):
+                collect(False)
+                dt = t0 + sched[k] - time.perf_counter()
                 if dt > 0:
                     time.sleep(dt)
-                ids.append(sv.submit(base.stack, base.tilts, **opts))
-            res = []
-            for r in ids:
-                sv.wait(r)
-                res.append(sv.fetch(r, aligned=False))
-                sv.release(r)
-            lat = np.array([x.latency for x in res])
-            span = max(x.t_done for x in res) - min(x.t_submit for x in res)
-            rows.append({"rate_per_s": lam, "requests": len(res),
+                st, tl, _ = base[draw[k]]
+                while True:
+                    try:
+                        ids.append(sv.submit(st, tl, opts[draw[k]],
+                                             borrow=True))
+                        break
+                    except mf.MFError as e:
+                        if e.code != mf.MF_EBUSY:
+                            raise
+                        collect(True)   # the oldest outstanding request
+                if (k + 1) % 50 == 0:
+                    print(f"rate {lam}: submitted {k + 1}/{args.requests}",
+                          file=sys.stderr, flush=True)
+            while len(done) < len(ids):
+                collect(True)
+            lat = np.array(lat)
+            span = max(done) - c0
+            rows.append({"rate_per_s": lam, "requests": len(done),
                          "throughput_series_per_hour":
-                             round(3600.0 * len(res) / span, 1),
+                             round(3600.0 * len(done) / span, 1),
                          "p50_ms": round(1000 * float(np.percentile(lat, 50)), 1),
                          "p99_ms": round(1000 * float(np.percentile(lat, 99)), 1)})
             print(json.dumps({"served_rate": rows[-1]}), file=sys.stderr,
@@ -283,10 +392,20 @@ def bench_served(mf, synth, args):
         "metric": "tilt series aligned/hour + p50/p99 latency (served form)",
         "value": best, "unit": "series/hour", "n_gpus": 1,
         "higher_is_better": True, "dtype": "f32",
-        "data": "synthetic (seeded 3000-blob phantom, int16 counts)",
-        "config": {"workload": "Poisson arrivals of 1024x1024x61 series, "
-                               "T=300, NPROJ 10",
-                   "workers": args.concurrency},
+        "data": f"synthetic: {len(base)} seeded base series of the batched "
+                "mixture, requests drawn from them",
+        "config": {"workload": "Poisson arrivals drawn from the mixture "
+                               "{1024^2, 2048^2, 5760x4092}, tilts "
+                               "U{35..61}, T {150,300}, NPROJ 10",
+                   "workers": args.concurrency, "queue_cap": args.queue,
+                   "base_sizes": [[st.shape[2], st.shape[1], st.shape[0], tb]
+                                  for st, _, tb in base]},
+        "admission": {"budget_gb": round(budget / 1e9, 1),
+                      "charge_gb": charges},
+        "plain_seconds_per_series": [round(x, 3) for x in plain],
+        "plain_series_per_hour": round(3600.0 / mean_plain, 1),
+        "plain_p50_ms": round(1000 * float(np.percentile(plain, 50)), 1),
+        "plain_p99_ms": round(1000 * float(np.percentile(plain, 99)), 1),
         "rates": rows,
     }
     print(json.dumps(out), flush=True)
@@ -304,9 +423,15 @@ def main():
                     default="plain",
                     help="plain: the per-series form (the reported metric); "
                          "batched / served: the multi-series forms")
-    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=64,
+                    help="batched form: series in the mixture")
     ap.add_argument("--concurrency", type=int, default=3)
-    ap.add_argument("--requests", type=int, default=16)
+    ap.add_argument("--requests", type=int, default=200,
+                    help="served form: requests per rate")
+    ap.add_argument("--pool", type=int, default=12,
+                    help="served form: base series requests are drawn from")
+    ap.add_argument("--queue", type=int, default=8,
+                    help="served form: requests that may wait")
     ap.add_argument("--rates", default="0.5,1,2,4,8")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -379,6 +379,23 @@ mf_status mf_series_trace_clear(mf_series* s);
 mf_status mf_batch_run(mf_series** series, int n, const mf_options* o,
                        int concurrency);
 
+/* The streaming batched form: mf_run on each series with its own options
+ * (opts[i], n entries), 'concurrency' series at a time on their own
+ * streams, largest series (nx*ny*nz) first; each series is trimmed after
+ * its run.  correct nonzero: each series is then corrected at its
+ * opts[i].out_bin into a page-locked block of the worker (kept by the
+ * context between calls) and trimmed again.  done (may be NULL) is called
+ * from the worker once per series, with the series' index, status and,
+ * when corrected, its aligned stack [nz][by][bx]; that memory is valid only
+ * during the call.  Callbacks of different workers run concurrently.
+ * Returns the first failing status in index order. */
+typedef void (*mf_batch_done)(int index, mf_status status,
+                              const float* aligned, int bx, int by, int nz,
+                              void* user);
+mf_status mf_batch_run_each(mf_series** series, int n, const mf_options* opts,
+                            int concurrency, int correct, mf_batch_done done,
+                            void* user);
+
 /* Upper bound of the device bytes one series of this shape holds from
  * mf_series_create through mf_correct (the served form's admission
  * charge). */
@@ -399,10 +416,20 @@ mf_status mf_server_create(mf_ctx* ctx, int workers, size_t hbm_budget,
 /* Waits for running series; queued ones fail with "server stopped". */
 void mf_server_destroy(mf_server* sv);
 /* MF_EBUSY: 'queue_cap' requests already wait; MF_ENOMEM: the series could
- * never fit the budget, or no page-locked host memory is left for its copy. */
+ * never fit the budget, or no host memory is left for its copy (the message
+ * says which, mf_last_error of the server's context).  The copy goes to
+ * page-locked memory where the system grants it, else to ordinary memory. */
 mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
                     const float* stack_zyx, const float* tilts,
                     const mf_options* o, long long* id);
+/* mf_submit without the copy: the server reads stack_zyx in place when the
+ * request starts, so the caller keeps it valid and unchanged until the
+ * request is done or failed (mf_poll / mf_wait).  A stack in page-locked
+ * memory (mf_host_alloc) is then uploaded at the PCIe rate with no host
+ * copy at all. */
+mf_status mf_submit_borrowed(mf_server* sv, int nx, int ny, int nz,
+                             const float* stack_zyx, const float* tilts,
+                             const mf_options* o, long long* id);
 /* Seconds since the server was created: the clock of mf_fetch's times3. */
 mf_status mf_server_clock(mf_server* sv, double* seconds);
 mf_status mf_poll(mf_server* sv, long long id, int* state);

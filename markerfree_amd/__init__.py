@@ -718,6 +718,8 @@ _sig("mf_fetch", C.c_int, _vp, C.c_longlong, C.POINTER(C.c_double),
      C.POINTER(C.c_float), C.POINTER(C.c_int), C.POINTER(C.c_int),
      C.POINTER(C.c_double), C.c_char_p, C.c_size_t)
 _sig("mf_release", C.c_int, _vp, C.c_longlong)
+_sig("mf_submit_borrowed", C.c_int, _vp, C.c_int, C.c_int, C.c_int, _fp, _fp,
+     C.POINTER(Options), C.POINTER(C.c_longlong))
 _sig("mf_server_clock", C.c_int, _vp, C.POINTER(C.c_double))
 _sig("mf_server_stats", C.c_int, _vp, C.POINTER(C.c_size_t),
      C.POINTER(C.c_size_t), C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -731,11 +733,45 @@ def series_hbm_bytes(nx, ny, nz, options: Options = None, **kw) -> int:
     return int(_lib.mf_series_hbm_bytes(nx, ny, nz, C.byref(o)))
 
 
+_BATCH_DONE = C.CFUNCTYPE(None, C.c_int, C.c_int, _fp, C.c_int, C.c_int,
+                         C.c_int, _vp)
+_sig("mf_batch_run_each", C.c_int, C.POINTER(_vp), C.c_int, C.POINTER(Options),
+     C.c_int, C.c_int, _BATCH_DONE, _vp)
+
+
 def batch_run(series, options: Options = None, concurrency=4, **kw):
     """mf_batch_run: mf_run on every series, 'concurrency' at a time."""
     o = options if options is not None else default_options(**kw)
     arr = (_vp * len(series))(*[s.h for s in series])
     st = _lib.mf_batch_run(arr, len(series), C.byref(o), concurrency)
+    if st:
+        msgs = [_lib.mf_series_last_error(s.h).decode() for s in series]
+        raise MFError(st, "; ".join(m for m in msgs if m))
+
+
+def batch_run_each(series, options, concurrency=4, correct=False,
+                   on_done=None):
+    """mf_batch_run_each: every series with its own Options, largest first;
+    on_done(index, status, aligned) runs in the worker after each series,
+    'aligned' a read-only view of its aligned stack (correct=True) valid only
+    during the call, else None."""
+    n = len(series)
+    assert len(options) == n
+    arr = (_vp * n)(*[s.h for s in series])
+    opts = (Options * n)(*options)
+
+    def _done(i, st, ptr, bx, by, nz, _user):
+        if on_done is None:
+            return
+        a = None
+        if ptr:
+            a = np.ctypeslib.as_array(ptr, shape=(nz, by, bx))
+            a.flags.writeable = False
+        on_done(i, st, a)
+
+    cb = _BATCH_DONE(_done)
+    st = _lib.mf_batch_run_each(arr, n, opts, concurrency, int(bool(correct)),
+                                cb, None)
     if st:
         msgs = [_lib.mf_series_last_error(s.h).decode() for s in series]
         raise MFError(st, "; ".join(m for m in msgs if m))
@@ -762,21 +798,31 @@ class Server:
                                          C.byref(h)))
         self.ctx, self.h = ctx, h.value
         self._shapes = {}
+        self._borrowed = {}
         ctx._series.add(self)
 
-    def submit(self, stack, tilts, options: Options = None, **kw) -> int:
+    def submit(self, stack, tilts, options: Options = None, borrow=False,
+               **kw) -> int:
+        """mf_submit (the server copies the stack), or with borrow=True
+        mf_submit_borrowed (the server reads 'stack' in place; this object
+        keeps a reference to it until the request is released)."""
         o = options if options is not None else default_options(**kw)
+        if borrow:
+            assert stack.dtype == np.float32 and stack.flags.c_contiguous
         stack = np.ascontiguousarray(stack, np.float32)
         t = np.ascontiguousarray(tilts, np.float32)
         nz, ny, nx = stack.shape
         rid = C.c_longlong()
-        st = _lib.mf_submit(self.h, nx, ny, nz, f32p(stack), f32p(t),
-                            C.byref(o), C.byref(rid))
+        fn = _lib.mf_submit_borrowed if borrow else _lib.mf_submit
+        st = fn(self.h, nx, ny, nz, f32p(stack), f32p(t), C.byref(o),
+                C.byref(rid))
         if st:
-            raise MFError(st, {MF_EBUSY: "queue full",
-                               MF_ENOMEM: "series exceeds the HBM budget"}
-                          .get(st, "submit failed"))
+            msg = "queue full" if st == MF_EBUSY else \
+                _lib.mf_last_error(self.ctx.h).decode() or "submit failed"
+            raise MFError(st, msg)
         self._shapes[rid.value] = nz
+        if borrow:
+            self._borrowed[rid.value] = stack
         return rid.value
 
     def poll(self, rid) -> int:
@@ -816,6 +862,7 @@ class Server:
     def release(self, rid):
         self.ctx._check(_lib.mf_release(self.h, rid))
         self._shapes.pop(rid, None)
+        self._borrowed.pop(rid, None)
 
     def clock(self) -> float:
         """Seconds since the server started (the clock of Result times)."""

@@ -18,21 +18,43 @@ using namespace mf;
 
 namespace {
 
+// Runs f; a thrown error becomes a status and its message.  HIP keeps the
+// last failed call's error per thread and hands it to the next
+// hipGetLastError, so it is cleared here: the failure has been reported, and
+// the next call on this thread (another request of a server worker, for
+// example) must not inherit it.
 template <typename F>
-mf_status guard(mf_ctx* ctx, F&& f) {
+mf_status guard_msg(std::string* msg, F&& f) {
+  mf_status st = MF_OK;
   try {
     f();
     return MF_OK;
   } catch (const Error& e) {
-    if (ctx) ctx->last_error = e.what();
-    return e.code;
+    *msg = e.what();
+    st = e.code;
   } catch (const std::bad_alloc&) {
-    if (ctx) ctx->last_error = "host allocation failed";
-    return MF_ENOMEM;
+    *msg = "host allocation failed";
+    st = MF_ENOMEM;
   } catch (const std::exception& e) {
-    if (ctx) ctx->last_error = e.what();
-    return MF_EDEVICE;
+    *msg = e.what();
+    st = MF_EDEVICE;
   }
+  (void)hipGetLastError();
+  return st;
+}
+
+void set_ctx_error(mf_ctx* ctx, const std::string& msg) {
+  if (!ctx) return;
+  std::lock_guard<std::mutex> g(ctx->err_mu);
+  ctx->last_error = msg;
+}
+
+template <typename F>
+mf_status guard(mf_ctx* ctx, F&& f) {
+  std::string msg;
+  const mf_status st = guard_msg(&msg, f);
+  if (st != MF_OK) set_ctx_error(ctx, msg);
+  return st;
 }
 
 struct DeviceScope {
@@ -142,8 +164,10 @@ mf_status mf_host_alloc(size_t bytes, void** h_out) {
   if (!h_out) return MF_EINVAL;
   *h_out = nullptr;
   void* p = nullptr;
-  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();  // reported here; later checks must not see it
     return MF_ENOMEM;
+  }
   *h_out = p;
   return MF_OK;
 }
@@ -623,11 +647,15 @@ namespace {
 template <typename F>
 mf_status sguard(mf_series* s, F&& f) {
   if (!s) return MF_EINVAL;
-  mf_status st = guard(s->ctx, [&] {
+  std::string msg;
+  const mf_status st = guard_msg(&msg, [&] {
     DeviceScope d(s->ctx);
     f();
   });
-  if (st != MF_OK) s->last_error = s->ctx->last_error;
+  if (st != MF_OK) {
+    s->last_error = msg;
+    set_ctx_error(s->ctx, msg);
+  }
   return st;
 }
 

@@ -1,5 +1,6 @@
 #include <algorithm>
 #include "context.h"
+#include "pinned.h"
 
 #include "fft.h"
 
@@ -97,6 +98,7 @@ void* dev_alloc(mf_ctx* ctx, size_t bytes) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
+    (void)hipGetLastError();  // thrown below; later checks must not see it
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->in_use -= bytes;
     throw_hip(e, "hipMalloc", __FILE__, __LINE__);
@@ -180,6 +182,7 @@ mf_ctx::~mf_ctx() {
   slab_ws.release();
   tables.clear();
   fft.reset();
+  out_pool.reset();
   for (auto& kv : user_allocs) (void)hipFree(kv.first);
   user_allocs.clear();
   timer.reset();

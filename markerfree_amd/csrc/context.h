@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.h"
+#include "pinned.h"
 
 struct mf_ctx;
 
@@ -116,6 +117,9 @@ struct mf_ctx {
   std::map<void*, size_t> user_allocs;  // mf_dev_malloc bookkeeping
   mf::DevBuf slab_ws;                   // materialised-slab workspace
   std::unique_ptr<mf::FftCache> fft;
+  // Page-locked output blocks of the batched form (mf_batch_run_each),
+  // kept between calls.
+  std::unique_ptr<mf::PinnedPool> out_pool;
   std::map<std::string, std::unique_ptr<mf::DevBuf>> tables;
   std::map<std::string, int> tune;      // launch-shape tunables
   std::atomic<unsigned long long> tune_gen{0};  // changes to 'tune'
@@ -126,6 +130,6 @@ struct mf_ctx {
   // are not repeated per launch.
   std::map<const void*, size_t> dyn_lds;
   std::map<std::tuple<const void*, int, size_t>, int> occ;
-  std::mutex mu, table_mu, tune_mu, launch_mu;
+  std::mutex mu, table_mu, tune_mu, launch_mu, err_mu;
   ~mf_ctx();
 };

@@ -25,6 +25,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <new>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -32,86 +33,46 @@
 
 #include "context.h"
 #include "markerfree/mf_api.h"
+#include "pinned.h"
 #include "series.h"
 
 namespace {
 
 using Clock = std::chrono::steady_clock;
+using mf::PinnedPool;
 
 double seconds_since(Clock::time_point t0) {
   return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
-// Free page-locked blocks kept for reuse, at most 'cap' of them.  The first
-// host write to newly page-locked memory costs about 0.3 ms per MB (page
-// population), several times a plain copy, so a request's stack copy and
-// its aligned stack go into recycled blocks when one of a fitting size is
-// free (no more than twice the size asked for).  The server sets the cap to
-// the number of blocks its admitted and queued requests can hold at once
-// (two per running request, one per queued request): the pool keeps no more
-// blocks than the requests themselves can hold.
-class PinnedPool {
- public:
-  explicit PinnedPool(size_t cap) : cap_(cap) {}
-  PinnedPool(const PinnedPool&) = delete;
-  PinnedPool& operator=(const PinnedPool&) = delete;
-  ~PinnedPool() {
-    for (auto& b : free_) mf_host_free(b.second);
-  }
-  // A block of at least n floats (its size in *got), or nullptr.
-  float* take(size_t n, size_t* got) {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      auto it = free_.lower_bound(n);
-      if (it != free_.end() && it->first <= 2 * n) {
-        float* p = it->second;
-        *got = it->first;
-        free_.erase(it);
-        return p;
-      }
-    }
-    void* q = nullptr;
-    if (mf_host_alloc(n * sizeof(float), &q) != MF_OK) return nullptr;
-    *got = n;
-    return (float*)q;
-  }
-  void give(float* p, size_t n) {
-    {
-      std::lock_guard<std::mutex> lk(mu_);
-      if (free_.size() < cap_) {
-        free_.emplace(n, p);
-        return;
-      }
-    }
-    mf_host_free(p);
-  }
-
- private:
-  std::mutex mu_;
-  std::multimap<size_t, float*> free_;  // size in floats -> block
-  size_t cap_;
-};
-
-// Page-locked float buffer from the pool: the request's stack copy and its
-// aligned stack move to and from the device by DMA at the PCIe rate.
+// Host float buffer, page-locked from the pool where the system grants it
+// (the request's stack copy and its aligned stack then move to and from the
+// device by DMA at the PCIe rate), else ordinary memory (staged copies).
 class PinnedFloats {
  public:
   PinnedFloats() = default;
   PinnedFloats(const PinnedFloats&) = delete;
   PinnedFloats& operator=(const PinnedFloats&) = delete;
   ~PinnedFloats() { clear(); }
-  // Returns false when page-locked memory cannot be had.
+  // Returns false only when no host memory at all can be had.
   bool resize(PinnedPool* pool, size_t n) {
     clear();
     if (!n) return true;
     p_ = pool->take(n, &cap_);
-    if (!p_) return false;
-    pool_ = pool;
+    if (p_) {
+      pool_ = pool;
+    } else {
+      heap_.reset(new (std::nothrow) float[n]);
+      if (!heap_) return false;
+      p_ = heap_.get();
+      cap_ = n;
+    }
     n_ = n;
     return true;
   }
   void clear() {
-    if (p_) pool_->give(p_, cap_);
+    if (p_ && pool_) pool_->give(p_, cap_);
+    heap_.reset();
     p_ = nullptr;
     pool_ = nullptr;
     n_ = cap_ = 0;
@@ -122,6 +83,7 @@ class PinnedFloats {
 
  private:
   PinnedPool* pool_ = nullptr;
+  std::unique_ptr<float[]> heap_;
   float* p_ = nullptr;
   size_t n_ = 0, cap_ = 0;
 };
@@ -130,6 +92,7 @@ struct Request {
   long long id = 0;
   int nx = 0, ny = 0, nz = 0;
   PinnedFloats stack;               // host copy (file order)
+  const float* borrowed = nullptr;  // the caller's stack (mf_submit_borrowed)
   std::vector<float> tilts;
   mf_options opt{};
   size_t bytes = 0;                 // admission charge
@@ -172,8 +135,9 @@ namespace {
 // that sees MF_REQ_DONE also sees the results and t_done.
 bool process(mf_ctx* ctx, PinnedPool* pool, Request& r, std::string* err) {
   mf_series* s = nullptr;
-  if (mf_series_create(ctx, r.nx, r.ny, r.nz, r.stack.data(), r.tilts.data(),
-                       &s) != MF_OK) {
+  const float* raw = r.borrowed ? r.borrowed : r.stack.data();
+  if (mf_series_create(ctx, r.nx, r.ny, r.nz, raw, r.tilts.data(), &s) !=
+      MF_OK) {
     *err = mf_last_error(ctx);
     return false;
   }
@@ -187,7 +151,7 @@ bool process(mf_ctx* ctx, PinnedPool* pool, Request& r, std::string* err) {
     return fail();
   if (!r.aligned.resize(pool, (size_t)r.bx * r.by * r.nz)) {
     mf_series_destroy(s);
-    *err = "no page-locked host memory for the aligned stack";
+    *err = "no host memory for the aligned stack";
     return false;
   }
   if (mf_correct(s, r.opt.out_bin, r.aligned.data(), &r.bx, &r.by) != MF_OK)
@@ -243,6 +207,7 @@ void worker_loop(mf_server* sv) {
       sv->completed++;
       r->t_done = seconds_since(sv->t0);
       r->stack.clear();
+      r->borrowed = nullptr;  // the caller may reuse its stack now
     }
     sv->cv_work.notify_all();
     sv->cv_done.notify_all();
@@ -270,35 +235,80 @@ mf_status guarded(std::string* err, F&& f) {
 
 extern "C" {
 
-mf_status mf_batch_run(mf_series** series, int n, const mf_options* o,
-                       int concurrency) {
-  if (!series || n < 0 || !o) return MF_EINVAL;
+mf_status mf_batch_run_each(mf_series** series, int n, const mf_options* opts,
+                            int concurrency, int correct, mf_batch_done done,
+                            void* user) {
+  if (!series || n < 0 || !opts) return MF_EINVAL;
   for (int i = 0; i < n; i++)
-    if (!series[i]) return MF_EINVAL;
+    if (!series[i] || opts[i].out_bin < 1) return MF_EINVAL;
+  mf_ctx* ctx = n > 0 ? series[0]->ctx : nullptr;
+  for (int i = 0; i < n; i++)
+    if (series[i]->ctx != ctx) return MF_EINVAL;
   const int workers = std::max(1, std::min(concurrency, n));
+  // Largest series first: the longest ones do not start last and leave the
+  // other workers idle at the end.
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    const mf_series *x = series[a], *y = series[b];
+    return (long long)x->nx * x->ny * x->nz > (long long)y->nx * y->ny * y->nz;
+  });
+  PinnedPool* pool = nullptr;
+  if (correct && n > 0) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (!ctx->out_pool) ctx->out_pool.reset(new PinnedPool(0));
+    ctx->out_pool->keep_at_least((size_t)workers);
+    pool = ctx->out_pool.get();
+  }
   std::vector<mf_status> st(n, MF_OK);
   std::mutex mu;
   int next = 0;
   auto loop = [&] {
+    PinnedFloats out;       // this worker's output block
     for (;;) {
       int i;
       {
         std::lock_guard<std::mutex> g(mu);
-        if (next >= n) return;
-        i = next++;
+        if (next >= n) break;
+        i = order[next++];
       }
-      st[i] = mf_run(series[i], o);
+      mf_series* s = series[i];
+      mf_status r = mf_run(s, &opts[i]);
       // Only the stack and the parameters are needed from here on.
-      if (st[i] == MF_OK) st[i] = mf_series_trim(series[i]);
+      if (r == MF_OK) r = mf_series_trim(s);
+      int bx = 0, by = 0;
+      if (r == MF_OK && pool) {
+        r = mf_correct(s, opts[i].out_bin, nullptr, &bx, &by);
+        const size_t need = (size_t)bx * by * s->nz;
+        if (r == MF_OK && need > out.size() && !out.resize(pool, need)) {
+          s->last_error = "no host memory for the aligned stack";
+          r = MF_ENOMEM;
+        }
+        if (r == MF_OK)
+          r = mf_correct(s, opts[i].out_bin, out.data(), &bx, &by);
+        if (r == MF_OK) r = mf_series_trim(s);  // the correction's buffer
+      }
+      st[i] = r;
+      if (done)
+        done(i, r, r == MF_OK && pool ? out.data() : nullptr, bx, by, s->nz,
+             user);
     }
   };
-  std::vector<std::thread> pool;
-  for (int w = 1; w < workers; w++) pool.emplace_back(loop);
+  std::vector<std::thread> pool_threads;
+  for (int w = 1; w < workers; w++) pool_threads.emplace_back(loop);
   loop();
-  for (auto& t : pool) t.join();
+  for (auto& t : pool_threads) t.join();
   for (int i = 0; i < n; i++)
     if (st[i] != MF_OK) return st[i];
   return MF_OK;
+}
+
+mf_status mf_batch_run(mf_series** series, int n, const mf_options* o,
+                       int concurrency) {
+  if (!series || n < 0 || !o) return MF_EINVAL;
+  std::vector<mf_options> opts(std::max(n, 0), *o);
+  return mf_batch_run_each(series, n, opts.data(), concurrency, 0, nullptr,
+                           nullptr);
 }
 
 size_t mf_series_hbm_bytes(int nx, int ny, int nz, const mf_options* o) {
@@ -330,6 +340,7 @@ mf_status mf_server_create(mf_ctx* ctx, int workers, size_t hbm_budget,
     for (int w = 0; w < workers; w++) sv->workers.emplace_back(worker_loop, sv);
   });
   if (st != MF_OK) {
+    std::lock_guard<std::mutex> g(ctx->err_mu);
     ctx->last_error = err;
     delete sv;
     return st;
@@ -349,9 +360,11 @@ void mf_server_destroy(mf_server* sv) {
   delete sv;
 }
 
-mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
-                    const float* stack_zyx, const float* tilts,
-                    const mf_options* o, long long* id) {
+namespace {
+
+mf_status submit(mf_server* sv, int nx, int ny, int nz, const float* stack_zyx,
+                 const float* tilts, const mf_options* o, bool borrow,
+                 long long* id) {
   if (!sv || !stack_zyx || !tilts || !o || !id || nx < 8 || ny < 8 || nz < 3)
     return MF_EINVAL;
   auto r = std::make_shared<Request>();
@@ -364,12 +377,24 @@ mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
   {
     std::lock_guard<std::mutex> lk(sv->mu);
     if (sv->stopping) return MF_ESTATE;
-    if (r->bytes > sv->budget) return MF_ENOMEM;  // could never be admitted
+    if (r->bytes > sv->budget) {
+      std::lock_guard<std::mutex> g(sv->ctx->err_mu);
+      sv->ctx->last_error = "the series' device bytes exceed the server's budget";
+      return MF_ENOMEM;  // could never be admitted
+    }
     if ((int)sv->queue.size() >= sv->queue_cap) return MF_EBUSY;
   }
   const size_t n = (size_t)nx * ny * nz;
-  if (!r->stack.resize(sv->pool.get(), n)) return MF_ENOMEM;
-  std::memcpy(r->stack.data(), stack_zyx, n * sizeof(float));
+  if (borrow) {
+    r->borrowed = stack_zyx;
+  } else {
+    if (!r->stack.resize(sv->pool.get(), n)) {
+      std::lock_guard<std::mutex> g(sv->ctx->err_mu);
+      sv->ctx->last_error = "no host memory for the stack's copy";
+      return MF_ENOMEM;
+    }
+    std::memcpy(r->stack.data(), stack_zyx, n * sizeof(float));
+  }
   r->tilts.assign(tilts, tilts + nz);
   {
     std::lock_guard<std::mutex> lk(sv->mu);
@@ -381,6 +406,20 @@ mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
   }
   sv->cv_work.notify_all();
   return MF_OK;
+}
+
+}  // namespace
+
+mf_status mf_submit(mf_server* sv, int nx, int ny, int nz,
+                    const float* stack_zyx, const float* tilts,
+                    const mf_options* o, long long* id) {
+  return submit(sv, nx, ny, nz, stack_zyx, tilts, o, false, id);
+}
+
+mf_status mf_submit_borrowed(mf_server* sv, int nx, int ny, int nz,
+                             const float* stack_zyx, const float* tilts,
+                             const mf_options* o, long long* id) {
+  return submit(sv, nx, ny, nz, stack_zyx, tilts, o, true, id);
 }
 
 mf_status mf_server_clock(mf_server* sv, double* seconds) {

@@ -199,3 +199,76 @@ def test_gpu_trim_keeps_stack_and_parameters(ctx, mf):
     G.projalign_setup(0.0, OPTS["align_z"], OPTS["nproj"])
     assert np.isfinite(G.projalign_iter())
     G.close()
+
+
+@pytest.mark.gpu
+def test_gpu_batch_each_streams_corrected_stacks(ctx, mf):
+    """The streaming batched form: per-series options (thickness and output
+    binning differ), each series corrected into a worker's page-locked block
+    and handed to the callback; every stack and parameter set equals the
+    series run alone, one call per series, largest series started first."""
+    mix = _mix()
+    opts = [mf.default_options(align_z=64, nproj=10, out_bin=1),
+            mf.default_options(align_z=96, nproj=10, out_bin=2),
+            mf.default_options(align_z=64, nproj=10, out_bin=1)]
+    alone = []
+    for s, o in zip(mix, opts):
+        G = mf.Series(ctx, s.stack, s.tilts)
+        G.run(o)
+        alone.append((G.params(), G.correct(o.out_bin)))
+        G.close()
+    for conc in (1, 3):
+        series = [mf.Series(ctx, s.stack, s.tilts) for s in mix]
+        got, calls = {}, []
+
+        def on_done(i, st, aligned):
+            calls.append(i)
+            assert st == mf.MF_OK
+            got[i] = aligned.copy()
+
+        mf.batch_run_each(series, opts, concurrency=conc, correct=True,
+                          on_done=on_done)
+        assert sorted(calls) == [0, 1, 2]
+        if conc == 1:
+            sizes = [np.prod(mix[i].stack.shape) for i in calls]
+            assert sizes == sorted(sizes, reverse=True)
+        for i, G in enumerate(series):
+            _same((G.params(), got[i]), alone[i])
+            G.close()
+
+
+@pytest.mark.gpu
+def test_gpu_borrowed_submission_equals_alone(ctx, mf):
+    """mf_submit_borrowed: the server reads the caller's stack in place (one
+    page-locked, one ordinary) instead of copying it; results equal the
+    series run alone, and the caller's stacks are left unchanged."""
+    mix = _mix()
+    alone = [_alone(mf, ctx, s) for s in mix[:2]]
+    pin = mf.pinned_empty(mix[0].stack.shape)
+    pin[:] = mix[0].stack
+    plain = mix[1].stack.copy()
+    with mf.Server(ctx, workers=2) as sv:
+        ids = [sv.submit(pin, mix[0].tilts, borrow=True, **OPTS),
+               sv.submit(plain, mix[1].tilts, borrow=True, **OPTS)]
+        for k, rid in enumerate(ids):
+            assert sv.wait(rid) == mf.REQ_DONE
+            r = sv.fetch(rid)
+            _same(((r.shift_x, r.shift_y, r.rotate, r.offset), r.aligned),
+                  alone[k])
+            sv.release(rid)
+    np.testing.assert_array_equal(pin, mix[0].stack)
+    np.testing.assert_array_equal(plain, mix[1].stack)
+
+
+@pytest.mark.gpu
+def test_gpu_failed_allocations_do_not_poison_later_calls(ctx, mf):
+    """A device allocation larger than the device is refused with
+    MF_ENOMEM, and the HIP error it leaves on the thread is cleared: the
+    next series run on the same thread still succeeds with the same
+    results."""
+    s = _mix()[0]
+    ref = _alone(mf, ctx, s)
+    with pytest.raises(mf.MFError) as e:
+        ctx.empty((1 << 40,))              # 4 TiB of device memory
+    assert e.value.code == mf.MF_ENOMEM
+    _same(_alone(mf, ctx, s), ref)
