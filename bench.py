@@ -21,7 +21,9 @@ reprojection it writes, 4*(n+1)*bx*by.  That kernel is bound by VALU issue,
 not HBM: 'valu' reports its issue rate from the committed counter summary
 (SQ_INSTS_VALU over SQ_BUSY_CYCLES), and 'gvoxel_per_s' the backprojection
 it delivers.  --mode materialized times bp_slab, whose algorithmic bytes are
-the slab it writes plus the sinogram rows it reads, 4*(2bx*T*by) + 4*n*bx*by.
+the slab it writes plus the sinogram rows it reads, 4*(2bx*T*by) + 4*n*bx*by;
+--mode slab4 / slab2 time bp_slab_rows, the same slab written four (two)
+image rows per voxel.
 'traffic' is the FETCH_SIZE + WRITE_SIZE counter reading per launch from the
 committed summaries.
 
@@ -416,7 +418,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["materialized", "fused"],
+    ap.add_argument("--mode", choices=["materialized", "fused", "slab4",
+                                       "slab2"],
                     default="fused")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["plain", "batched", "served"],
@@ -450,7 +453,8 @@ def main():
 
     cfg = synth.PRESETS["single"]
     series = synth.generate(cfg)
-    mode = 0 if args.mode == "materialized" else 1
+    mode = {"materialized": 0, "fused": 1, "slab4": 6, "slab2": 7}[args.mode]
+    fused = mode == 1
     align_z, nproj = 300, 10
     ctx = mf.Context(0)
     n = args.warmup + args.steps
@@ -478,7 +482,7 @@ def main():
     for s in pool[:args.warmup]:
         step(s)
     ctx.synchronize()
-    kname = "bp_slab" if mode == 0 else "bprp_fused"
+    kname = {0: "bp_slab", 1: "bprp_fused"}.get(mode, "bp_slab_rows")
     # HIP events around every TIME_EVERY-th backprojection launch only (each
     # timed launch costs two stream markers, a few microseconds of idle
     # device; sampling keeps that out of the series time).
@@ -501,7 +505,7 @@ def main():
     tilts_final = s_last.tilts()
     bx = by = 1024
     sweep_bytes, sweep_vox, targets = pa_sweep_bytes(
-        mf, tilts_final, bx, by, align_z, nproj, fused=mode != 0)
+        mf, tilts_final, bx, by, align_z, nproj, fused=fused)
     sweeps = k_n / max(args.steps, 1) / max(targets, 1)
     avg_ms = k_ms / max(k_timed, 1)
     bytes_per_launch = sweep_bytes / targets
@@ -573,7 +577,7 @@ def main():
     prof = f"traffic_{kname}_series_1024x1024x61_T300.md"
     tr = profile_value(prof, "traffic_bytes_per_launch")
     out["roofline"]["traffic"] = int(tr) if tr is not None else None
-    if mode != 0:
+    if fused:
         cnt = "counters_bprp_free_series_1024x1024x61_T300.md"
         frac = profile_value(cnt, "valu_issue_frac")
         clock = profile_value(cnt, "clock_ghz")

@@ -26,7 +26,6 @@
 // counts use the runtime-n instantiation (N = 0).
 #include <algorithm>
 #include <map>
-#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -1080,26 +1079,27 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         const float vfx = (inb ? truncf(X) : 0.f) + ox;
         const float vfz = (inb ? truncf(Z) : 0.f) + oz;
         Slot vals[N];
-        unsigned okm = 0;  // bit i: projection i in range
+        float wsel[N];  // projection i's weight, 0 where out of range
 #pragma unroll
         for (int i = 0; i < N; i++) {
           const float rr = fmaf(vfx, p.c[i], fmaf(vfz, p.s[i], cx));
           // The same unsigned range test as k_bprp_tab.
           const bool ok = __float_as_uint(rr) <= iend_bits;
-          okm |= ok ? (1u << i) : 0u;
+          wsel[i] = ok ? wv[i] : 0.f;
           // Out of range: any slot of the row (the clamped one), read with
           // weight 0.
           const int ix = (int)__builtin_amdgcn_fmed3f(rr, 0.f, iend);
           const unsigned at = rowv[i] + ((unsigned)ix << kSlotShift<R>);
           vals[i] = lds_slot<R>(at);
         }
-        float acc[R], ws = 0.f;
+        // The weight sum in reference order from 0 (0 + w0 = w0: w0 >= +0).
+        float acc[R], ws = wsel[0];
 #pragma unroll
         for (int r = 0; r < R; r++) acc[r] = 0.f;
 #pragma unroll
         for (int i = 0; i < N; i++) {
-          const float wi = ((okm >> i) & 1u) ? wv[i] : 0.f;
-          ws += wi;
+          const float wi = wsel[i];
+          if (i > 0) ws += wi;
           float v4[R];
           FreeSlot<R>::split(vals[i], v4);
 #pragma unroll
