@@ -226,6 +226,8 @@ def bench_batched(mf, synth, args):
         if s.stack.shape[1:] not in seen:
             seen.add(s.stack.shape[1:])
             warm.append(i)
+    if args.passes == "plain":
+        warm = []   # the trace of the pass alone (first kernel = its start)
     W = [mf.Series(ctx, mix[i][0].stack, mix[i][0].tilts) for i in warm]
     mf.batch_run_each(W, [opts[i] for i in warm],
                       concurrency=args.concurrency, correct=True)
@@ -259,6 +261,14 @@ def bench_batched(mf, synth, args):
             g.close()
         return dt, params, sums, peak
 
+    if args.passes == "plain":
+        # The plain pass alone (for a device busy-time trace).
+        t_plain, _, _, peak = run(1, "plain")
+        print(json.dumps({"plain_pass_seconds": round(t_plain, 3),
+                          "series": n, "peak_hbm_gb": round(peak / 1e9, 1)}),
+              flush=True)
+        ctx.close()
+        return
     t_plain, p_plain, s_plain, _ = run(1, "plain")
     t_batch, p_batch, s_batch, peak = run(args.concurrency, "batched")
     same = s_plain == s_batch and all(
@@ -429,6 +439,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64,
                     help="batched form: series in the mixture")
     ap.add_argument("--concurrency", type=int, default=3)
+    ap.add_argument("--passes", choices=["both", "plain"], default="both",
+                    help="batched form: 'plain' runs the one-at-a-time pass "
+                         "only")
     ap.add_argument("--requests", type=int, default=200,
                     help="served form: requests per rate")
     ap.add_argument("--pool", type=int, default=12,
