@@ -62,6 +62,38 @@ def test_correct_random_fill_is_nearby_raw_pixel(orc):
             assert abs(sx_ - xr) <= 25 and abs(sy_ - yr) <= 25
 
 
+def test_random_fill_keeps_the_window_bug_q10(orc):
+    """mGRandom (CorrStack.cu:116-140) draws the x offset as next % 2601 - 25
+    (quirk Q10: the window is 51 x 51 but x uses the whole LCG range) and the
+    y offset as next / 51 - 25.  On an image wide enough to accept them, the
+    fill pixels of out-of-image positions therefore lie up to ~2575 columns
+    to the right of the mirrored source but never more than 25 rows away; a
+    corrected window (next % 51) would keep both within 25.  The fill is a
+    raw pixel, deterministic, and different positions draw different
+    pixels."""
+    w, h = 3200, 64
+    raw = np.arange(w * h, dtype=np.float32).reshape(h, w)
+    out = orc.correct_proj(raw, 2400.0, 0.0, 0.0, True, True, 1, False)
+    again = orc.correct_proj(raw, 2400.0, 0.0, 0.0, True, True, 1, False)
+    np.testing.assert_array_equal(out, again)
+    dxs, dys = [], []
+    for i in range(1000, w):
+        for y in (3, 20, 40, 61):
+            v = int(out[y, i])
+            sy_, sx_ = divmod(v, w)
+            # Nearest source pixel (int)(q + 0.5) of q = i + 0.5 + 2400
+            # (quirk Q11), past the image's right edge, then mirrored.
+            x = i + 2401
+            xr = w - 1 - (x % w)
+            yr = y + 1 if y + 1 < h else h - 1 - ((y + 1) % h)
+            dxs.append(sx_ - xr)
+            dys.append(sy_ - yr)
+    dxs, dys = np.array(dxs), np.array(dys)
+    assert np.abs(dys).max() <= 25
+    assert dxs.min() >= -25 and dxs.max() > 1000
+    assert len(np.unique(dxs)) > 100
+
+
 def test_binning_sums(orc):
     raw = np.arange(20 * 16, dtype=np.float32).reshape(20, 16)
     out = orc.correct_proj(raw, -1.0, -1.0, 0.0, True, False, 2, False)
