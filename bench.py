@@ -64,13 +64,20 @@ def profile_value(name, key):
 
 
 def cpu_model():
+    """The host CPU's model name with its family/model numbers (virtualised
+    hosts may report only a brand name)."""
+    info = {}
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (t.strip() for t in line.split(":", 1))
+                info.setdefault(k, v)
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    if "cpu family" in info and "model" in info:
+        name += f" (family {info['cpu family']}, model {info['model']})"
+    return name
 
 
 def pa_sweep_bytes(mf, tilts, bx, by, tb, nproj, fused):
