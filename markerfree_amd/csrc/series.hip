@@ -136,10 +136,8 @@ void series_preprocess(mf_series* s) {
                mm.data());
   float stack_min = 1e30f;
   if (s->verbose) printf("Set positivity ...\n  num   residue     min       max\n");
-  std::vector<int> perm(nz);
   for (int a = 0; a < nz; a++) {
     const int i = rev ? nz - 1 - a : a;
-    perm[a] = i;
     if (stack_min > mm[2 * a]) stack_min = mm[2 * a];
     if (s->verbose)
       printf("%4d      %4d    %8.2f  %8.2f\n", i, nz - i, mm[2 * a], mm[2 * a + 1]);
@@ -147,7 +145,7 @@ void series_preprocess(mf_series* s) {
   if (s->verbose) printf("StackMin:%f\n", stack_min);
   float* stk = (float*)s->stack.ensure(s->ctx, plane * nz * sizeof(float));
   reorder_subtract(s->ctx, s->st, (const float*)s->raw.get(), stk, plane, nz,
-                   perm.data(), stack_min < 0 ? stack_min : 0.0f);
+                   rev, stack_min < 0 ? stack_min : 0.0f);
   if (s->verbose) printf("Positivity set.\n\nLinear mass normalization...\n");
   std::vector<float> means(nz);
   box_means(s->ctx, s->st, stk, s->nx, s->ny, nz, s->nx * 1 / 6, s->ny * 1 / 6,

@@ -28,9 +28,10 @@ struct CorrProj {
 // values (Preprocess.cu:7-85, 179-200); out [nz][2] on the host.
 void slice_minmax(mf_ctx* ctx, hipStream_t st, const float* d, size_t plane,
                   int nz, DevBuf& tmp, float* h_out);
-// stack[perm[a]] = raw[a] (- sub for valid pixels when sub != 0).
+// stack[a] = raw[a], or stack[nz-1-a] = raw[a] when 'rev' (descending
+// files), minus sub for valid pixels (>= -1e10) when sub != 0; one launch.
 void reorder_subtract(mf_ctx* ctx, hipStream_t st, const float* d_raw,
-                      float* d_stack, size_t plane, int nz, const int* h_perm,
+                      float* d_stack, size_t plane, int nz, bool rev,
                       float sub);
 // Mean of the central box of each slice, skipping values <= -1e20
 // (Preprocess.cu:216-237); FP64 sums, host out [nz].

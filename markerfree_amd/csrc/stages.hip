@@ -61,6 +61,21 @@ __global__ __launch_bounds__(kRedThreads) void k_minmax(const float* d,
   }
 }
 
+// Image blockIdx.y of the raw stack to position blockIdx.y (nz - 1 -
+// blockIdx.y when 'rev') of the stack, minus 'sub' where valid (0: a plain
+// copy): the reordering and the positivity shift in one pass.
+__global__ void k_reorder_subtract(const float* raw, float* stack,
+                                   size_t plane, int nz, int rev, float sub) {
+  const int a = blockIdx.y;
+  const float* src = raw + (size_t)a * plane;
+  float* dst = stack + (size_t)(rev ? nz - 1 - a : a) * plane;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < plane;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float v = src[i];
+    dst[i] = (sub != 0.0f && !(v < -1e10f)) ? v - sub : v;
+  }
+}
+
 __global__ void k_subtract(float* d, size_t n, float sub) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -408,18 +423,13 @@ void slice_minmax(mf_ctx* ctx, hipStream_t st, const float* d, size_t plane,
 }
 
 void reorder_subtract(mf_ctx* ctx, hipStream_t st, const float* d_raw,
-                      float* d_stack, size_t plane, int nz, const int* h_perm,
+                      float* d_stack, size_t plane, int nz, bool rev,
                       float sub) {
   (void)ctx;
-  for (int a = 0; a < nz; a++)
-    MF_HIP(hipMemcpyAsync(d_stack + (size_t)h_perm[a] * plane,
-                          d_raw + (size_t)a * plane, plane * sizeof(float),
-                          hipMemcpyDeviceToDevice, st));
-  if (sub != 0.0f) {
-    hipLaunchKernelGGL(k_subtract, dim3(4096), dim3(256), 0, st, d_stack,
-                       plane * nz, sub);
-    MF_HIP(hipGetLastError());
-  }
+  const int blocks = (int)std::min<size_t>(cdiv((long long)plane, 256 * 4), 1024);
+  hipLaunchKernelGGL(k_reorder_subtract, dim3(blocks, nz), dim3(256), 0, st,
+                     d_raw, d_stack, plane, nz, rev ? 1 : 0, sub);
+  MF_HIP(hipGetLastError());
 }
 
 void box_means(mf_ctx* ctx, hipStream_t st, const float* d, int nx, int ny,
