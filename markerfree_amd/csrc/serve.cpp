@@ -28,6 +28,7 @@
 #include <new>
 #include <mutex>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -238,10 +239,11 @@ extern "C" {
 mf_status mf_batch_run_each(mf_series** series, int n, const mf_options* opts,
                             int concurrency, int correct, mf_batch_done done,
                             void* user) {
+  if (n == 0) return MF_OK;
   if (!series || n < 0 || !opts) return MF_EINVAL;
   for (int i = 0; i < n; i++)
     if (!series[i] || opts[i].out_bin < 1) return MF_EINVAL;
-  mf_ctx* ctx = n > 0 ? series[0]->ctx : nullptr;
+  mf_ctx* ctx = series[0]->ctx;
   for (int i = 0; i < n; i++)
     if (series[i]->ctx != ctx) return MF_EINVAL;
   const int workers = std::max(1, std::min(concurrency, n));
@@ -254,7 +256,7 @@ mf_status mf_batch_run_each(mf_series** series, int n, const mf_options* opts,
     return (long long)x->nx * x->ny * x->nz > (long long)y->nx * y->ny * y->nz;
   });
   PinnedPool* pool = nullptr;
-  if (correct && n > 0) {
+  if (correct) {
     std::lock_guard<std::mutex> g(ctx->mu);
     if (!ctx->out_pool) ctx->out_pool.reset(new PinnedPool(0));
     ctx->out_pool->keep_at_least((size_t)workers);
@@ -295,7 +297,12 @@ mf_status mf_batch_run_each(mf_series** series, int n, const mf_options* opts,
     }
   };
   std::vector<std::thread> pool_threads;
-  for (int w = 1; w < workers; w++) pool_threads.emplace_back(loop);
+  try {
+    for (int w = 1; w < workers; w++) pool_threads.emplace_back(loop);
+  } catch (const std::system_error&) {
+    // Fewer host threads than asked for: the ones started and this thread
+    // share the series.
+  }
   loop();
   for (auto& t : pool_threads) t.join();
   for (int i = 0; i < n; i++)
@@ -305,6 +312,7 @@ mf_status mf_batch_run_each(mf_series** series, int n, const mf_options* opts,
 
 mf_status mf_batch_run(mf_series** series, int n, const mf_options* o,
                        int concurrency) {
+  if (n == 0) return MF_OK;
   if (!series || n < 0 || !o) return MF_EINVAL;
   std::vector<mf_options> opts(std::max(n, 0), *o);
   return mf_batch_run_each(series, n, opts.data(), concurrency, 0, nullptr,
@@ -340,9 +348,11 @@ mf_status mf_server_create(mf_ctx* ctx, int workers, size_t hbm_budget,
     for (int w = 0; w < workers; w++) sv->workers.emplace_back(worker_loop, sv);
   });
   if (st != MF_OK) {
-    std::lock_guard<std::mutex> g(ctx->err_mu);
-    ctx->last_error = err;
-    delete sv;
+    {
+      std::lock_guard<std::mutex> g(ctx->err_mu);
+      ctx->last_error = err;
+    }
+    if (sv) mf_server_destroy(sv);  // joins any worker already started
     return st;
   }
   *out = sv;

@@ -30,3 +30,10 @@ def test_version_and_errors_without_gpu(mf):
         assert e.code == mf.MF_EINVAL
     else:
         raise AssertionError("out-of-range target accepted")
+
+
+def test_empty_batches_need_no_gpu(mf):
+    """An empty batch is a no-op in both batched entry points (no device is
+    touched), and per-series options must match the series count."""
+    mf.batch_run([], align_z=64, nproj=10)
+    mf.batch_run_each([], [], concurrency=3, correct=True)
