@@ -38,6 +38,11 @@ namespace {
 
 constexpr int kBpThreads = 512;
 constexpr int kBpBlock = 256;  // voxels along x per wave block (4 per lane)
+// The R-row slab's block (2 voxels per lane): a block is classified as a
+// whole (every projection in range, none, or mixed), and about a fifth of
+// the 128-voxel blocks of a 1024-column slab are mixed against a third of
+// the 256-voxel ones.
+constexpr int kBpBlockR = 128;
 constexpr int kFuThreads = 512;  // rays per fused work unit
 constexpr int kMaxFixedN = 16;
 
@@ -92,6 +97,24 @@ __device__ int load_sino_rows(const BpParams& p, int n, int y, float* sv) {
 __device__ __forceinline__ float div_by_rcp(float a, float d, float r) {
   const float q = a * r;
   return fmaf(fmaf(-q, d, a), r, q);
+}
+
+// Slab voxels are written once and read back by the next kernel only after
+// more data than the last-level cache holds, so they are stored as streaming
+// (non-temporal) writes: no dirty slab lines remain in the caches to be
+// written back while the reprojection reads.
+__device__ __forceinline__ void stream_store(float* at, float v) {
+  __builtin_nontemporal_store(v, at);
+}
+__device__ __forceinline__ void stream_store(float4* at, const float4& v) {
+  typedef float Nat __attribute__((ext_vector_type(4)));
+  const Nat n = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(n, reinterpret_cast<Nat*>(at));
+}
+__device__ __forceinline__ void stream_store(float2* at, const float2& v) {
+  typedef float Nat __attribute__((ext_vector_type(2)));
+  const Nat n = {v.x, v.y};
+  __builtin_nontemporal_store(n, reinterpret_cast<Nat*>(at));
 }
 
 // A uniform value copied into a vector register, so that a select between it
@@ -280,11 +303,12 @@ __global__ __launch_bounds__(kBpThreads) void k_bp_slab(const BpParams p,
         }
         if (ix0 + kBpBlock <= vx) {
 #pragma unroll
-          for (int v = 0; v < V; v++) orow[ix0 + 64 * v] = res[v];
+          for (int v = 0; v < V; v++) stream_store(orow + ix0 + 64 * v, res[v]);
         } else {
 #pragma unroll
           for (int v = 0; v < V; v++)
-            if (ix0 + lane + 64 * v < vx) orow[ix0 + 64 * v] = res[v];
+            if (ix0 + lane + 64 * v < vx)
+              stream_store(orow + ix0 + 64 * v, res[v]);
         }
       }
     }
@@ -1234,7 +1258,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bp_slabr(const BpParams p,
   typedef typename FreeSlot<R>::T Slot;
   extern __shared__ float4 smem_slab[];
   Slot* sq = reinterpret_cast<Slot*>(smem_slab);  // [n][bx]
-  constexpr int V = kBpBlock / 64;
+  constexpr int V = kBpBlockR / 64;
   const int n = N ? N : p.n;
   const int bx = p.bx;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1310,8 +1334,8 @@ __global__ __launch_bounds__(kRowsThreads) void k_bp_slabr(const BpParams p,
       for (int i = 0; i < (N ? N : 1); i++)
         zs[i] = N ? fmaf(fz, p.s[i], cx) : 0.f;
       Slot* orow = slab + ((size_t)g * p.tb + iz) * vx + lane;
-      for (int ix0 = 0; ix0 < vx; ix0 += kBpBlock) {
-        const int ixl = min(ix0 + kBpBlock - 1, vx - 1);
+      for (int ix0 = 0; ix0 < vx; ix0 += kBpBlockR) {
+        const int ixl = min(ix0 + kBpBlockR - 1, vx - 1);
         const float rf = fmaf((float)ix0 + 0.5f - hvx, lp.c, lz);
         const float rl = fmaf((float)ixl + 0.5f - hvx, lp.c, lz);
         const unsigned long long m_in =
@@ -1375,7 +1399,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bp_slabr(const BpParams p,
 #pragma unroll
         for (int v = 0; v < V; v++)
           if (ix0 + lane + 64 * v < vx)
-            orow[ix0 + 64 * v] = FreeSlot<R>::make(res[v]);
+            stream_store(orow + ix0 + 64 * v, FreeSlot<R>::make(res[v]));
       }
     }
   }

@@ -277,6 +277,10 @@ def test_gpu_slab_rows_bitexact(ctx, orc, bx, by, tb, rows):
     neighbour (an invalid slab)."""
     stack, tilts = _series(bx, by, 41, 3.0, seed=3 * bx + by)
     stack[20, by // 2, bx // 3: bx // 3 + 9] = INVALID
+    # Skipped samples at the detector edges, which the slab blocks with
+    # projections partly in range reach.
+    stack[19, by - 1, :5] = INVALID
+    stack[21, 0, bx - 5:] = INVALID
     nz = 41
     mode = 6 if rows == 4 else 7
     groups = -(-by // rows)
@@ -370,7 +374,7 @@ def test_gpu_reproject_invalid_samples(ctx, orc):
     stack[4, 1, :] = INVALID
     skip = np.zeros(21, np.uint8)
     skip[15] = 1
-    for mode in (0, 1, 2, 3, 4, 5):
+    for mode in (0, 1, 2, 3, 4, 5, 6, 7):
         _check_target(ctx, orc, stack, tilts, skip, 15, 10, 30, mode)
 
 
@@ -380,7 +384,7 @@ def test_gpu_reproject_empty_range(ctx, orc):
     skip = np.ones(11, np.uint8)
     d_stack = ctx.array(stack)
     d_out = ctx.empty((3, 64))
-    for mode in (0, 1, 2, 3, 4, 5):
+    for mode in (0, 1, 2, 3, 4, 5, 6, 7):
         ctx.reproject(d_stack, 64, 3, 11, 20, tilts, skip, 5, 10, d_out,
                       mode=mode)
         assert np.all(d_out.download() == np.float32(INVALID))
