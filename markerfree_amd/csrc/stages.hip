@@ -296,10 +296,43 @@ __global__ __launch_bounds__(kRowThreads) void k_rweight(
   }
 }
 
+// Unbinned corrected pixel (x, y) of projection pr from the raw value v it
+// gathered (or did not: 'in' false), as corr_pixel computes it for a 1 x 1
+// bin: the random fill or the invalid value outside the image, and the bin
+// sum's 0 + v.
+__device__ __forceinline__ void corr_coords(const CorrArgs& a,
+                                            const CorrProj& pr, int x, int y,
+                                            int* rx, int* ry, bool* in) {
+  const float hx = (float)a.cx * 0.5f, hy = (float)a.ny * 0.5f;
+  const float ox = pr.shift_x + (float)a.nx * 0.5f;
+  const float oy = pr.shift_y + (float)a.ny * 0.5f;
+  const float px = (float)x + 0.5f - hx;
+  const float py = (float)y + 0.5f - hy;
+  float qx = px * pr.cos_a - py * pr.sin_a;
+  float qy = px * pr.sin_a + py * pr.cos_a;
+  qx += ox;
+  qy += oy;
+  *rx = (int)(qx + 0.5f);
+  *ry = (int)(qy + 0.5f);
+  *in = *rx >= 0 && *rx < a.nx && *ry >= 0 && *ry < a.ny;
+}
+__device__ __forceinline__ float corr_finish(const CorrArgs& a, const float* raw,
+                                             float v, int rx, int ry, bool in) {
+  if (!in) v = a.fill ? corr_random(rx, ry, a.nx, a.ny, raw) : kInvalid;
+  float sum = 0.f;
+  sum += v;
+  return v < -1e10f ? kInvalid : sum;
+}
+
 // Correction + binning (k_correct_bin) fused with the R-weighting of the
 // rows it produces (k_rweight): nl binned rows of projection z0 + blockIdx.y
 // per workgroup go from the raw image to the unpadded R-weighted output
-// without passing through HBM.  Same operations, same results.
+// without passing through HBM.  Same operations, same results.  The
+// workgroup's latency is its chain of memory round trips, so they are
+// shortened: unbinned images gather all of a thread's pixels (kCorrBatch
+// complex entries) before any is used, and the twiddles and the ramp filter
+// go to LDS in the same round trip as the first gathers.
+constexpr int kCorrBatch = 4;
 __global__ __launch_bounds__(kRowThreads) void k_correct_rweight(
     const CorrArgs ca, FftDevPlan P, int nl, const float* filt) {
   extern __shared__ float2 sm2[];
@@ -309,31 +342,97 @@ __global__ __launch_bounds__(kRowThreads) void k_correct_rweight(
   float2* b = a + nl * ls;
   float2* X = b + nl * ls;
   float2* tw = X + nl * xs;
+  float* fl = reinterpret_cast<float*>(tw + ls);  // ramp filter, N + 1 values
   const int r0 = blockIdx.x * nl;
   const int nv = min(nl, ca.by - r0);
   const int z = ca.z0 + (int)blockIdx.y * ca.zstep;
   const CorrProj pr = ca.proj[z];
   const size_t plane = (size_t)ca.nx * ca.ny;
   const float* raw = ca.raw + z * plane;
-  fft_load_tw(P, tw);
-  if (corr_transposed(ca, pr)) {
-    const float* rt = ca.rawT + z * plane;
-    for (LineIter it(N); it.l < nl; it.next()) {
-      int l = it.l, k = it.k;
-      float2 v = make_float2(0.f, 0.f);
-      if (l < nv)
-        v = make_float2(corr_pixel<true>(ca, pr, raw, rt, 2 * k, r0 + l),
-                        corr_pixel<true>(ca, pr, raw, rt, 2 * k + 1, r0 + l));
-      a[l * ls + k] = v;
+  const bool tr = corr_transposed(ca, pr);
+  const float* rt = tr ? ca.rawT + z * plane : nullptr;
+  if (ca.binx == 1 && ca.biny == 1) {
+    // Entries e = l * N + k (row l, complex column k: pixels 2k, 2k + 1),
+    // kCorrBatch per thread per round, every gather of a round in flight.
+    const int ne = nl * N;
+    const int rounds = cdiv(ne, kCorrBatch * (int)blockDim.x);
+    // The first kCorrBatch twiddles and filter values of each thread, loaded
+    // into registers ahead of the first gathers and written to LDS after
+    // them (the rest, for lines above 1024 points, in a loop).
+    const int bd = blockDim.x, ln = N + 1;
+    float2 twr[kCorrBatch];
+    float flr[kCorrBatch];
+#pragma unroll
+    for (int j = 0; j < kCorrBatch; j++) {
+      const int i = (int)threadIdx.x + j * bd;
+      twr[j] = i < ls ? P.tw[i] : make_float2(0.f, 0.f);
+      flr[j] = i < ln ? filt[i] : 0.f;
+    }
+    for (int rd = 0; rd < rounds; rd++) {
+      const int e0 = rd * kCorrBatch * (int)blockDim.x + (int)threadIdx.x;
+      int rx[kCorrBatch][2], ry[kCorrBatch][2];
+      bool in[kCorrBatch][2];
+      float v[kCorrBatch][2];
+#pragma unroll
+      for (int j = 0; j < kCorrBatch; j++) {
+        const int e = e0 + j * (int)blockDim.x;
+        const int l = e / N, k = e - l * N;
+        const bool use = e < ne && l < nv;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          corr_coords(ca, pr, 2 * k + h, r0 + l, &rx[j][h], &ry[j][h], &in[j][h]);
+          in[j][h] = in[j][h] && use;
+          v[j][h] = in[j][h] ? (tr ? raw_pix(rt, ca.ny, ry[j][h], rx[j][h])
+                                   : raw_pix(raw, ca.nx, rx[j][h], ry[j][h]))
+                             : 0.f;
+        }
+      }
+      if (rd == 0) {
+#pragma unroll
+        for (int j = 0; j < kCorrBatch; j++) {
+          const int i = (int)threadIdx.x + j * bd;
+          if (i < ls) tw[i] = twr[j];
+          if (i < ln) fl[i] = flr[j];
+        }
+        for (int i = (int)threadIdx.x + kCorrBatch * bd; i < ls; i += bd)
+          tw[i] = P.tw[i];
+        for (int i = (int)threadIdx.x + kCorrBatch * bd; i < ln; i += bd)
+          fl[i] = filt[i];
+      }
+#pragma unroll
+      for (int j = 0; j < kCorrBatch; j++) {
+        const int e = e0 + j * (int)blockDim.x;
+        if (e >= ne) continue;
+        const int l = e / N, k = e - l * N;
+        float2 o = make_float2(0.f, 0.f);
+        if (l < nv)
+          o = make_float2(
+              corr_finish(ca, raw, v[j][0], rx[j][0], ry[j][0], in[j][0]),
+              corr_finish(ca, raw, v[j][1], rx[j][1], ry[j][1], in[j][1]));
+        a[l * ls + k] = o;
+      }
     }
   } else {
-    for (LineIter it(N); it.l < nl; it.next()) {
-      int l = it.l, k = it.k;
-      float2 v = make_float2(0.f, 0.f);
-      if (l < nv)
-        v = make_float2(corr_pixel<false>(ca, pr, raw, nullptr, 2 * k, r0 + l),
-                        corr_pixel<false>(ca, pr, raw, nullptr, 2 * k + 1, r0 + l));
-      a[l * ls + k] = v;
+    fft_load_tw(P, tw);
+    for (int k = threadIdx.x; k <= N; k += blockDim.x) fl[k] = filt[k];
+    if (tr) {
+      for (LineIter it(N); it.l < nl; it.next()) {
+        int l = it.l, k = it.k;
+        float2 v = make_float2(0.f, 0.f);
+        if (l < nv)
+          v = make_float2(corr_pixel<true>(ca, pr, raw, rt, 2 * k, r0 + l),
+                          corr_pixel<true>(ca, pr, raw, rt, 2 * k + 1, r0 + l));
+        a[l * ls + k] = v;
+      }
+    } else {
+      for (LineIter it(N); it.l < nl; it.next()) {
+        int l = it.l, k = it.k;
+        float2 v = make_float2(0.f, 0.f);
+        if (l < nv)
+          v = make_float2(corr_pixel<false>(ca, pr, raw, nullptr, 2 * k, r0 + l),
+                          corr_pixel<false>(ca, pr, raw, nullptr, 2 * k + 1, r0 + l));
+        a[l * ls + k] = v;
+      }
     }
   }
   __syncthreads();
@@ -341,7 +440,7 @@ __global__ __launch_bounds__(kRowThreads) void k_correct_rweight(
   for (LineIter it(xs); it.l < nl; it.next()) {
     int l = it.l, k = it.k;
     r2c_unpack(Z + l * ls, N, P.half, 1.0f, k, [&](int kk, float2 v) {
-      const float f = filt[kk];
+      const float f = fl[kk];
       X[l * xs + kk] = make_float2((v.x / fn) * f, (v.y / fn) * f);
     });
   }
@@ -538,13 +637,16 @@ void correct_rweight(mf_ctx* ctx, hipStream_t st, const float* d_raw,
   const std::string key = "corr_rw/" + std::to_string(g.bx) + "x" +
                           std::to_string(g.by) + "x" + std::to_string(count) + "/";
   const long long want = (long long)g.by * count / (8LL * std::max(ctx->num_cu, 1));
-  const int fit = std::max(1, (int)((48 * 1024 - (size_t)ls * sizeof(float2)) / per));
+  const int fit = std::max(
+      1, (int)((48 * 1024 - (size_t)ls * sizeof(float2) -
+                (size_t)(P.n + 1) * sizeof(float)) / per));
   int nl = (int)std::max(1LL, std::min<long long>({want, (long long)fit, 16LL}));
   nl = std::max(1, std::min(tune_get(ctx, key + "lines", nl), fit));
   int threads = tune_get(ctx, key + "threads", kRowThreads);
   MF_REQUIRE(threads >= 64 && threads % 64 == 0 && threads <= kRowThreads,
              "workgroup size must be a multiple of 64 up to 256");
-  const size_t lds = nl * per + (size_t)ls * sizeof(float2);
+  const size_t lds = nl * per + (size_t)ls * sizeof(float2) +
+                     (size_t)(P.n + 1) * sizeof(float);
   MF_REQUIRE(lds <= 160 * 1024, "R-weighting row exceeds LDS");
   ensure_dyn_lds(ctx, (const void*)k_correct_rweight, lds);
   const float* filt = rweight_filter(ctx, g.bx);
