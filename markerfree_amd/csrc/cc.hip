@@ -231,6 +231,28 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_rows_fwd(
   const CcImg im = cc_img<TAB>(imgs, blockIdx.y);
   const int r0 = blockIdx.x * nl;
   const int nv = min(nl, h - r0);
+  // The first kRowPre entries of the thread (pixel and mask pairs) are
+  // loaded before the tables and the moments, so their memory round trip
+  // overlaps those.
+  constexpr int kRowPre = 8;
+  LineIter it(N);
+  float pv[kRowPre][2], pm[kRowPre][2];
+  int pat[kRowPre];
+#pragma unroll
+  for (int j = 0; j < kRowPre; j++) {
+    const int l = it.l, k = it.k;
+    pat[j] = l < nl ? l * ls + k : -1;
+    pv[j][0] = pv[j][1] = -1e30f;
+    pm[j][0] = pm[j][1] = 0.f;
+    if (l < nv) {
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        pv[j][q] = im.src[(size_t)(r0 + l) * im.pitch + 2 * k + q];
+        pm[j][q] = mask[(size_t)(r0 + l) * w + 2 * k + q];
+      }
+    }
+    it.next();
+  }
   fft_load_tw(P, tw);
   if (threadIdx.x < 64) {
     float mean, sd;
@@ -242,9 +264,7 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_rows_fwd(
   }
   __syncthreads();
   const float mean = sh_ms[0], sd = sh_ms[1];
-  auto norm = [&](int y, int x) {
-    const float v = im.src[(size_t)y * im.pitch + x];
-    const float mk = mask[(size_t)y * w + x];
+  auto normv = [&](float v, float mk) {
     float o = 0.0f;
     if (v >= -1e10f) {
       const float nv_ = (v - mean) / sd;
@@ -252,8 +272,16 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_rows_fwd(
     }
     return o;
   };
-  // Four of a thread's entries per batch, their loads issued together.
-  for (LineIter it(N); it.l < nl;) {
+  auto norm = [&](int y, int x) {
+    return normv(im.src[(size_t)y * im.pitch + x], mask[(size_t)y * w + x]);
+  };
+#pragma unroll
+  for (int j = 0; j < kRowPre; j++)
+    if (pat[j] >= 0)
+      a[pat[j]] = make_float2(normv(pv[j][0], pm[j][0]), normv(pv[j][1], pm[j][1]));
+  // The rest: four of a thread's entries per batch, their loads issued
+  // together.
+  for (; it.l < nl;) {
     float2 v[4];
     int at[4];
 #pragma unroll
@@ -366,10 +394,20 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_cols_inpl(
   const CcImg ia = cc_img<TAB>(imgs, 2 * pair), ib = cc_img<TAB>(imgs, 2 * pair + 1);
   const float2* A = reinterpret_cast<const float2*>(ia.dst);
   float2* B = reinterpret_cast<float2*>(ib.dst);
-  fft_load_tw(P, tw);
-  for (int r = threadIdx.x; r < 2 * N; r += kCcThreads) {
-    const int img = r >= N, row = r - img * N;
-    x[r] = (img ? B : A)[(size_t)row * nc + c0];
+  // The column entries of both images (2N / kCcThreads per thread), all
+  // loads issued before the first LDS write, then the twiddles.
+  {
+    constexpr int kPer = 2 * N / kCcThreads;
+    float2 v[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int r = (int)threadIdx.x + j * kCcThreads;
+      const float2* src = j < kPer / 2 ? A : B;  // r < N: image A
+      v[j] = src[(size_t)(r - (j < kPer / 2 ? 0 : N)) * nc + c0];
+    }
+    fft_load_tw(P, tw);
+#pragma unroll
+    for (int j = 0; j < kPer; j++) x[threadIdx.x + j * kCcThreads] = v[j];
   }
   __syncthreads();
   fft_pow2_inplace<N, false, 2, kCcThreads, N>(x, tw);
@@ -523,9 +561,29 @@ __global__ __launch_bounds__(kCcThreads) void k_cc_rows_inv(
   const int nv = min(nl, h - r0);
   const int bs = ls + 1, cp = w / 2 + 1;
   float2* map = reinterpret_cast<float2*>(im.dst);
-  fft_load_tw(P, tw);
-  // Four of a thread's entries per batch, their loads issued together.
-  for (LineIter it(N + 1); it.l < nl;) {
+  // The thread's first kRowPre spectral entries are loaded before the
+  // twiddles, so the two round trips overlap.
+  constexpr int kRowPre = 8;
+  LineIter it(N + 1);
+  {
+    float2 v[kRowPre];
+    int at[kRowPre];
+#pragma unroll
+    for (int j = 0; j < kRowPre; j++) {
+      const int l = it.l, k = it.k;
+      at[j] = l < nl ? l * bs + k : -1;
+      v[j] = make_float2(0.f, 0.f);
+      if (l < nv) v[j] = map[(size_t)(r0 + l) * cp + k];
+      it.next();
+    }
+    fft_load_tw(P, tw);
+#pragma unroll
+    for (int j = 0; j < kRowPre; j++)
+      if (at[j] >= 0) b[at[j]] = v[j];
+  }
+  // The rest: four of a thread's entries per batch, their loads issued
+  // together.
+  for (; it.l < nl;) {
     float2 v[4];
     int at[4];
 #pragma unroll

@@ -415,9 +415,21 @@ __host__ __device__ __forceinline__ int fft_lds_len(const FftDevPlan& P) {
 
 // Copies the twiddle table of P (fft_lds_len(P) entries) into LDS; the
 // caller synchronises before the first transform.
+// Twiddles to LDS, four per thread per round with the round's loads all in
+// flight before its LDS writes (one memory round trip for up to 4 blockDim.x
+// entries; a load-then-write loop waits once per entry).
 __device__ __forceinline__ void fft_load_tw(const FftDevPlan& P, float2* tw) {
   const int L = fft_lds_len(P);
-  for (int i = threadIdx.x; i < L; i += blockDim.x) tw[i] = P.tw[i];
+  const int bd = blockDim.x;
+  for (int i0 = threadIdx.x; i0 < L; i0 += 4 * bd) {
+    float2 t[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      t[j] = i0 + j * bd < L ? P.tw[i0 + j * bd] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (i0 + j * bd < L) tw[i0 + j * bd] = t[j];
+  }
 }
 
 // Complex transform of nl lines (length P.n, stride fft_lds_len(P)) held in
