@@ -487,7 +487,11 @@ def main():
 
     def step(s):
         s.run(opts)
-        return s.correct(1, out=aligned)
+        out = s.correct(1, out=aligned)
+        # The series' work buffers go back to the context (its block cache),
+        # as in the batched and served forms, for the next series to reuse.
+        s.trim()
+        return out
 
     # Counter calibration: one device-to-device copy of a known size (1 GiB,
     # above the 256 MB last-level cache) outside the timed region; under

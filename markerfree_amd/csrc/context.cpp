@@ -86,32 +86,82 @@ void KernelTimer::query(const std::string& name, double* total_ms,
   *n = it == done_.end() ? 0 : it->second.second;
 }
 
+namespace {
+// Block sizes: 2 MiB multiples from 1 MiB up, so that buffers of nearby
+// sizes share cached blocks; 4 KiB multiples below.
+size_t block_bytes(size_t b) {
+  const size_t g = b >= ((size_t)1 << 20) ? ((size_t)2 << 20) : 4096;
+  return (b + g - 1) / g * g;
+}
+}  // namespace
+
 void* dev_alloc(mf_ctx* ctx, size_t bytes) {
   if (bytes == 0) bytes = 256;
+  const size_t rb = block_bytes(bytes);
   {
     std::lock_guard<std::mutex> g(ctx->mu);
     if (ctx->budget && ctx->in_use + bytes > ctx->budget)
       throw Error(MF_ENOMEM, "HBM budget of the context exceeded");
     ctx->in_use += bytes;
     ctx->peak = std::max(ctx->peak, ctx->in_use);
+    // A cached block of at least rb and at most 5/4 rb bytes.
+    auto it = ctx->free_blocks.lower_bound(rb);
+    if (it != ctx->free_blocks.end() && it->first <= rb + rb / 4) {
+      void* p = it->second;
+      ctx->block_size[p] = it->first;
+      ctx->cached -= it->first;
+      ctx->free_blocks.erase(it);
+      return p;
+    }
   }
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, bytes);
+  hipError_t e = hipMalloc(&p, rb);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    dev_flush(ctx);  // the cache's blocks back to the device, then retry
+    e = hipMalloc(&p, rb);
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();  // thrown below; later checks must not see it
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->in_use -= bytes;
     throw_hip(e, "hipMalloc", __FILE__, __LINE__);
   }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->block_size[p] = rb;
   return p;
 }
 
-void dev_free(mf_ctx* ctx, void* p, size_t bytes) {
+void dev_free(mf_ctx* ctx, void* p, size_t bytes, bool cache) {
   if (!p) return;
-  (void)hipFree(p);
   if (bytes == 0) bytes = 256;
-  std::lock_guard<std::mutex> g(ctx->mu);
-  ctx->in_use -= bytes;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->in_use -= bytes;
+    size_t sz = 0;
+    auto it = ctx->block_size.find(p);
+    if (it != ctx->block_size.end()) {
+      sz = it->second;
+      ctx->block_size.erase(it);
+    }
+    if (cache && sz >= ((size_t)1 << 20) &&
+        ctx->cached + sz <= ctx->cache_cap) {
+      ctx->free_blocks.emplace(sz, p);
+      ctx->cached += sz;
+      return;
+    }
+  }
+  (void)hipFree(p);
+}
+
+void dev_flush(mf_ctx* ctx) {
+  std::multimap<size_t, void*> blocks;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    blocks.swap(ctx->free_blocks);
+    ctx->cached = 0;
+  }
+  for (auto& kv : blocks) (void)hipFree(kv.second);
 }
 
 const void* ctx_table(mf_ctx* ctx, const std::string& key,
@@ -166,10 +216,10 @@ void* DevBuf::ensure(mf_ctx* ctx, size_t bytes) {
   return p_;
 }
 
-void DevBuf::release() {
+void DevBuf::release(bool cache) {
   if (p_) {
     if (ctx_) (void)hipStreamSynchronize(ctx_->stream);
-    dev_free(ctx_, p_, n_);
+    dev_free(ctx_, p_, n_, cache);
   }
   p_ = nullptr;
   n_ = 0;
@@ -185,6 +235,7 @@ mf_ctx::~mf_ctx() {
   out_pool.reset();
   for (auto& kv : user_allocs) (void)hipFree(kv.first);
   user_allocs.clear();
+  mf::dev_flush(this);
   timer.reset();
   if (stream) (void)hipStreamDestroy(stream);
 }

@@ -71,7 +71,9 @@ class DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf();
   void* ensure(mf_ctx* ctx, size_t bytes);
-  void release();
+  // 'cache': the caller has synchronised every stream that used the buffer,
+  // so its block may go to the context's cache for reuse by any stream.
+  void release(bool cache = false);
   void* get() const { return p_; }
   size_t size() const { return n_; }
 
@@ -81,8 +83,15 @@ class DevBuf {
   size_t n_ = 0;
 };
 
+// Device memory through the context: budget and peak accounting in
+// requested bytes, blocks in rounded sizes.  dev_free with 'cache' keeps a
+// block of 1 MiB or more (up to the context's cache_cap bytes in all) for a
+// later dev_alloc of nearly its size, instead of returning it; the caller
+// guarantees that no pending device work still uses it.  dev_flush returns
+// every cached block (dev_alloc does so once when hipMalloc runs out).
 void* dev_alloc(mf_ctx* ctx, size_t bytes);
-void dev_free(mf_ctx* ctx, void* p, size_t bytes);
+void dev_free(mf_ctx* ctx, void* p, size_t bytes, bool cache = false);
+void dev_flush(mf_ctx* ctx);
 
 // Device copy of a constant host table, uploaded once per key and owned by
 // the context (filters, tapers, trigonometric tables).  'build' runs only the
@@ -112,6 +121,12 @@ struct mf_ctx {
   size_t budget = 0;      // bytes the context may hold (0: unlimited)
   size_t in_use = 0;      // bytes currently allocated through the context
   size_t peak = 0;        // largest in_use since creation or the last reset
+  // Blocks of released series buffers kept for reuse (mf::dev_free), by
+  // size; 'cached' bytes in all, at most cache_cap.
+  std::multimap<size_t, void*> free_blocks;
+  std::map<void*, size_t> block_size;  // live blocks' rounded sizes
+  size_t cached = 0;
+  size_t cache_cap = (size_t)16 << 30;
   std::string last_error;
   mf::KernelTimer timer;
   std::map<void*, size_t> user_allocs;  // mf_dev_malloc bookkeeping

@@ -24,7 +24,7 @@ mf_series::~mf_series() {
                         &pa_reproj, &pa_work, &pa_mom, &pa_dev,
                         &pa_slab, &pa_tab, &axw.lines, &axw.reg, &axw.small,
                         &axw.tstack, &offw.buf})
-    b->release();
+    b->release(true);  // the series' stream is synchronised
   for (hipEvent_t e : up_ev)
     if (e) (void)hipEventDestroy(e);
   if (pa_graph) (void)hipGraphExecDestroy(pa_graph);
@@ -690,7 +690,7 @@ void series_trim(mf_series* s) {
                     &s->pa_dev, &s->pa_slab, &s->pa_tab, &s->axw.lines,
                     &s->axw.reg, &s->axw.small, &s->axw.tstack, &s->axw.negf,
                     &s->offw.buf})
-    b->release();
+    b->release(true);  // the series' stream is synchronised
   s->axw.t_of = nullptr;
   s->axw.neg_of = nullptr;
   s->tr_ready = false;
