@@ -436,8 +436,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", choices=["materialized", "fused", "slab4",
-                                       "slab2"],
-                    default="fused")
+                                       "slab2", "fused_exact"],
+                    default="fused",
+                    help="fused: the default fused form (mode 8); "
+                         "fused_exact: the fused form bit-identical to the "
+                         "reference's evaluation order (mode 1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["plain", "batched", "served"],
                     default="plain",
@@ -473,8 +476,9 @@ def main():
 
     cfg = synth.PRESETS["single"]
     series = synth.generate(cfg)
-    mode = {"materialized": 0, "fused": 1, "slab4": 6, "slab2": 7}[args.mode]
-    fused = mode == 1
+    mode = {"materialized": 0, "fused": 8, "slab4": 6, "slab2": 7,
+            "fused_exact": 1}[args.mode]
+    fused = mode in (1, 8)
     align_z, nproj = 300, 10
     ctx = mf.Context(0)
     n = args.warmup + args.steps
@@ -506,7 +510,8 @@ def main():
     for s in pool[:args.warmup]:
         step(s)
     ctx.synchronize()
-    kname = {0: "bp_slab", 1: "bprp_fused"}.get(mode, "bp_slab_rows")
+    kname = {0: "bp_slab", 1: "bprp_fused", 8: "bprp_fused"}.get(
+        mode, "bp_slab_rows")
     # HIP events around every TIME_EVERY-th backprojection launch only (each
     # timed launch costs two stream markers, a few microseconds of idle
     # device; sampling keeps that out of the series time).

@@ -624,7 +624,7 @@ class Series:
     def apply_offset(self):
         self._check(_lib.mf_apply_offset(self.h))
 
-    def projalign_setup(self, zshift, align_z, nproj, reproj_mode=1):
+    def projalign_setup(self, zshift, align_z, nproj, reproj_mode=8):
         self._check(_lib.mf_projalign_setup(self.h, zshift, align_z, nproj,
                                             reproj_mode))
 

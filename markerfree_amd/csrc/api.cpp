@@ -340,7 +340,7 @@ mf_status mf_reproject(mf_ctx* ctx, const float* d_stack, int bx, int by,
   if (!ctx) return MF_EINVAL;
   return guard(ctx, [&] {
     MF_REQUIRE(d_stack && tilts && skip && d_reproj, "null argument");
-    MF_REQUIRE(mode >= 0 && mode <= 7, "mode must be 0..7");
+    MF_REQUIRE(mode >= 0 && mode <= 8, "mode must be 0..8");
     DeviceScope d(ctx);
     reproject_target(ctx, ctx->stream, d_stack, bx, by, nz, tb, tilts, skip,
                      target, nproj, mode, d_reproj, d_slab);
@@ -590,7 +590,7 @@ void mf_options_default(mf_options* o) {
   o->out_bin = 1;
   o->nproj = 10;
   o->verbose = 0;
-  o->reproj_mode = 1;
+  o->reproj_mode = 8;
   o->pa_pairs = 1;
   o->graphs = 0;
 }
@@ -791,7 +791,7 @@ mf_status mf_apply_offset(mf_series* s) {
 mf_status mf_projalign_setup(mf_series* s, float zshift, int align_z,
                              int nproj, int reproj_mode) {
   return sguard(s, [&] {
-    MF_REQUIRE(reproj_mode >= 0 && reproj_mode <= 7, "reproj_mode is 0..7");
+    MF_REQUIRE(reproj_mode >= 0 && reproj_mode <= 8, "reproj_mode is 0..8");
     s->reproj_mode = reproj_mode;
     series_projalign_setup(s, zshift, align_z, nproj);
   });
@@ -816,8 +816,8 @@ mf_status mf_run(mf_series* s, const mf_options* o) {
   return sguard(s, [&] {
     MF_REQUIRE(o, "null options");
     MF_REQUIRE(o->nproj > 0, "nProj must be > 0");
-    MF_REQUIRE(o->reproj_mode >= 0 && o->reproj_mode <= 7,
-               "reproj_mode is 0..7");
+    MF_REQUIRE(o->reproj_mode >= 0 && o->reproj_mode <= 8,
+               "reproj_mode is 0..8");
     series_run(s, o);
   });
 }

@@ -1025,7 +1025,13 @@ __device__ void load_sino_slots(const BpParams& p, int n, const int* y,
   __syncthreads();
 }
 
-template <int N, int R>
+// DIV1 (mode 8): in the test-free interval, where every sample's voxel has
+// the same weight sum W, the ray adds the samples' numerators and divides
+// their sum by W once, instead of dividing each sample (three instructions
+// per row and sample fewer).  The result differs from the reference's
+// per-voxel division only in rounding, well inside 1e-5 relative of the FP64
+// evaluation; skipped samples still send the ray to the exact per-row form.
+template <int N, int R, bool DIV1>
 __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
                                                             const RpParams q,
                                                             int per) {
@@ -1187,7 +1193,11 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
 #pragma unroll
     for (int r = 0; r < R; r++) sum[r] = 0.0f;
     tested(t0, a);
+    bool seen = false;
     if (has_fast) {
+      float sf[R];  // DIV1: the interval's numerators
+#pragma unroll
+      for (int r = 0; r < R; r++) sf[r] = 0.f;
       for (int t = ta; t < tb; t++) {
         const float fz = (float)t + z0;
         const float vfx = truncf(fmaf(-fz, q.s, tx)) + ox;
@@ -1209,13 +1219,24 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
           for (int r = 0; r < R; r++) acc[r] = fmaf(v4[r], p.w[i], acc[r]);
         }
 #pragma unroll
-        for (int r = 0; r < R; r++)
-          sum[r] += div_by_rcp(acc[r], p.cnt_all, p.rcp_all);
+        for (int r = 0; r < R; r++) {
+          if (DIV1)
+            sf[r] += acc[r];
+          else
+            sum[r] += div_by_rcp(acc[r], p.cnt_all, p.rcp_all);
+        }
+      }
+      if (DIV1) {
+        // A skipped sample (-inf) left its numerator sum -inf or NaN.
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          seen |= !isfinite(sf[r]);
+          sum[r] += sf[r] / p.cnt_all;
+        }
       }
       nuse += tb - ta;
     }
     tested(b, t1);
-    bool seen = false;
 #pragma unroll
     for (int r = 0; r < R; r++) seen |= sum[r] != sum[r];
     if (seen) {
@@ -1483,12 +1504,14 @@ struct KernelTable<0> {
 template <int N>
 struct TabKernelTable {
   static void fill(FusedKernel* tk, FusedKernel* tk2, FusedKernel* f2,
-                   FusedKernel* f4) {
+                   FusedKernel* f4, FusedKernel* d2, FusedKernel* d4) {
     tk[N] = k_bprp_tab<N>;
     tk2[N] = k_bprp_rows<N, 2>;
-    f2[N] = k_bprp_free<N, 2>;
-    f4[N] = k_bprp_free<N, 4>;
-    TabKernelTable<N - 1>::fill(tk, tk2, f2, f4);
+    f2[N] = k_bprp_free<N, 2, false>;
+    f4[N] = k_bprp_free<N, 4, false>;
+    d2[N] = k_bprp_free<N, 2, true>;
+    d4[N] = k_bprp_free<N, 4, true>;
+    TabKernelTable<N - 1>::fill(tk, tk2, f2, f4, d2, d4);
   }
 };
 typedef void (*SlabKernel)(BpParams, int);
@@ -1509,8 +1532,8 @@ struct SlabKernelTable<-1> {
 template <>
 struct TabKernelTable<0> {
   static void fill(FusedKernel* tk, FusedKernel* tk2, FusedKernel* f2,
-                   FusedKernel* f4) {
-    tk[0] = tk2[0] = f2[0] = f4[0] = nullptr;
+                   FusedKernel* f4, FusedKernel* d2, FusedKernel* d4) {
+    tk[0] = tk2[0] = f2[0] = f4[0] = d2[0] = d4[0] = nullptr;
   }
 };
 
@@ -1520,6 +1543,9 @@ struct Kernels {
   FusedKernel rows2[kTabMaxN + 1];
   FusedKernel free2[kTabMaxN + 1];
   FusedKernel free4[kTabMaxN + 1];
+  // The same with one division per ray's test-free interval (mode 8).
+  FusedKernel div2[kTabMaxN + 1];
+  FusedKernel div4[kTabMaxN + 1];
   // Materialised R-row backprojection, n = 1..kTabMaxN fixed, 0 runtime.
   SlabKernel slab4[kTabMaxN + 1];
   SlabKernel slab2[kTabMaxN + 1];
@@ -1528,7 +1554,7 @@ struct Kernels {
   std::map<const void*, size_t> static_lds;
   Kernels() {
     KernelTable<kMaxFixedN>::fill(fu);
-    TabKernelTable<kTabMaxN>::fill(tab, rows2, free2, free4);
+    TabKernelTable<kTabMaxN>::fill(tab, rows2, free2, free4, div2, div4);
     SlabKernelTable<kTabMaxN>::fill(slab4, slab2);
     auto note = [&](const void* k) {
       if (!k) return;
@@ -1537,7 +1563,7 @@ struct Kernels {
       static_lds[k] = a.sharedSizeBytes;
     };
     for (FusedKernel k : fu) note((const void*)k);
-    for (FusedKernel* t : {tab, rows2, free2, free4})
+    for (FusedKernel* t : {tab, rows2, free2, free4, div2, div4})
       for (int n = 0; n <= kTabMaxN; n++) note((const void*)t[n]);
     note((const void*)k_bp_slab);
     for (SlabKernel* t : {slab4, slab2})
@@ -1649,7 +1675,7 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   const size_t lds = bp_lds_bytes(n, bx);
   MF_REQUIRE(lds <= 160 * 1024, "sinogram rows exceed LDS");
   const int kid = n <= kMaxFixedN ? n : 0;
-  if (mode >= 1 && mode <= 5) {
+  if ((mode >= 1 && mode <= 5) || mode == 8) {
     if (n == 0) {
       hipLaunchKernelGGL(k_fill, dim3(cdiv((long long)bx * by, 256)),
                          dim3(256), 0, st, d_reproj, (size_t)bx * by,
@@ -1668,18 +1694,23 @@ void reproject_target(mf_ctx* ctx, hipStream_t st, const float* d_stack,
     // tables and two rows without.
     const bool tab = (mode == 1 || mode >= 3) && n <= kTabMaxN;
     const Kernels& ks = kernels();
+    // Mode 8: mode 1's selection with the one-division forms of the
+    // table-free kernels.
+    const bool m1 = mode == 1 || mode == 8;
+    const FusedKernel* f4s = mode == 8 ? ks.div4 : ks.free4;
+    const FusedKernel* f2s = mode == 8 ? ks.div2 : ks.free2;
     int rows = 0;
-    if (tab && mode == 1 && by >= 4 &&
-        lds_fits((const void*)ks.free4[n], free_lds_bytes(4, n, bx))) {
+    if (tab && m1 && by >= 4 &&
+        lds_fits((const void*)f4s[n], free_lds_bytes(4, n, bx))) {
       rows = 4;
-      k = ks.free4[n];
+      k = f4s[n];
       flds = free_lds_bytes(4, n, bx);
-    } else if (tab && (mode == 1 || mode == 5) && by > 1 &&
-               lds_fits((const void*)ks.free2[n], free_lds_bytes(2, n, bx))) {
+    } else if (tab && (m1 || mode == 5) && by > 1 &&
+               lds_fits((const void*)f2s[n], free_lds_bytes(2, n, bx))) {
       rows = 2;
-      k = ks.free2[n];
+      k = f2s[n];
       flds = free_lds_bytes(2, n, bx);
-    } else if (tab && (mode == 1 || mode == 4) && by > 1 &&
+    } else if (tab && (m1 || mode == 4) && by > 1 &&
                lds_fits((const void*)ks.rows2[n], rows_lds_bytes(n, bx, tb))) {
       rows = 2;
       k = ks.rows2[n];

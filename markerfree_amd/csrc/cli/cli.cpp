@@ -37,8 +37,9 @@ struct Options {
   std::string method;
   int iteration = 0;
   float gamma = 0;
-  int reproj_mode = 1;  // not a reference flag: MF_REPROJ_MODE=0 selects
-                        // the materialised slab
+  int reproj_mode = 8;  // not a reference flag: MF_REPROJ_MODE=0 selects
+                        // the materialised slab, MF_REPROJ_MODE=1 the fused
+                        // form bit-identical to the reference's evaluation
 };
 
 // UsageDual (opts.h:43-65).
@@ -401,6 +402,7 @@ int main(int argc, char** argv) {
   Options o;
   const char* rm = getenv("MF_REPROJ_MODE");
   if (rm && rm[0] == '0') o.reproj_mode = 0;
+  if (rm && rm[0] == '1') o.reproj_mode = 1;
   if (get_opts(argc, argv, &o) <= 0) {
     printf("***WRONG INPUT.\n");
     fflush(stdout);

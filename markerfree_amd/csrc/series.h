@@ -14,7 +14,7 @@ struct mf_series {
   hipStream_t st = nullptr;  // each series owns a stream (batched form)
   int nx = 0, ny = 0, nz = 0;
   int verbose = 0;
-  int reproj_mode = 1;       // 0 materialised slab, 1 fused
+  int reproj_mode = 8;       // 0 materialised slab, 1 and 8 fused
   bool pa_pairs = true;      // sweep steps may measure two targets at once
   hipEvent_t up_ev[2] = {nullptr, nullptr};  // streaming uploads (alternate)
   long long up_n = 0;                        // uploads recorded

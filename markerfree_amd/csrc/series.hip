@@ -470,7 +470,7 @@ float series_projalign_iter(mf_series* s) {
   float* work = (float*)s->pa_work.get();
   double* mom = (double*)s->pa_mom.get();
   float* slab = nullptr;
-  if (s->reproj_mode == 0 || s->reproj_mode >= 6)
+  if (s->reproj_mode == 0 || s->reproj_mode == 6 || s->reproj_mode == 7)
     slab = (float*)s->pa_slab.ensure(
         ctx, slab_elems(g.bx, g.by, s->pa_tb, s->reproj_mode) * sizeof(float));
   // The sweep visits z0+1, z0-1, z0+2, z0-2, ...  A step takes the pair
@@ -665,7 +665,7 @@ size_t series_hbm_bytes(int nx, int ny, int nz, const mf_options* o) {
   const CorrGeom gp = CorrGeom::make(nx, ny, pb);
   size_t pa = (size_t)4 * nz * gp.by * gp.bx +
               (size_t)4 * 6 * gp.by * pad_x(gp.bx);
-  if (o->reproj_mode == 0 || o->reproj_mode >= 6)
+  if (o->reproj_mode == 0 || o->reproj_mode == 6 || o->reproj_mode == 7)
     pa += 4 * slab_elems(gp.bx, gp.by, o->align_z / pb / 2 * 2, o->reproj_mode);
   // Common lines (lines and spectra of every image and candidate, the
   // transposed stack), offset scan at bin 4, final correction, small tables.

@@ -18,7 +18,7 @@ def _same_params(a, b):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [1, 6, 0])
+@pytest.mark.parametrize("mode", [8, 1, 6, 0])
 def test_gpu_graph_run_matches_eager(ctx, mf, cfg1, mode):
     """mf_run on the 512^2 x 41 series with and without graphs (fused,
     4-row materialised and one-row materialised reprojection): identical

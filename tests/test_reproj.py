@@ -193,7 +193,7 @@ def _check_target(ctx, orc, stack, tilts, skip, target, nproj, tb, mode):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("bx,by,nz,step,tb", [
     (128, 16, 21, 3.0, 40),
     (256, 8, 41, 3.0, 100),   # small-series binned geometry, reduced rows
@@ -263,6 +263,45 @@ def test_gpu_fast_paths_bitexact_wide(ctx, orc, bx, by, tb):
                 np.testing.assert_array_equal(d_out.download(), ref)
     for d in (d_stack, d_out, d_slab):
         d.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bx,by,tb", [(1024, 6, 300), (512, 7, 150),
+                                      (1100, 5, 90), (96, 1, 30), (2048, 3, 40)])
+def test_gpu_one_division_mode(ctx, orc, bx, by, tb):
+    """Mode 8 divides each ray's test-free interval once by the shared weight
+    sum instead of every sample: at production and edge widths (four-row,
+    two-row and one-row kernels, partial last groups), with skipped samples
+    inside the test-free intervals, its reprojection agrees with the
+    oracle's FP64 evaluation to 1e-5 (relative L2), has the same valid rays,
+    and stays within 1e-5 of mode 1 ray by ray."""
+    nz = 41
+    stack, tilts = _series(bx, by, nz, 3.0, seed=5 * bx + by)
+    stack[20, by // 2, bx // 3: bx // 3 + 9] = INVALID
+    stack[22, by - 1, 7] = INVALID
+    stack[24, 0, 0] = INVALID
+    z0 = int(np.argmin(np.abs(tilts)))
+    d_stack = ctx.array(stack)
+    d_out = ctx.empty((by, bx))
+    for k in (1, 6, 12):
+        for target in (z0 + k, z0 - k):
+            skip = np.zeros(nz, np.uint8)
+            skip[target] = 1
+            ref = orc.reproject(stack, tb, tilts, skip, target, 10, acc64=True)
+            ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
+                          d_out, mode=8)
+            got = d_out.download()
+            ctx.reproject(d_stack, bx, by, nz, tb, tilts, skip, target, 10,
+                          d_out, mode=1)
+            exact = d_out.download()
+            vr, vg = ref > -1e10, got > -1e10
+            np.testing.assert_array_equal(vr, vg)
+            np.testing.assert_array_equal(exact > -1e10, vg)
+            assert rel_l2(got, ref, vr) <= 1e-5
+            err = np.abs(got[vg] - exact[vg])
+            assert np.all(err <= 1e-5 * np.maximum(np.abs(exact[vg]), 1.0))
+    d_stack.free()
+    d_out.free()
 
 
 @pytest.mark.gpu
@@ -374,7 +413,7 @@ def test_gpu_reproject_invalid_samples(ctx, orc):
     stack[4, 1, :] = INVALID
     skip = np.zeros(21, np.uint8)
     skip[15] = 1
-    for mode in (0, 1, 2, 3, 4, 5, 6, 7):
+    for mode in (0, 1, 2, 3, 4, 5, 6, 7, 8):
         _check_target(ctx, orc, stack, tilts, skip, 15, 10, 30, mode)
 
 
@@ -384,7 +423,7 @@ def test_gpu_reproject_empty_range(ctx, orc):
     skip = np.ones(11, np.uint8)
     d_stack = ctx.array(stack)
     d_out = ctx.empty((3, 64))
-    for mode in (0, 1, 2, 3, 4, 5, 6, 7):
+    for mode in (0, 1, 2, 3, 4, 5, 6, 7, 8):
         ctx.reproject(d_stack, 64, 3, 11, 20, tilts, skip, 5, 10, d_out,
                       mode=mode)
         assert np.all(d_out.download() == np.float32(INVALID))
