@@ -1028,9 +1028,13 @@ __device__ void load_sino_slots(const BpParams& p, int n, const int* y,
 // DIV1 (mode 8): in the test-free interval, where every sample's voxel has
 // the same weight sum W, the ray adds the samples' numerators and divides
 // their sum by W once, instead of dividing each sample (three instructions
-// per row and sample fewer).  The result differs from the reference's
-// per-voxel division only in rounding, well inside 1e-5 relative of the FP64
-// evaluation; skipped samples still send the ray to the exact per-row form.
+// per row and sample fewer); tested samples divide by the hardware
+// reciprocal of their weight sum (one ulp) with one multiply per row
+// instead of the correctly rounded reciprocal and quotient.  The result
+// differs from the reference's per-voxel division only in rounding, well
+// inside 1e-5 relative of the FP64 evaluation.  A skipped sample leaves a
+// non-finite ray sum (-inf, or NaN where a weight of 0 met it), which sends
+// the ray to the exact per-row form as before.
 template <int N, int R, bool DIV1>
 __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
                                                             const RpParams q,
@@ -1135,15 +1139,16 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
 #pragma unroll
           for (int r = 0; r < R; r++) acc[r] = fmaf(v4[r], wi, acc[r]);
         }
-        const float rc = rcp_rn(ws);
+        const float rc = DIV1 ? __builtin_amdgcn_rcpf(ws) : rcp_rn(ws);
         const bool use = inb && ws >= 0.001f;
         // An unused sample adds +0 (its quotient masked to zero bits; the
         // sums never hold -0), so no branch guards the quotients.
         const unsigned um = use ? ~0u : 0u;
 #pragma unroll
-        for (int r = 0; r < R; r++)
-          sum[r] += __uint_as_float(
-              __float_as_uint(div_by_rcp(acc[r], ws, rc)) & um);
+        for (int r = 0; r < R; r++) {
+          const float qv = DIV1 ? acc[r] * rc : div_by_rcp(acc[r], ws, rc);
+          sum[r] += __uint_as_float(__float_as_uint(qv) & um);
+        }
         nuse += use ? 1 : 0;
       }
     };
@@ -1193,7 +1198,6 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
 #pragma unroll
     for (int r = 0; r < R; r++) sum[r] = 0.0f;
     tested(t0, a);
-    bool seen = false;
     if (has_fast) {
       float sf[R];  // DIV1: the interval's numerators
 #pragma unroll
@@ -1227,18 +1231,16 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         }
       }
       if (DIV1) {
-        // A skipped sample (-inf) left its numerator sum -inf or NaN.
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-          seen |= !isfinite(sf[r]);
-          sum[r] += sf[r] / p.cnt_all;
-        }
+        for (int r = 0; r < R; r++) sum[r] += sf[r] / p.cnt_all;
       }
       nuse += tb - ta;
     }
     tested(b, t1);
+    bool seen = false;
 #pragma unroll
-    for (int r = 0; r < R; r++) seen |= sum[r] != sum[r];
+    for (int r = 0; r < R; r++)
+      seen |= DIV1 ? !isfinite(sum[r]) : sum[r] != sum[r];
     if (seen) {
 #pragma unroll
       for (int r = 0; r < R; r++) {
