@@ -606,7 +606,8 @@ def main():
     prof = f"traffic_{kname}_series_1024x1024x61_T300.md"
     tr = profile_value(prof, "traffic_bytes_per_launch")
     out["roofline"]["traffic"] = int(tr) if tr is not None else None
-    if fused:
+    if mode == 8:
+        # The committed counter summary is of the default mode's kernel.
         cnt = "counters_bprp_free_series_1024x1024x61_T300.md"
         frac = profile_value(cnt, "valu_issue_frac")
         clock = profile_value(cnt, "clock_ghz")
