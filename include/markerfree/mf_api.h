@@ -338,6 +338,18 @@ mf_status mf_projalign_once(mf_series* s);
  * [nz][by][bx] at OutBin 'out_bin' to the host (out may be NULL to query
  * bx, by). */
 mf_status mf_correct(mf_series* s, int out_bin, float* out, int* bx, int* by);
+/* mf_correct that returns once the correction and the download are queued
+ * on the series' stream: with 'out' in page-locked memory (mf_host_alloc)
+ * the download runs while the host goes on, for example to the next
+ * series' mf_run, and 'out' holds the aligned stack once mf_series_wait
+ * returns (other host memory is filled before the call returns).  'out'
+ * must stay allocated until then.  No counterpart in the reference (its
+ * CorrectStack writes the file synchronously). */
+mf_status mf_correct_async(mf_series* s, int out_bin, float* out, int* bx,
+                           int* by);
+/* Waits for all queued work of the series, the download of
+ * mf_correct_async included.  No counterpart in the reference. */
+mf_status mf_series_wait(mf_series* s);
 /* Process::DoIt (main/Process.cpp:34-92) minus file I/O: preprocess, coarse
  * alignment, axis and offset searches, projection matching. */
 mf_status mf_run(mf_series* s, const mf_options* o);

@@ -365,6 +365,8 @@ _sig("mf_projalign_setup", C.c_int, _vp, C.c_float, C.c_int, C.c_int, C.c_int)
 _sig("mf_projalign_iter", C.c_int, _vp, _fp)
 _sig("mf_projalign_once", C.c_int, _vp)
 _sig("mf_correct", C.c_int, _vp, C.c_int, _fp, _ip, _ip)
+_sig("mf_correct_async", C.c_int, _vp, C.c_int, _fp, _ip, _ip)
+_sig("mf_series_wait", C.c_int, _vp)
 _sig("mf_run", C.c_int, _vp, C.POINTER(Options))
 _sig("mf_pa_measurements", C.c_int, _vp, _fp, _fp)
 _sig("mf_axis_scores", C.c_int, _vp, _fp, C.c_int, _ip)
@@ -636,19 +638,29 @@ class Series:
     def projalign_once(self):
         self._check(_lib.mf_projalign_once(self.h))
 
-    def correct(self, out_bin=1, out=None):
+    def correct(self, out_bin=1, out=None, wait=True):
         """The aligned stack [nz][by][bx]; 'out', if given, receives it (for
-        example a Context.pinned_empty array reused across series)."""
+        example a Context.pinned_empty array reused across series).  With
+        wait=False the download into a page-locked 'out' is left running
+        (mf_correct_async): 'out' holds the stack after wait()."""
         bx, by = C.c_int(), C.c_int()
         self._check(_lib.mf_correct(self.h, out_bin, None, C.byref(bx),
                                     C.byref(by)))
         shape = (self.nz, by.value, bx.value)
         if out is None:
             out = np.empty(shape, np.float32)
-        if out.shape != shape:
-            raise ValueError(f"out has shape {out.shape}, expected {shape}")
-        self._check(_lib.mf_correct(self.h, out_bin, f32p(out), None, None))
+        if out.shape != shape or out.dtype != np.float32 or \
+                not out.flags.c_contiguous:
+            raise ValueError(f"out must be a contiguous float32 array of "
+                             f"shape {shape}")
+        fn = _lib.mf_correct if wait else _lib.mf_correct_async
+        self._check(fn(self.h, out_bin, f32p(out), None, None))
         return out
+
+    def wait(self):
+        """Wait for the series' queued work (an unfinished download of
+        correct(wait=False) included)."""
+        self._check(_lib.mf_series_wait(self.h))
 
     def run(self, options: Options = None, **kw):
         o = options if options is not None else default_options(**kw)

@@ -812,6 +812,15 @@ mf_status mf_correct(mf_series* s, int out_bin, float* out, int* bx, int* by) {
   return sguard(s, [&] { series_correct(s, out_bin, out, bx, by); });
 }
 
+mf_status mf_correct_async(mf_series* s, int out_bin, float* out, int* bx,
+                           int* by) {
+  return sguard(s, [&] { series_correct(s, out_bin, out, bx, by, false); });
+}
+
+mf_status mf_series_wait(mf_series* s) {
+  return sguard(s, [&] { series_wait(s); });
+}
+
 mf_status mf_run(mf_series* s, const mf_options* o) {
   return sguard(s, [&] {
     MF_REQUIRE(o, "null options");

@@ -536,7 +536,8 @@ __global__ __launch_bounds__(256) void k_offset_moments(int w, int h,
 const float* stack_transposed(mf_ctx* ctx, hipStream_t st,
                               const float* d_stack, int nx, int ny, int nz,
                               AxisWork& ws) {
-  float* t = (float*)ws.tstack.ensure(ctx, (size_t)nx * ny * nz * sizeof(float));
+  float* t = (float*)ws.tstack.ensure(ctx, (size_t)nx * ny * nz * sizeof(float),
+                                      st);
   if (ws.t_of != d_stack) {
     hipLaunchKernelGGL(k_transpose, dim3(cdiv(nx, 32), cdiv(ny, 32), nz),
                        dim3(256), 0, st, d_stack, t, nx, ny);
@@ -571,11 +572,11 @@ float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
   }
   // Workspace (the 'small' block also holds axis_search's score buffers).
   const size_t lines_f = (size_t)nz * num * pitch;
-  float* lines = (float*)ws.lines.ensure(ctx, lines_f * sizeof(float));
-  int* reg = (int*)ws.reg.ensure(ctx, (size_t)num * L * 2 * sizeof(int));
+  float* lines = (float*)ws.lines.ensure(ctx, lines_f * sizeof(float), st);
+  int* reg = (int*)ws.reg.ensure(ctx, (size_t)num * L * 2 * sizeof(int), st);
   char* small = (char*)ws.small.ensure(
       ctx, a.cs.size() * 4 + nz * sizeof(RadonImg) + (size_t)num * nc * 8 +
-               (size_t)num * nz * 4 + num * 4 + 256);
+               (size_t)num * nz * 4 + num * 4 + 256, st);
   float* d_cs = (float*)small;
   RadonImg* d_ri = (RadonImg*)(small + a.cs.size() * 4);
   MF_HIP(hipMemcpyAsync(d_cs, a.cs.data(), a.cs.size() * 4,
@@ -592,7 +593,8 @@ float* common_lines(mf_ctx* ctx, hipStream_t st, const float* d_stack, int nx,
     colike = colike || std::fabs(a.cs[2 * k]) > std::fabs(a.cs[2 * k + 1]);
   const float* tstack =
       colike ? stack_transposed(ctx, st, d_stack, nx, ny, nz, ws) : nullptr;
-  int* negf = (int*)ws.negf.ensure(ctx, (size_t)nz * kNegParts * sizeof(int));
+  int* negf = (int*)ws.negf.ensure(ctx, (size_t)nz * kNegParts * sizeof(int),
+                                   st);
   if (ws.neg_of != d_stack) {
     hipLaunchKernelGGL(k_image_negative, dim3(kNegParts, nz), dim3(256), 0, st,
                        d_stack, nx, ny, negf);
@@ -713,7 +715,7 @@ void ncc_moments(mf_ctx* ctx, hipStream_t st, const float* d_ref,
   j.img = d_img;
   const double ds = std::cos(kD2R * ref_tilt) / std::cos(kD2R * tilt);
   stretch_matrix((float)ds, 0.0f, j.m);
-  char* buf = (char*)ws.buf.ensure(ctx, sizeof(OffJob) + 6 * 8);
+  char* buf = (char*)ws.buf.ensure(ctx, sizeof(OffJob) + 6 * 8, st);
   OffJob* d_job = (OffJob*)buf;
   double* d_mom = (double*)(buf + sizeof(OffJob));
   MF_HIP(hipMemcpyAsync(d_job, &j, sizeof(OffJob), hipMemcpyHostToDevice, st));
@@ -760,7 +762,7 @@ float offset_search(mf_ctx* ctx, hipStream_t st, const float* d_corr, int w,
     for (int z = 0; z < nz; z++) ang[z] -= off;
   }
   const size_t nj = jobs.size();
-  char* buf = (char*)ws.buf.ensure(ctx, nj * sizeof(OffJob) + nj * 6 * 8);
+  char* buf = (char*)ws.buf.ensure(ctx, nj * sizeof(OffJob) + nj * 6 * 8, st);
   OffJob* d_jobs = (OffJob*)buf;
   double* d_mom = (double*)(buf + nj * sizeof(OffJob));
   MF_HIP(hipMemcpyAsync(d_jobs, jobs.data(), nj * sizeof(OffJob),

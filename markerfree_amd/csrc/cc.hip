@@ -807,7 +807,7 @@ void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
     im[2 * p + 1] = CcImg{S + p * img, pitch, B + p * img};
   }
   const size_t b0 = npair * sizeof(PairDesc), b1 = im.size() * sizeof(CcImg);
-  char* dt = (char*)tab.ensure(ctx, b0 + b1);
+  char* dt = (char*)tab.ensure(ctx, b0 + b1, st);
   MF_HIP(hipMemcpyAsync(dt, pairs.data(), b0, hipMemcpyHostToDevice, st));
   MF_HIP(hipMemcpyAsync(dt + b0, im.data(), b1, hipMemcpyHostToDevice, st));
   const PairDesc* d_pairs = (const PairDesc*)dt;
@@ -898,7 +898,8 @@ static __device__ __align__(256) unsigned char
 void peak2d(mf_ctx* ctx, hipStream_t st, const float* d_map, int w, int h,
             int pitch, int mode, DevBuf& ws, float* d_shift) {
   check_peak_args(w, h, pitch, mode);
-  Best* part = (Best*)ws.ensure(ctx, cdiv(h, kPeakRowsPerPart) * sizeof(Best));
+  Best* part = (Best*)ws.ensure(ctx, cdiv(h, kPeakRowsPerPart) * sizeof(Best),
+                                st);
   launch_peak(st, d_map, w, h, pitch, mode, part, d_shift);
 }
 

@@ -144,8 +144,7 @@ void dev_free(mf_ctx* ctx, void* p, size_t bytes, bool cache) {
       sz = it->second;
       ctx->block_size.erase(it);
     }
-    if (cache && sz >= ((size_t)1 << 20) &&
-        ctx->cached + sz <= ctx->cache_cap) {
+    if (cache && sz > 0 && ctx->cached + sz <= ctx->cache_cap) {
       ctx->free_blocks.emplace(sz, p);
       ctx->cached += sz;
       return;
@@ -162,6 +161,15 @@ void dev_flush(mf_ctx* ctx) {
     ctx->cached = 0;
   }
   for (auto& kv : blocks) (void)hipFree(kv.second);
+}
+
+bool host_pinned(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();  // ordinary memory: no error for later calls
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
 }
 
 const void* ctx_table(mf_ctx* ctx, const std::string& key,
@@ -207,8 +215,16 @@ int occupancy(mf_ctx* ctx, const void* k, int threads, size_t lds) {
 
 DevBuf::~DevBuf() { release(); }
 
-void* DevBuf::ensure(mf_ctx* ctx, size_t bytes) {
+void* DevBuf::ensure(mf_ctx* ctx, size_t bytes, hipStream_t st) {
   if (p_ && n_ >= bytes) return p_;
+  if (p_ && st) {
+    // Only this buffer's own stream is waited for: a hipFree here would wait
+    // for every stream of the device, another series' downloads included.
+    MF_HIP(hipStreamSynchronize(st));
+    dev_free(ctx_, p_, n_, true);
+    p_ = nullptr;
+    n_ = 0;
+  }
   release();
   ctx_ = ctx;
   p_ = dev_alloc(ctx, bytes);

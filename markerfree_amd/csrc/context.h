@@ -70,7 +70,10 @@ class DevBuf {
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf();
-  void* ensure(mf_ctx* ctx, size_t bytes);
+  // Grows the buffer to at least 'bytes'.  With the stream 'st' that last
+  // used it, a smaller old block waits for that stream only and goes to the
+  // context's cache; without, it is freed (hipFree waits for the device).
+  void* ensure(mf_ctx* ctx, size_t bytes, hipStream_t st = nullptr);
   // 'cache': the caller has synchronised every stream that used the buffer,
   // so its block may go to the context's cache for reuse by any stream.
   void release(bool cache = false);
@@ -84,14 +87,18 @@ class DevBuf {
 };
 
 // Device memory through the context: budget and peak accounting in
-// requested bytes, blocks in rounded sizes.  dev_free with 'cache' keeps a
-// block of 1 MiB or more (up to the context's cache_cap bytes in all) for a
-// later dev_alloc of nearly its size, instead of returning it; the caller
-// guarantees that no pending device work still uses it.  dev_flush returns
+// requested bytes, blocks in rounded sizes.  dev_free with 'cache' keeps the
+// block (up to the context's cache_cap bytes in all) for a later dev_alloc
+// of nearly its size, instead of returning it; the caller guarantees that
+// no pending device work still uses it.  dev_flush returns
 // every cached block (dev_alloc does so once when hipMalloc runs out).
 void* dev_alloc(mf_ctx* ctx, size_t bytes);
 void dev_free(mf_ctx* ctx, void* p, size_t bytes, bool cache = false);
 void dev_flush(mf_ctx* ctx);
+
+// Whether host address p lies in page-locked memory (hipHostMalloc), which
+// a device-to-host copy can fill while the host goes on.
+bool host_pinned(const void* p);
 
 // Device copy of a constant host table, uploaded once per key and owned by
 // the context (filters, tapers, trigonometric tables).  'build' runs only the

@@ -79,7 +79,9 @@ float series_projalign_iter(mf_series* s);
 void series_projalign_once(mf_series* s);
 // Final correction (Process::CorrectStack); out [nz][by][bx] host, or
 // nullptr to query the size only.
-void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by);
+void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by,
+                    bool wait = true);
+void series_wait(mf_series* s);
 void series_run(mf_series* s, const mf_options* o);
 // Releases the stage workspaces (coarse, axis, offset and projection-matching
 // buffers, the transposed stack); keeps the preprocessed stack, tilts and

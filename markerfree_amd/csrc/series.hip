@@ -83,7 +83,8 @@ const float* stack_t(mf_series* s) {
 }
 
 CorrProj* upload_corr(mf_series* s, const std::vector<CorrProj>& p) {
-  CorrProj* d = (CorrProj*)s->cproj.ensure(s->ctx, p.size() * sizeof(CorrProj));
+  CorrProj* d = (CorrProj*)s->cproj.ensure(s->ctx, p.size() * sizeof(CorrProj),
+                                           s->st);
   MF_HIP(hipMemcpyAsync(d, p.data(), p.size() * sizeof(CorrProj),
                         hipMemcpyHostToDevice, s->st));
   return d;
@@ -103,7 +104,7 @@ struct PaDev {
 PaDev pa_layout(mf_series* s) {
   const int nz = s->nz;
   const size_t bytes = 16 + (size_t)nz * (8 + 4 + 8 + 8 + 8 + 8) + 64;
-  char* p = (char*)s->pa_dev.ensure(s->ctx, bytes);
+  char* p = (char*)s->pa_dev.ensure(s->ctx, bytes, s->st);
   PaDev d;
   d.sx = (double*)p;
   d.sy = d.sx + nz;
@@ -143,7 +144,8 @@ void series_preprocess(mf_series* s) {
       printf("%4d      %4d    %8.2f  %8.2f\n", i, nz - i, mm[2 * a], mm[2 * a + 1]);
   }
   if (s->verbose) printf("StackMin:%f\n", stack_min);
-  float* stk = (float*)s->stack.ensure(s->ctx, plane * nz * sizeof(float));
+  float* stk = (float*)s->stack.ensure(s->ctx, plane * nz * sizeof(float),
+                                       s->st);
   reorder_subtract(s->ctx, s->st, (const float*)s->raw.get(), stk, plane, nz,
                    rev, stack_min < 0 ? stack_min : 0.0f);
   if (s->verbose) printf("Positivity set.\n\nLinear mass normalization...\n");
@@ -158,7 +160,11 @@ void series_preprocess(mf_series* s) {
   for (int i = 0; i < nz; i++) scale[i] = ref / (means[i] + 0.00001f);
   scale_slices(s->ctx, s->st, stk, plane, nz, s->tmp, scale.data());
   if (s->verbose) printf("Linear mass normalization: done.\n\n");
-  s->raw.release();
+  // The raw stack's block goes to the context's cache once this series'
+  // stream is done with it: freeing it would wait for every stream of the
+  // device, another series' download included.
+  MF_HIP(hipStreamSynchronize(s->st));
+  s->raw.release(true);
   s->tilts = s->file_tilts;
   if (rev) std::reverse(s->tilts.begin(), s->tilts.end());
   s->sx.assign(nz, 0.0);
@@ -189,7 +195,8 @@ float series_coarse_iter(mf_series* s) {
   const int nz = s->nz;
   const CorrGeom g = CorrGeom::make(s->nx, s->ny, s->tr_bin);
   const size_t plane = (size_t)g.bx * g.by;
-  float* corr = (float*)s->tr_corr.ensure(s->ctx, plane * nz * sizeof(float));
+  float* corr = (float*)s->tr_corr.ensure(s->ctx, plane * nz * sizeof(float),
+                                          s->st);
   CorrProj* cp = upload_corr(s, corr_params(s, true));
   correct_bin(s->ctx, s->st, (const float*)s->stack.get(), g, cp, 0, nz, true,
               corr, g.bx, plane, stack_t(s));
@@ -208,10 +215,14 @@ float series_coarse_iter(mf_series* s) {
   }
   const int np = (int)pairs.size();
   float* work = (float*)s->tr_work.ensure(
-      s->ctx, stretch_cc_work_floats(g.bx, g.by, std::max(np, 1)) * sizeof(float));
+      s->ctx,
+      stretch_cc_work_floats(g.bx, g.by, std::max(np, 1)) * sizeof(float),
+      s->st);
   double* mom = (double*)s->tr_mom.ensure(
-      s->ctx, cc_scratch_doubles(std::max(np, 1), g.by) * sizeof(double));
-  float* dsh = (float*)s->tr_shift.ensure(s->ctx, 2 * (np + 1) * sizeof(float));
+      s->ctx, cc_scratch_doubles(std::max(np, 1), g.by) * sizeof(double),
+      s->st);
+  float* dsh = (float*)s->tr_shift.ensure(s->ctx, 2 * (np + 1) * sizeof(float),
+                                          s->st);
   stretch_cc_pairs(s->ctx, s->st, corr, g.bx, g.by, pairs, 400.0f, work, mom,
                    s->tr_tab, dsh);
   std::vector<float> sh(2 * np);
@@ -294,7 +305,8 @@ float series_find_offset(mf_series* s) {
   MF_REQUIRE(s->preprocessed, "preprocess the series first");
   const CorrGeom g = CorrGeom::make(s->nx, s->ny, 4);
   const size_t plane = (size_t)g.bx * g.by;
-  float* corr = (float*)s->off_corr.ensure(s->ctx, plane * s->nz * sizeof(float));
+  float* corr = (float*)s->off_corr.ensure(
+      s->ctx, plane * s->nz * sizeof(float), s->st);
   CorrProj* cp = upload_corr(s, corr_params(s, false));
   correct_bin(s->ctx, s->st, (const float*)s->stack.get(), g, cp, 0, s->nz,
               false, corr, g.bx, plane, stack_t(s));
@@ -340,11 +352,11 @@ void series_projalign_setup(mf_series* s, float zshift, int align_z,
   s->pa_angles = s->tilts;
   const CorrGeom& g = s->pa_geom;
   const size_t plane = (size_t)g.bx * g.by, padb = pad_x(g.bx);
-  s->pa_corr.ensure(s->ctx, plane * s->nz * sizeof(float));
+  s->pa_corr.ensure(s->ctx, plane * s->nz * sizeof(float), s->st);
   // Two targets per sweep step at most: two reprojections, four work images.
-  s->pa_reproj.ensure(s->ctx, 2 * plane * sizeof(float));
-  s->pa_work.ensure(s->ctx, 4 * (size_t)g.by * padb * sizeof(float));
-  s->pa_mom.ensure(s->ctx, cc_scratch_doubles(2, g.by) * sizeof(double));
+  s->pa_reproj.ensure(s->ctx, 2 * plane * sizeof(float), s->st);
+  s->pa_work.ensure(s->ctx, 4 * (size_t)g.by * padb * sizeof(float), s->st);
+  s->pa_mom.ensure(s->ctx, cc_scratch_doubles(2, g.by) * sizeof(double), s->st);
   // New angles and thickness: a captured sweep no longer applies.
   s->pa_gen++;
   s->pa_ekey.clear();
@@ -472,7 +484,8 @@ float series_projalign_iter(mf_series* s) {
   float* slab = nullptr;
   if (s->reproj_mode == 0 || s->reproj_mode == 6 || s->reproj_mode == 7)
     slab = (float*)s->pa_slab.ensure(
-        ctx, slab_elems(g.bx, g.by, s->pa_tb, s->reproj_mode) * sizeof(float));
+        ctx, slab_elems(g.bx, g.by, s->pa_tb, s->reproj_mode) * sizeof(float),
+        s->st);
   // The sweep visits z0+1, z0-1, z0+2, z0-2, ...  A step takes the pair
   // (z0+i, z0-i) together when the second target's reference range, formed
   // exactly as it would be after the first is aligned, does not contain the
@@ -520,7 +533,7 @@ float series_projalign_iter(mf_series* s) {
     }
   CcImg* d_tab = nullptr;
   if (!tab.empty()) {
-    d_tab = (CcImg*)s->pa_tab.ensure(ctx, tab.size() * sizeof(CcImg));
+    d_tab = (CcImg*)s->pa_tab.ensure(ctx, tab.size() * sizeof(CcImg), s->st);
     MF_HIP(hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(CcImg),
                           hipMemcpyHostToDevice, s->st));
   }
@@ -629,9 +642,12 @@ void series_projalign_once(mf_series* s) {
   s->ev(MF_EV_ONCE_END, (float)calls);
 }
 
-void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by) {
+void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by,
+                    bool wait) {
   // Process::CorrectStack (Process.cpp:329-372): rotation + shift, random
-  // fill, no R-weighting, OutBin sum binning.
+  // fill, no R-weighting, OutBin sum binning.  Without 'wait' the download
+  // into page-locked 'out' is left running on the series' stream (any other
+  // host memory is copied before the call returns).
   MF_REQUIRE(s->preprocessed, "preprocess the series first");
   MF_REQUIRE(out_bin >= 1, "OutBin must be >= 1");
   const CorrGeom g = CorrGeom::make(s->nx, s->ny, out_bin);
@@ -639,14 +655,17 @@ void series_correct(mf_series* s, int out_bin, float* out, int* bx, int* by) {
   if (by) *by = g.by;
   if (!out) return;
   const size_t plane = (size_t)g.bx * g.by;
-  float* d = (float*)s->tmp.ensure(s->ctx, plane * s->nz * sizeof(float));
+  float* d = (float*)s->tmp.ensure(s->ctx, plane * s->nz * sizeof(float),
+                                   s->st);
   CorrProj* cp = upload_corr(s, corr_params(s, false));
   correct_bin(s->ctx, s->st, (const float*)s->stack.get(), g, cp, 0, s->nz,
               true, d, g.bx, plane, stack_t(s));
   MF_HIP(hipMemcpyAsync(out, d, plane * s->nz * sizeof(float),
                         hipMemcpyDeviceToHost, s->st));
-  MF_HIP(hipStreamSynchronize(s->st));
+  if (wait || !host_pinned(out)) MF_HIP(hipStreamSynchronize(s->st));
 }
+
+void series_wait(mf_series* s) { MF_HIP(hipStreamSynchronize(s->st)); }
 
 size_t series_hbm_bytes(int nx, int ny, int nz, const mf_options* o) {
   const size_t raw = (size_t)4 * nx * ny * nz;

@@ -502,7 +502,7 @@ const float* rweight_filter(mf_ctx* ctx, int n) {
 void slice_minmax(mf_ctx* ctx, hipStream_t st, const float* d, size_t plane,
                   int nz, DevBuf& tmp, float* h_out) {
   const size_t nparts = (size_t)nz * kMinMaxParts;
-  float* buf = (float*)tmp.ensure(ctx, sizeof(float) * 2 * nparts);
+  float* buf = (float*)tmp.ensure(ctx, sizeof(float) * 2 * nparts, st);
   hipLaunchKernelGGL(k_minmax, dim3(kMinMaxParts, nz), dim3(kRedThreads), 0, st,
                      d, plane, buf);
   MF_HIP(hipGetLastError());
@@ -534,7 +534,7 @@ void reorder_subtract(mf_ctx* ctx, hipStream_t st, const float* d_raw,
 void box_means(mf_ctx* ctx, hipStream_t st, const float* d, int nx, int ny,
                int nz, int x0, int y0, int w, int h, DevBuf& tmp,
                float* h_out) {
-  double* buf = (double*)tmp.ensure(ctx, sizeof(double) * 2 * nz);
+  double* buf = (double*)tmp.ensure(ctx, sizeof(double) * 2 * nz, st);
   hipLaunchKernelGGL(k_box_mean, dim3(nz), dim3(kRedThreads), 0, st, d, nx, ny,
                      x0, y0, w, h, buf);
   MF_HIP(hipGetLastError());
@@ -548,7 +548,7 @@ void box_means(mf_ctx* ctx, hipStream_t st, const float* d, int nx, int ny,
 
 void scale_slices(mf_ctx* ctx, hipStream_t st, float* d, size_t plane, int nz,
                   DevBuf& tmp, const float* h_scale) {
-  float* s = (float*)tmp.ensure(ctx, sizeof(float) * nz);
+  float* s = (float*)tmp.ensure(ctx, sizeof(float) * nz, st);
   MF_HIP(hipMemcpyAsync(s, h_scale, sizeof(float) * nz, hipMemcpyHostToDevice,
                         st));
   hipLaunchKernelGGL(k_scale, dim3(256, nz), dim3(256), 0, st, d, plane, s);

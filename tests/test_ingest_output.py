@@ -106,6 +106,44 @@ def test_gpu_final_correction_matches_oracle(ctx, mf, orc, asym, out_bin):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("out_bin", [1, 2])
+def test_gpu_async_correction_same_stack(ctx, mf, asym, out_bin):
+    """mf_correct_async: the download into page-locked memory left running
+    while another series preprocesses and corrects, then mf_series_wait,
+    gives mf_correct's stack bit for bit; the same host buffer then takes
+    the other series' stack; ordinary memory is filled before the call
+    returns."""
+    s = asym
+    nz = len(s.tilts)
+    rng = np.random.default_rng(10 + out_bin)
+    G = [mf.Series(ctx, s.stack, s.tilts) for _ in range(2)]
+    rot = np.full(nz, 84.3, np.float32)
+    for k, g in enumerate(G):
+        g.preprocess()
+        g.set_params(rng.uniform(-9, 9, nz), rng.uniform(-9, 9, nz), rot + k)
+    want = [g.correct(out_bin) for g in G]
+    pin = mf.pinned_empty(want[0].shape)
+    pin[:] = 0.0
+    got = G[0].correct(out_bin, out=pin, wait=False)
+    assert got is pin
+    H = mf.Series(ctx, s.stack, s.tilts)   # other work meanwhile
+    H.preprocess()
+    other = H.correct(out_bin)
+    G[0].wait()
+    np.testing.assert_array_equal(pin, want[0])
+    G[0].trim()
+    G[1].correct(out_bin, out=pin, wait=False)
+    G[1].wait()
+    np.testing.assert_array_equal(pin, want[1])
+    plain = np.zeros_like(want[1])
+    G[1].correct(out_bin, out=plain, wait=False)
+    np.testing.assert_array_equal(plain, want[1])
+    assert other.shape == want[0].shape
+    for g in G + [H]:
+        g.close()
+
+
 def _read_xf(path):
     rows = [list(map(float, line.split()))
             for line in path.read_text().splitlines()]
