@@ -225,7 +225,12 @@ def bench_batched(mf, synth, args):
     host memory, 'concurrency' series at a time, largest first; against the
     same series one at a time (the plain per-series form).  Raw stacks are
     resident in HBM before each timed pass."""
-    mix = generate_all(synth, mixture(synth, args.batch), "batched")
+    if args.mix == "single":
+        # The single-series workload (1024x1024x61, T 300), 'batch' series.
+        one = synth.generate(synth.PRESETS["single"])
+        mix = [(one, 300)] * args.batch
+    else:
+        mix = generate_all(synth, mixture(synth, args.batch), "batched")
     n = len(mix)
     opts = [mf.default_options(align_z=tb, nproj=10) for _, tb in mix]
     ctx = mf.Context(0)
@@ -292,10 +297,15 @@ def bench_batched(mf, synth, args):
         "metric": "tilt series aligned/hour (batched form)",
         "value": round(3600.0 * n / t_batch, 2), "unit": "series/hour",
         "n_gpus": 1, "higher_is_better": True, "dtype": "f32",
-        "data": "synthetic mixture (seeded per series)",
-        "config": {"workload": f"{n} series, sizes {{1024^2, 2048^2, "
-                               "5760x4092}, tilts U{35..61}, range "
-                               "+-U[50,60], T {150,300}, NPROJ 10, float32",
+        "data": ("synthetic (seeded 3000-blob phantom, int16 counts)"
+                 if args.mix == "single" else
+                 "synthetic mixture (seeded per series)"),
+        "config": {"workload": (
+                       f"{n} series 1024x1024x61, T 300, NPROJ 10 (the "
+                       "single-series workload)" if args.mix == "single" else
+                       f"{n} series, sizes {{1024^2, 2048^2, 5760x4092}}, "
+                       "tilts U{35..61}, range +-U[50,60], T {150,300}, "
+                       "NPROJ 10, float32"),
                    "concurrency": args.concurrency, "series_per_size": counts,
                    "sizes": [[s.stack.shape[2], s.stack.shape[1],
                               s.stack.shape[0], tb] for s, tb in mix]},
@@ -454,6 +464,10 @@ def main():
                          "batched / served: the multi-series forms")
     ap.add_argument("--batch", type=int, default=64,
                     help="batched form: series in the mixture")
+    ap.add_argument("--mix", choices=["mixture", "single"],
+                    default="mixture",
+                    help="batched form: the size mixture, or 'batch' copies "
+                         "of the single-series workload")
     ap.add_argument("--concurrency", type=int, default=3)
     ap.add_argument("--passes", choices=["both", "plain"], default="both",
                     help="batched form: 'plain' runs the one-at-a-time pass "
