@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <map>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "bprp.h"
@@ -388,7 +389,8 @@ __device__ __forceinline__ int wave_min(int v) {
 // a tau interval.  Evaluated in FP64; the caller intersects it with the
 // ray's own range.
 __device__ void fast_interval(const BpParams& p, const RpParams& q, int n,
-                              float tx, float tz, float z0, int* ta, int* tb) {
+                              float tx, float tz, float z0, int* ta, int* tb,
+                              int* ia = nullptr, int* ib = nullptr) {
   const double vx = 2.0 * p.bx, hvx = p.bx, htb = 0.5 * p.tb;
   const double s = q.s, c = q.c;
   double lo = -1e30, hi = 1e30;
@@ -407,6 +409,15 @@ __device__ void fast_interval(const BpParams& p, const RpParams& q, int n,
   };
   clip(tx, -s, 2.0, vx - 3.0);             // X inside the slab
   clip(tz, c, 2.0, (double)p.tb - 3.0);    // Z inside the slab
+  if (ia) {
+    // The samples inside the slab (same margins), whatever the projections.
+    if (!(lo <= hi)) {
+      *ia = *ib = 0;
+    } else {
+      *ia = (int)fmax(-1e9, ceil(lo - (double)z0)) + 1;
+      *ib = (int)fmin(1e9, floor(hi - (double)z0));
+    }
+  }
 #pragma unroll 1
   for (int i = 0; i < n; i++) {
     const double ci = p.c[i], si = p.s[i];
@@ -1077,8 +1088,8 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
     const float z0 = (-xp * q.s) / q.c - 0.5f * (float)q.L;
     int t0, t1;
     rp_sample_range(q, tx, tz, z0, &t0, &t1);
-    int fa, fb;
-    fast_interval(p, q, N, tx, tz, z0, &fa, &fb);
+    int fa, fb, ia, ib;
+    fast_interval(p, q, N, tx, tz, z0, &fa, &fb, &ia, &ib);
     fa = max(fa, t0);
     fb = min(fb, t1);
     // The wave's common fast interval, in scalar registers (the fast loop's
@@ -1093,8 +1104,10 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
     // Samples that take the reference's tests, on the premise that no sample
     // is skipped: then the rows share the weight sum, the validity and the
     // count.  A skipped sample (-inf) makes a counted sample's quotient NaN,
-    // which reaches the row's sum and sends the ray to 'exact'.
-    auto tested = [&](int from, int to) {
+    // which reaches the row's sum and sends the ray to 'exact'.  'inslab'
+    // (a std::bool_constant) true: the samples are known to lie inside the
+    // slab, so the slab bounds are not tested.
+    auto tested = [&](int from, int to, auto inslab) {
       // Weights and slot-row addresses in vector registers: the scalar file
       // cannot hold them next to the range masks of this loop.
       float wv[N];
@@ -1109,7 +1122,8 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         const float fz = (float)t + z0;
         const float X = fmaf(-fz, q.s, tx);
         const float Z = fmaf(fz, q.c, tz);
-        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const bool inb = decltype(inslab)::value ||
+                         (X >= 0.f && X < iex && Z >= 0.f && Z < iez);
         const float vfx = (inb ? truncf(X) : 0.f) + ox;
         const float vfz = (inb ? truncf(Z) : 0.f) + oz;
         Slot vals[N];
@@ -1197,7 +1211,14 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
     };
 #pragma unroll
     for (int r = 0; r < R; r++) sum[r] = 0.0f;
-    tested(t0, a);
+    // [t0, a): margin samples [t0, m0) with the slab bounds tested, then
+    // in-slab ones [m0, m1).  With a fast interval [a, b) lies inside the
+    // lane's in-slab interval [ia, ib), so m1 = a; without, a = b = t1 and
+    // the rest [m1, t1) is left to the second part.
+    const int m0 = min(max(ia, t0), a);
+    const int m1 = max(min(ib, a), m0);
+    tested(t0, m0, std::false_type{});
+    tested(m0, m1, std::true_type{});
     if (has_fast) {
       float sf[R];  // DIV1: the interval's numerators
 #pragma unroll
@@ -1236,7 +1257,11 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
       }
       nuse += tb - ta;
     }
-    tested(b, t1);
+    // [c0, t1): in-slab samples [c0, n1), then margin samples [n1, t1).
+    const int c0 = has_fast ? b : m1;
+    const int n1 = max(min(ib, t1), c0);
+    tested(c0, n1, std::true_type{});
+    tested(n1, t1, std::false_type{});
     bool seen = false;
 #pragma unroll
     for (int r = 0; r < R; r++)

@@ -85,27 +85,35 @@ __global__ void k_subtract(float* d, size_t n, float sub) {
   }
 }
 
-__global__ __launch_bounds__(kRedThreads) void k_box_mean(
+// Sum and count of the pixels above -1e20 in the box [x0, x0+w) x [y0, y0+h)
+// of image blockIdx.y, in kBoxParts row bands (part blockIdx.x); FP64 sums,
+// combined by the host in part order.
+constexpr int kBoxParts = 16;
+__global__ __launch_bounds__(256) void k_box_mean(
     const float* d, int nx, int ny, int x0, int y0, int w, int h,
     double* out) {
   __shared__ double sh[16];
   __shared__ int shc[16];
-  const float* p = d + (size_t)blockIdx.x * nx * ny;
+  const float* p = d + (size_t)blockIdx.y * nx * ny;
+  const int rows = (h + kBoxParts - 1) / kBoxParts;
+  const int ya = min(h, (int)blockIdx.x * rows), yb = min(h, ya + rows);
   double s = 0.0;
   int c = 0;
-  const int n = w * h;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    int y = i / w, x = i - y * w;
-    float v = p[(size_t)(y + y0) * nx + x + x0];
-    if (v <= -1e20f) continue;
-    s += v;
-    c++;
+  for (int y = ya; y < yb; y++) {
+    const float* row = p + (size_t)(y + y0) * nx + x0;
+    for (int x = threadIdx.x; x < w; x += blockDim.x) {
+      const float v = row[x];
+      if (v <= -1e20f) continue;
+      s += v;
+      c++;
+    }
   }
   s = block_reduce(s, sh, [](double a, double b) { return a + b; }, 0.0);
   c = block_reduce(c, shc, [](int a, int b) { return a + b; }, 0);
   if (threadIdx.x == 0) {
-    out[2 * blockIdx.x] = s;
-    out[2 * blockIdx.x + 1] = (double)c;
+    double* o = out + ((size_t)blockIdx.y * kBoxParts + blockIdx.x) * 2;
+    o[0] = s;
+    o[1] = (double)c;
   }
 }
 
@@ -534,16 +542,23 @@ void reorder_subtract(mf_ctx* ctx, hipStream_t st, const float* d_raw,
 void box_means(mf_ctx* ctx, hipStream_t st, const float* d, int nx, int ny,
                int nz, int x0, int y0, int w, int h, DevBuf& tmp,
                float* h_out) {
-  double* buf = (double*)tmp.ensure(ctx, sizeof(double) * 2 * nz, st);
-  hipLaunchKernelGGL(k_box_mean, dim3(nz), dim3(kRedThreads), 0, st, d, nx, ny,
-                     x0, y0, w, h, buf);
+  const size_t nb = (size_t)2 * nz * kBoxParts;
+  double* buf = (double*)tmp.ensure(ctx, sizeof(double) * nb, st);
+  hipLaunchKernelGGL(k_box_mean, dim3(kBoxParts, nz), dim3(256), 0, st, d, nx,
+                     ny, x0, y0, w, h, buf);
   MF_HIP(hipGetLastError());
-  std::vector<double> hb(2 * nz);
-  MF_HIP(hipMemcpyAsync(hb.data(), buf, sizeof(double) * 2 * nz,
+  std::vector<double> hb(nb);
+  MF_HIP(hipMemcpyAsync(hb.data(), buf, sizeof(double) * nb,
                         hipMemcpyDeviceToHost, st));
   MF_HIP(hipStreamSynchronize(st));
-  for (int z = 0; z < nz; z++)
-    h_out[z] = hb[2 * z + 1] == 0 ? 0.0f : (float)(hb[2 * z] / hb[2 * z + 1]);
+  for (int z = 0; z < nz; z++) {
+    double sum = 0.0, cnt = 0.0;
+    for (int k = 0; k < kBoxParts; k++) {
+      sum += hb[(z * kBoxParts + k) * 2];
+      cnt += hb[(z * kBoxParts + k) * 2 + 1];
+    }
+    h_out[z] = cnt == 0 ? 0.0f : (float)(sum / cnt);
+  }
 }
 
 void scale_slices(mf_ctx* ctx, hipStream_t st, float* d, size_t plane, int nz,
