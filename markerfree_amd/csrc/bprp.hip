@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_rp_slab(const RpParams q) {
     const float fz = (float)t + z0;
     const float X = fmaf(-fz, q.s, tx);
     const float Z = fmaf(fz, q.c, tz);
-    const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+    const bool inb = (X >= 0.f) & (X < iex) & (Z >= 0.f) & (Z < iez);
     const int ix = inb ? (int)X : 0, iz = inb ? (int)Z : 0;
     const float v = vol[(size_t)vx * iz + ix];
     const bool ok = inb && (v >= -1e10f);
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_fused(const BpParams p,
         const float fz = (float)t + z0;
         const float X = fmaf(-fz, q.s, tx);
         const float Z = fmaf(fz, q.c, tz);
-        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const bool inb = (X >= 0.f) & (X < iex) & (Z >= 0.f) & (Z < iez);
         const float vfx = (float)(inb ? (int)X : 0) + ox;
         const float vfz = (float)(inb ? (int)Z : 0) + oz;
         float acc = 0.f, wsum = 0.f;
@@ -657,7 +657,7 @@ __global__ __launch_bounds__(kFuThreads) void k_bprp_tab(const BpParams p,
         const float fz = (float)t + z0;
         const float X = fmaf(-fz, q.s, tx);
         const float Z = fmaf(fz, q.c, tz);
-        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const bool inb = (X >= 0.f) & (X < iex) & (Z >= 0.f) & (Z < iez);
         const float vfx = (inb ? truncf(X) : 0.f) + ox;
         float zs[ZS];
         load_z<N>(ztab, inb ? (int)Z : 0, zs);
@@ -865,7 +865,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_rows(const BpParams p,
         const float fz = (float)t + z0;
         const float X = fmaf(-fz, q.s, tx);
         const float Z = fmaf(fz, q.c, tz);
-        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const bool inb = (X >= 0.f) & (X < iex) & (Z >= 0.f) & (Z < iez);
         const float vfx = (inb ? truncf(X) : 0.f) + ox;
         float zs[ZS];
         load_z<N>(ztab, inb ? (int)Z : 0, zs);
@@ -1123,7 +1123,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         const float X = fmaf(-fz, q.s, tx);
         const float Z = fmaf(fz, q.c, tz);
         const bool inb = decltype(inslab)::value ||
-                         (X >= 0.f && X < iex && Z >= 0.f && Z < iez);
+                         ((X >= 0.f) & (X < iex) & (Z >= 0.f) & (Z < iez));
         const float vfx = (inb ? truncf(X) : 0.f) + ox;
         const float vfz = (inb ? truncf(Z) : 0.f) + oz;
         Slot vals[N];
@@ -1173,7 +1173,7 @@ __global__ __launch_bounds__(kRowsThreads) void k_bprp_free(const BpParams p,
         const float fz = (float)t + z0;
         const float X = fmaf(-fz, q.s, tx);
         const float Z = fmaf(fz, q.c, tz);
-        const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+        const bool inb = (X >= 0.f) & (X < iex) & (Z >= 0.f) & (Z < iez);
         const float vfx = (inb ? truncf(X) : 0.f) + ox;
         const float vfz = (inb ? truncf(Z) : 0.f) + oz;
         Slot vals[N];
@@ -1484,7 +1484,7 @@ __global__ __launch_bounds__(256) void k_rp_slabr(const RpParams q) {
     const float fz = (float)t + z0;
     const float X = fmaf(-fz, q.s, tx);
     const float Z = fmaf(fz, q.c, tz);
-    const bool inb = X >= 0.f && X < iex && Z >= 0.f && Z < iez;
+    const bool inb = (X >= 0.f) & (X < iex) & (Z >= 0.f) & (Z < iez);
     const int ix = inb ? (int)X : 0, iz = inb ? (int)Z : 0;
     float v[R];
     FreeSlot<R>::split(vol[(size_t)vx * iz + ix], v);
