@@ -106,17 +106,18 @@ __global__ __launch_bounds__(256) void k_stretch_pairs(
   const int pitch = pad_x(w);
   const float* img = stack + (size_t)pd.img * w * h;
   const float hw = 0.5f * (float)w, hh = 0.5f * (float)h;
-  // Grid-strided, a few pixels per thread.
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < w * h;
-       idx += gridDim.x * blockDim.x) {
-    const int y = idx / w, x = idx - y * w;
+  // Rows blockIdx.x, blockIdx.x + gridDim.x, ...; the threads stride along
+  // each row (no division per pixel).
+  for (int y = blockIdx.x; y < h; y += gridDim.x)
+  for (int x = threadIdx.x; x < w; x += blockDim.x) {
     const float X = (float)x - hw + 0.5f;
     const float Y = (float)((double)y - 0.5 * (double)h + 0.5);
     const float buf = X * pd.m[0] + Y * pd.m[1];
     const float Y2 = X * pd.m[1] + Y * pd.m[2] + hh - 0.5f;
     const float X2 = buf + hw - 0.5f;
     float v;
-    if (X2 >= 0.f && X2 < (float)(w - 1) && Y2 >= 0.f && Y2 < (float)(h - 1)) {
+    if ((X2 >= 0.f) & (X2 < (float)(w - 1)) & (Y2 >= 0.f) &
+        (Y2 < (float)(h - 1))) {
       const int ix = (int)X2, iy = (int)Y2;
       const float fx = X2 - (float)ix, fy = Y2 - (float)iy;
       const float f2 = 1.0f - fx, f3 = 1.0f - fy;
@@ -811,7 +812,7 @@ void stretch_cc_pairs(mf_ctx* ctx, hipStream_t st, const float* d_stack,
   MF_HIP(hipMemcpyAsync(dt, pairs.data(), b0, hipMemcpyHostToDevice, st));
   MF_HIP(hipMemcpyAsync(dt + b0, im.data(), b1, hipMemcpyHostToDevice, st));
   const PairDesc* d_pairs = (const PairDesc*)dt;
-  const dim3 g2(cdiv((long long)w * h, 256 * 4), npair);
+  const dim3 g2(std::max(1, std::min(h, 256)), npair);
   hipLaunchKernelGGL(k_stretch_pairs, g2, dim3(256), 0, st, d_stack, w, h,
                      d_pairs, S, 1);
   MF_HIP(hipGetLastError());
