@@ -238,16 +238,20 @@ __device__ __forceinline__ float corr_pixel(const CorrArgs& a,
 
 // Binned output pixels, grid-strided over the image (a few per thread: one
 // per thread made the launch dispatch-bound).
+constexpr int kBinChunk = 1024;  // pixels of a row per workgroup
 __global__ __launch_bounds__(256) void k_correct_bin(const CorrArgs a) {
   const int z = a.z0 + blockIdx.y;
   const CorrProj pr = a.proj[z];
   const size_t plane = (size_t)a.nx * a.ny;
   const float* raw = a.raw + z * plane;
   const bool tr = corr_transposed(a, pr);
-  const int n = a.bx * a.by;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n;
-       idx += gridDim.x * blockDim.x) {
-    const int y = idx / a.bx, x = idx - y * a.bx;
+  // Workgroup b: row b / nch, columns [c, c + kBinChunk) with c the chunk
+  // b % nch times kBinChunk (one division per workgroup, none per pixel).
+  const int nch = (a.bx + kBinChunk - 1) / kBinChunk;
+  const int y = (int)blockIdx.x / nch;
+  const int c = ((int)blockIdx.x - y * nch) * kBinChunk;
+  const int ce = min(c + kBinChunk, a.bx);
+  for (int x = c + (int)threadIdx.x; x < ce; x += blockDim.x) {
     float* o = a.out + (size_t)blockIdx.y * a.out_stride + (size_t)y * a.out_pitch + x;
     if (tr)
       *o = corr_pixel<true>(a, pr, raw, a.rawT + z * plane, x, y);
@@ -257,7 +261,6 @@ __global__ __launch_bounds__(256) void k_correct_bin(const CorrArgs a) {
 }
 
 constexpr int kRowThreads = 256;
-constexpr int kBinPixelsPerThread = 4;
 
 // R-weighting of nl padded rows per workgroup: R2C (length bx), scale by
 // 1/N and the ramp filter, C2R, unpadded write (RWeight::DoIt + Unpad).
@@ -594,9 +597,9 @@ void correct_bin(mf_ctx* ctx, hipStream_t st, const float* d_raw,
   a.out_stride = out_stride;
   a.zstep = 1;
   TimedLaunch tl(ctx->timer, "correct", st);
+  // One row chunk of up to kBinChunk pixels per workgroup (four per thread).
   hipLaunchKernelGGL(k_correct_bin,
-                     dim3(cdiv((long long)g.bx * g.by, 256 * kBinPixelsPerThread),
-                          count),
+                     dim3(g.by * cdiv(g.bx, kBinChunk), count),
                      dim3(256), 0, st, a);
   MF_HIP(hipGetLastError());
 }
